@@ -1,0 +1,557 @@
+// api.hip -- the C ABI (include/ann_gpu.h): context, potential upload, batch,
+// entry points, kernel timing.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "net_dev.h"
+
+namespace ann {
+
+static thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+Launch::Launch(ann_ctx* c, const char* n) : ctx(c), name(n) {
+    if (!ctx->prof) return;
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess) return;
+    if (hipEventCreate(&b) != hipSuccess) { hipEventDestroy(a); return; }
+    hipEventRecord(a, ctx->stream);
+    ctx->timers[name].ev.push_back({a, b});
+}
+Launch::~Launch() {
+    if (!ctx->prof) return;
+    auto& v = ctx->timers[name].ev;
+    if (!v.empty()) hipEventRecord(v.back().second, ctx->stream);
+}
+
+hipError_t struc_index(ann_ctx* ctx);
+bool fast_shape_supported(int ng, int nk);
+hipError_t train_step_impl(ann_ctx* ctx, int form, const double* Eref, const double* Fref, double wE, double wF,
+                           double gamma, double* loss, double* grad);
+
+template <typename T>
+static hipError_t dalloc(T*& p, size_t n) {
+    if (p) { hipFree(p); p = nullptr; }
+    return hipMalloc((void**)&p, sizeof(T) * std::max<size_t>(n, 1));
+}
+
+}  // namespace ann
+
+using namespace ann;
+
+#define CHECK_CTX(c) \
+    if (!(c)) return fail(ANN_ERR_ARG, "null context")
+
+extern "C" const char* ann_last_error(void) { return g_err.c_str(); }
+
+static int upload_model(ann_ctx* ctx, const ann_model_desc* d) {
+    const int S = d->nspecies;
+    if (S < 1 || S > kMaxSpecies) return fail(ANN_ERR_UNSUPPORTED, "number of species must be in [1, 8]");
+    if (!(d->rc > 0)) return fail(ANN_ERR_ARG, "invalid cutoff radius");
+    if (d->cutoff < 1 || d->cutoff > 4) return fail(ANN_ERR_ARG, "invalid cutoff name");
+    const int NP = S * (S + 1) / 2;
+    DevModel& M = ctx->hm;
+    std::memset(&M, 0, sizeof(M));
+    M.S = S;
+    M.NP = NP;
+    M.cutoff = d->cutoff;
+    M.rc = (float)d->rc;
+    M.rci = (float)(1.0 / d->rc);
+    M.pirc = (float)(3.141592653589793 / d->rc);
+    std::vector<float> rrs, reta, aeta, azeta, alam;
+    // radial blocks
+    std::vector<int> nin(S, 0);
+    for (int a = 0; a < S; ++a) {
+        int goff = 0;
+        for (int b = 0; b < S; ++b) {
+            const int t = a * S + b, n = d->rad_n[t];
+            if (n < 0 || n > kMaxFunc) return fail(ANN_ERR_UNSUPPORTED, "radial basis larger than 32 functions");
+            if (n > 0 && (d->rad_name[t] < 1 || d->rad_name[t] > 6)) return fail(ANN_ERR_ARG, "invalid radial function");
+            RadBlock& rb = M.rad[t];
+            rb.name = d->rad_name[t];
+            rb.n = n;
+            rb.goff = goff;
+            rb.par = (int)rrs.size();
+            for (int m = 0; m < n; ++m) {
+                rrs.push_back((float)d->rad_rs[d->rad_off[t] + m]);
+                reta.push_back((float)d->rad_eta[d->rad_off[t] + m]);
+            }
+            goff += n;
+        }
+        nin[a] = goff;
+    }
+    // angular blocks
+    M.all_grid = 1;
+    for (int a = 0; a < S; ++a) {
+        std::vector<double> uni;
+        for (int p = 0; p < NP; ++p) {
+            const int t = a * NP + p, n = d->ang_n[t];
+            if (n < 0 || n > kMaxFunc) return fail(ANN_ERR_UNSUPPORTED, "angular basis larger than 32 functions");
+            if (n > 0 && (d->ang_name[t] < 1 || d->ang_name[t] > 6)) return fail(ANN_ERR_ARG, "invalid angular function");
+            AngBlock& ab = M.ang[t];
+            ab.name = d->ang_name[t];
+            ab.n = n;
+            ab.goff = nin[a];
+            ab.par = (int)aeta.size();
+            const double* eta = d->ang_eta + d->ang_off[t];
+            const double* zeta = d->ang_zeta + d->ang_off[t];
+            const int* lam = d->ang_lambda + d->ang_off[t];
+            for (int m = 0; m < n; ++m) {
+                if (!(eta[m] > 0)) return fail(ANN_ERR_ARG, "angular width must be positive");
+                aeta.push_back((float)eta[m]);
+                azeta.push_back((float)zeta[m]);
+                alam.push_back((float)lam[m]);
+            }
+            nin[a] += n;
+            if (n == 0) { ab.grid = 1; continue; }
+            // grid detection: functions = {width g} x {(lambda, zeta) k}
+            std::vector<double> ge;
+            std::vector<std::pair<int, double>> ks;
+            std::vector<int> gi(n), ki(n);
+            for (int m = 0; m < n; ++m) {
+                auto it = std::find(ge.begin(), ge.end(), eta[m]);
+                gi[m] = (int)(it - ge.begin());
+                if (it == ge.end()) ge.push_back(eta[m]);
+                const std::pair<int, double> key(lam[m], zeta[m]);
+                auto kt = std::find(ks.begin(), ks.end(), key);
+                ki[m] = (int)(kt - ks.begin());
+                if (kt == ks.end()) ks.push_back(key);
+            }
+            const int ng = (int)ge.size(), nk = (int)ks.size();
+            bool grid = ab.name == 1 && ng <= kGridG && nk <= kGridK && ng * nk == n && fast_shape_supported(ng, nk);
+            if (grid) {
+                std::vector<int> seen(ng * nk, 0);
+                for (int m = 0; m < n; ++m) {
+                    if (seen[gi[m] * nk + ki[m]]++) grid = false;
+                    ab.outm[gi[m] * nk + ki[m]] = m;
+                }
+                for (int k = 0; k < nk; ++k) {
+                    ab.lamh[k] = (float)(0.5 * ks[k].first);
+                    ab.zeta[k] = (float)ks[k].second;
+                    const double z = ks[k].second;
+                    ab.zint[k] = (z == std::floor(z) && z >= 1 && z <= 64) ? (int)z : -1;
+                    if (!(z >= 1.0)) grid = false;
+                }
+                for (int g = 0; g < ng && grid; ++g) {
+                    const double ie = 1.0 / (ge[g] * ge[g]);
+                    auto it = std::find(uni.begin(), uni.end(), ie);
+                    if (it == uni.end()) {
+                        if ((int)uni.size() >= kMaxUnionEta) { grid = false; break; }
+                        uni.push_back(ie);
+                        it = uni.end() - 1;
+                    }
+                    ab.gu[g] = (int)(it - uni.begin());
+                }
+            }
+            ab.grid = grid ? 1 : 0;
+            ab.ng = ng;
+            ab.nk = nk;
+            if (!grid) M.all_grid = 0;
+        }
+        M.nunion[a] = (int)uni.size();
+        for (size_t u = 0; u < uni.size(); ++u) M.uieta2[a][u] = (float)uni[u];
+        M.nin[a] = nin[a];
+    }
+    int maxin = 1;
+    for (int a = 0; a < S; ++a) maxin = std::max(maxin, nin[a]);
+    ctx->max_nin = maxin;
+    ctx->ldg = (maxin + 3) / 4 * 4;
+    M.ldg = ctx->ldg;
+    // networks
+    std::vector<float> W, iw, ib;
+    int ptot = 0;
+    for (int a = 0; a < S; ++a) {
+        NetDesc& D = M.net[a];
+        D.nin = nin[a];
+        D.nlayer = d->nn_nlayer[a];
+        if (D.nlayer < 1 || D.nlayer > kMaxLayers) return fail(ANN_ERR_UNSUPPORTED, "networks may have 1..4 layers");
+        D.neuron = d->nn_neuron[a];
+        if (!neuron_supported(D.neuron)) return fail(ANN_ERR_UNSUPPORTED, "unsupported neuron type");
+        D.c = (float)d->nn_c[a];
+        int prev = nin[a], cnt = 0;
+        for (int l = 0; l < D.nlayer; ++l) {
+            D.nodes[l] = d->nn_nodes[d->nn_nodes_off[a] + l];
+            if (D.nodes[l] < 1 || D.nodes[l] > kMaxWidth) return fail(ANN_ERR_UNSUPPORTED, "layer width must be 1..64");
+        }
+        if (D.nodes[D.nlayer - 1] != 1) return fail(ANN_ERR_UNSUPPORTED, "network output size must be 1");
+        D.poff = ptot;
+        for (int l = 0; l < D.nlayer; ++l) { D.boff[l] = ptot + cnt; cnt += D.nodes[l]; }
+        for (int l = 0; l < D.nlayer; ++l) { D.woff[l] = ptot + cnt; cnt += D.nodes[l] * prev; prev = D.nodes[l]; }
+        D.nparam = cnt;
+        for (int q = 0; q < cnt; ++q) {
+            const double v = d->params[d->nn_par_off[a] + q];
+            ctx->params.push_back(v);
+            W.push_back((float)v);
+        }
+        ptot += cnt;
+        D.inoff = (int)iw.size();
+        for (int m = 0; m < nin[a]; ++m) {
+            iw.push_back((float)d->inpw[d->nn_inp_off[a] + m]);
+            ib.push_back((float)d->inpb[d->nn_inp_off[a] + m]);
+        }
+        D.outw = (float)d->outw[a];
+        D.outb = (float)d->outb[a];
+        D.evac = d->evac[a];
+    }
+    ctx->nparams = ptot;
+    ctx->S = S;
+    // device copies
+    ANN_HIP(hipMalloc(&ctx->dm, sizeof(DevModel)));
+    ANN_HIP(hipMemcpy(ctx->dm, &M, sizeof(DevModel), hipMemcpyHostToDevice));
+    auto up = [](float*& dst, const std::vector<float>& v) -> hipError_t {
+        hipError_t e = hipMalloc(&dst, sizeof(float) * std::max<size_t>(v.size(), 1));
+        if (e != hipSuccess || v.empty()) return e;
+        return hipMemcpy(dst, v.data(), sizeof(float) * v.size(), hipMemcpyHostToDevice);
+    };
+    ANN_HIP(up(ctx->da.rad_rs, rrs));
+    ANN_HIP(up(ctx->da.rad_eta, reta));
+    ANN_HIP(up(ctx->da.ang_eta, aeta));
+    ANN_HIP(up(ctx->da.ang_zeta, azeta));
+    ANN_HIP(up(ctx->da.ang_lam, alam));
+    ANN_HIP(up(ctx->da.W, W));
+    ANN_HIP(up(ctx->da.inpw, iw));
+    ANN_HIP(up(ctx->da.inpb, ib));
+    ANN_HIP(hipMalloc(&ctx->da.params, sizeof(double) * ptot));
+    ANN_HIP(hipMemcpy(ctx->da.params, ctx->params.data(), sizeof(double) * ptot, hipMemcpyHostToDevice));
+    ANN_HIP(hipMalloc(&ctx->da.adam_m, sizeof(double) * ptot));
+    ANN_HIP(hipMalloc(&ctx->da.adam_v, sizeof(double) * ptot));
+    ANN_HIP(hipMemset(ctx->da.adam_m, 0, sizeof(double) * ptot));
+    ANN_HIP(hipMemset(ctx->da.adam_v, 0, sizeof(double) * ptot));
+    ANN_HIP(hipMalloc(&ctx->d_flag, sizeof(int) * 4));
+    return ANN_OK;
+}
+
+extern "C" int ann_create(const ann_model_desc* desc, ann_ctx** out) {
+    if (!desc || !out) return fail(ANN_ERR_ARG, "null argument");
+    *out = nullptr;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fail(ANN_ERR_HIP, "no HIP device");
+    ann_ctx* ctx = new ann_ctx;
+    ctx->device = dev;
+    const int rc = upload_model(ctx, desc);
+    if (rc != ANN_OK) {
+        ann_destroy(ctx);
+        return rc;
+    }
+    *out = ctx;
+    return ANN_OK;
+}
+
+extern "C" int ann_destroy(ann_ctx* ctx) {
+    if (!ctx) return ANN_OK;
+    hipDeviceSynchronize();
+    void* ptrs[] = {ctx->dm, ctx->da.rad_rs, ctx->da.rad_eta, ctx->da.ang_eta, ctx->da.ang_zeta, ctx->da.ang_lam,
+                    ctx->da.W, ctx->da.inpw, ctx->da.inpb, ctx->da.params, ctx->da.adam_m, ctx->da.adam_v,
+                    ctx->d_struc_of, ctx->d_cellinfo, ctx->d_cellmat, ctx->d_bin_of, ctx->d_shift, ctx->d_bin_cnt,
+                    ctx->d_bin_start, ctx->d_bin_atoms, ctx->d_keys, ctx->d_cnt, ctx->d_flag, ctx->nl.off,
+                    ctx->nl.idx, ctx->nl.img, ctx->nl.rev, ctx->nl.geo, ctx->nl.nsp, ctx->d_G, ctx->d_dEdG,
+                    ctx->d_Eatom, ctx->d_P, ctx->d_u, ctx->d_v, ctx->d_F, ctx->d_E, ctx->d_gE, ctx->d_atomvir,
+                    ctx->d_scr, ctx->d_gpart, ctx->d_rank, ctx->d_spcount};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    for (auto& kv : ctx->timers)
+        for (auto& e : kv.second.ev) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
+    delete ctx;
+    return ANN_OK;
+}
+
+extern "C" int ann_set_stream(ann_ctx* ctx, void* s) {
+    CHECK_CTX(ctx);
+    ctx->stream = (hipStream_t)s;
+    return ANN_OK;
+}
+
+extern "C" int ann_info(ann_ctx* ctx, int* S, int* max_nin, int* ldg, int* np) {
+    CHECK_CTX(ctx);
+    if (S) *S = ctx->S;
+    if (max_nin) *max_nin = ctx->max_nin;
+    if (ldg) *ldg = ctx->ldg;
+    if (np) *np = ctx->nparams;
+    return ANN_OK;
+}
+
+extern "C" int ann_set_params(ann_ctx* ctx, const double* p, int n) {
+    CHECK_CTX(ctx);
+    if (!p || n != ctx->nparams) return fail(ANN_ERR_ARG, "parameter count mismatch");
+    ctx->params.assign(p, p + n);
+    std::vector<float> W(p, p + n);
+    ANN_HIP(hipMemcpyAsync(ctx->da.params, p, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+    ANN_HIP(hipMemcpyAsync(ctx->da.W, W.data(), sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+    ANN_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->adam_t = 0;
+    ANN_HIP(hipMemsetAsync(ctx->da.adam_m, 0, sizeof(double) * n, ctx->stream));
+    ANN_HIP(hipMemsetAsync(ctx->da.adam_v, 0, sizeof(double) * n, ctx->stream));
+    return ANN_OK;
+}
+
+extern "C" int ann_get_params(ann_ctx* ctx, double* p, int n) {
+    CHECK_CTX(ctx);
+    if (!p || n != ctx->nparams) return fail(ANN_ERR_ARG, "parameter count mismatch");
+    ANN_HIP(hipMemcpyAsync(p, ctx->da.params, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+    ANN_HIP(hipStreamSynchronize(ctx->stream));
+    return ANN_OK;
+}
+
+extern "C" int ann_set_scaling(ann_ctx* ctx, int a, const double* inpw, const double* inpb, double outw,
+                               double outb) {
+    CHECK_CTX(ctx);
+    if (a < 0 || a >= ctx->S || !inpw || !inpb) return fail(ANN_ERR_ARG, "invalid species or null scaling");
+    NetDesc& D = ctx->hm.net[a];
+    std::vector<float> w(inpw, inpw + D.nin), b(inpb, inpb + D.nin);
+    D.outw = (float)outw;
+    D.outb = (float)outb;
+    ANN_HIP(hipMemcpyAsync(ctx->da.inpw + D.inoff, w.data(), sizeof(float) * D.nin, hipMemcpyHostToDevice, ctx->stream));
+    ANN_HIP(hipMemcpyAsync(ctx->da.inpb + D.inoff, b.data(), sizeof(float) * D.nin, hipMemcpyHostToDevice, ctx->stream));
+    ANN_HIP(hipMemcpyAsync(ctx->dm, &ctx->hm, sizeof(DevModel), hipMemcpyHostToDevice, ctx->stream));
+    ANN_HIP(hipStreamSynchronize(ctx->stream));
+    return ANN_OK;
+}
+
+extern "C" int ann_set_batch(ann_ctx* ctx, const ann_batch* b) {
+    CHECK_CTX(ctx);
+    if (!b || b->nstruc < 1 || !b->atom_off || !b->d_atom_off || !b->d_pos || !b->d_cells || !b->d_species)
+        return fail(ANN_ERR_ARG, "invalid batch");
+    const int S = b->nstruc;
+    if (b->atom_off[0] != 0) return fail(ANN_ERR_ARG, "atom_off[0] must be 0");
+    for (int s = 0; s < S; ++s)
+        if (b->atom_off[s + 1] <= b->atom_off[s]) return fail(ANN_ERR_ARG, "every structure needs at least one atom");
+    const int N = b->atom_off[S];
+    ctx->nstruc = S;
+    ctx->natoms = N;
+    ctx->atom_off.assign(b->atom_off, b->atom_off + S + 1);
+    ctx->d_atom_off = b->d_atom_off;
+    ctx->d_pos = b->d_pos;
+    ctx->d_cells = b->d_cells;
+    ctx->d_species = b->d_species;
+    if (N > ctx->atom_cap) {
+        ANN_HIP(dalloc(ctx->d_struc_of, N));
+        ANN_HIP(dalloc(ctx->d_bin_of, N));
+        ANN_HIP(dalloc(ctx->d_shift, 3 * (size_t)N));
+        ANN_HIP(dalloc(ctx->d_bin_atoms, N));
+        ANN_HIP(dalloc(ctx->d_cnt, N + 1));
+        ANN_HIP(dalloc(ctx->nl.off, N + 1));
+        ctx->atom_cap = N;
+    }
+    ANN_HIP(hipMemsetAsync(ctx->d_cnt, 0, sizeof(int) * (N + 1), ctx->stream));
+    ANN_HIP(struc_index(ctx));
+    ctx->have_batch = true;
+    ctx->have_nlist = false;
+    ctx->G_valid = false;
+    ctx->rank_valid = false;
+    return ANN_OK;
+}
+
+extern "C" int ann_neighbors(ann_ctx* ctx, double rc) {
+    CHECK_CTX(ctx);
+    if (!ctx->have_batch) return fail(ANN_ERR_STATE, "no batch set");
+    const double r = rc > 0 ? rc : (double)ctx->hm.rc;
+    const double rmodel = rc > 0 ? rc : 0;
+    (void)rmodel;
+    hipError_t e = neighbors_build(ctx, rc > 0 ? rc : ctx->hm.rc);
+    if (e == hipErrorInvalidValue) return ANN_ERR_CAPACITY;
+    if (e != hipSuccess) return fail(ANN_ERR_HIP, std::string("neighbour build: ") + hipGetErrorString(e));
+    ctx->have_nlist = true;
+    ctx->G_valid = false;
+    ctx->nl_rc = r;
+    return ANN_OK;
+}
+
+extern "C" int ann_nlist_size(ann_ctx* ctx, int64_t* np, int* maxn) {
+    CHECK_CTX(ctx);
+    if (!ctx->have_nlist) return fail(ANN_ERR_STATE, "no neighbour list");
+    if (np) *np = ctx->nl.npairs;
+    if (maxn) *maxn = ctx->nl.maxn;
+    return ANN_OK;
+}
+
+namespace ann {
+__global__ void k_unpack_img(int64_t P, const uint32_t* img, int* out) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= P) return;
+    int a, b, c;
+    unpack_img(img[s], a, b, c);
+    out[3 * s] = a;
+    out[3 * s + 1] = b;
+    out[3 * s + 2] = c;
+}
+}  // namespace ann
+
+extern "C" int ann_nlist_copy(ann_ctx* ctx, int* off, int* idx, int* img, int* rev, float* geo) {
+    CHECK_CTX(ctx);
+    if (!ctx->have_nlist) return fail(ANN_ERR_STATE, "no neighbour list");
+    const int64_t P = ctx->nl.npairs;
+    hipStream_t st = ctx->stream;
+    if (off) ANN_HIP(hipMemcpyAsync(off, ctx->nl.off, sizeof(int) * (ctx->natoms + 1), hipMemcpyDeviceToDevice, st));
+    if (P > 0) {
+        if (idx) ANN_HIP(hipMemcpyAsync(idx, ctx->nl.idx, sizeof(int) * P, hipMemcpyDeviceToDevice, st));
+        if (rev) ANN_HIP(hipMemcpyAsync(rev, ctx->nl.rev, sizeof(int) * P, hipMemcpyDeviceToDevice, st));
+        if (geo) ANN_HIP(hipMemcpyAsync(geo, ctx->nl.geo, sizeof(float4) * P, hipMemcpyDeviceToDevice, st));
+        if (img) {
+            k_unpack_img<<<(unsigned)((P + 255) / 256), 256, 0, st>>>(P, ctx->nl.img, img);
+            ANN_HIP(hipGetLastError());
+        }
+    }
+    return ANN_OK;
+}
+
+static int need_nlist(ann_ctx* ctx) {
+    if (!ctx) return fail(ANN_ERR_ARG, "null context");
+    if (!ctx->have_nlist) return fail(ANN_ERR_STATE, "build the neighbour list first (ann_neighbors)");
+    return ANN_OK;
+}
+
+static int check_form(int form) {
+    if (form != ANN_FORM_PLAIN && form != ANN_FORM_FAST) return fail(ANN_ERR_ARG, "invalid form");
+    return ANN_OK;
+}
+
+#define RET_HIP(expr, what)                                                                  \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess) return fail(ANN_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+extern "C" int ann_symm_fwd(ann_ctx* ctx, int form, float* G) {
+    int rc = need_nlist(ctx); if (rc) return rc;
+    if ((rc = check_form(form))) return rc;
+    if (!G) return fail(ANN_ERR_ARG, "null output");
+    RET_HIP(symm_fwd(ctx, form, G), "symm_fwd");
+    return ANN_OK;
+}
+
+extern "C" int ann_net_fwd(ann_ctx* ctx, int form, const float* G, float* Eatom, float* dEdG) {
+    int rc = need_nlist(ctx); if (rc) return rc;
+    if ((rc = check_form(form))) return rc;
+    if (!G || !Eatom) return fail(ANN_ERR_ARG, "null argument");
+    RET_HIP(net_fwd(ctx, form, G, Eatom, dEdG), "net_fwd");
+    return ANN_OK;
+}
+
+extern "C" int ann_symm_bwd(ann_ctx* ctx, int form, const float* dEdG, float* P) {
+    int rc = need_nlist(ctx); if (rc) return rc;
+    if ((rc = check_form(form))) return rc;
+    if (!dEdG || !P) return fail(ANN_ERR_ARG, "null argument");
+    RET_HIP(symm_bwd(ctx, form, dEdG, P), "symm_bwd");
+    return ANN_OK;
+}
+
+static hipError_t ensure_work(ann_ctx* ctx) {
+    const int64_t N = ctx->natoms, P = ctx->nl.npairs;
+    if (N > ctx->work_atoms) {
+        hipError_t e;
+        if ((e = dalloc(ctx->d_G, (size_t)N * ctx->ldg)) != hipSuccess) return e;
+        if ((e = dalloc(ctx->d_dEdG, (size_t)N * ctx->ldg)) != hipSuccess) return e;
+        if ((e = dalloc(ctx->d_v, (size_t)N * ctx->ldg)) != hipSuccess) return e;
+        if ((e = dalloc(ctx->d_Eatom, N)) != hipSuccess) return e;
+        if ((e = dalloc(ctx->d_u, 3 * (size_t)N)) != hipSuccess) return e;
+        if ((e = dalloc(ctx->d_F, 3 * (size_t)N)) != hipSuccess) return e;
+        if ((e = dalloc(ctx->d_atomvir, 9 * (size_t)N)) != hipSuccess) return e;
+        ctx->work_atoms = N;
+    }
+    if (P > ctx->work_pairs || !ctx->d_P) {
+        hipError_t e;
+        if ((e = dalloc(ctx->d_P, 3 * (size_t)std::max<int64_t>(P, 1))) != hipSuccess) return e;
+        ctx->work_pairs = std::max<int64_t>(P, 1);
+    }
+    if (!ctx->d_E || ctx->nstruc > ctx->e_cap) {
+        hipError_t e;
+        if ((e = dalloc(ctx->d_E, ctx->nstruc)) != hipSuccess) return e;
+        if ((e = dalloc(ctx->d_gE, ctx->nstruc)) != hipSuccess) return e;
+        ctx->e_cap = ctx->nstruc;
+    }
+    return hipSuccess;
+}
+
+namespace ann {
+hipError_t ensure_work_buffers(ann_ctx* ctx) { return ensure_work(ctx); }
+}
+
+extern "C" int ann_force_reduce(ann_ctx* ctx, const float* P, const float* Eatom, int with_evac, double* F,
+                                double* E, double* V) {
+    int rc = need_nlist(ctx); if (rc) return rc;
+    if (!P || (E && !Eatom)) return fail(ANN_ERR_ARG, "null argument");
+    RET_HIP(ensure_work(ctx), "allocation");
+    RET_HIP(force_reduce(ctx, P, Eatom, with_evac, F, E, V), "force_reduce");
+    return ANN_OK;
+}
+
+extern "C" int ann_energy_forces(ann_ctx* ctx, int form, int with_evac, double* E, double* F, double* V) {
+    int rc = need_nlist(ctx); if (rc) return rc;
+    if ((rc = check_form(form))) return rc;
+    RET_HIP(ensure_work(ctx), "allocation");
+    RET_HIP(symm_fwd(ctx, form, ctx->d_G), "symm_fwd");
+    RET_HIP(net_fwd(ctx, form, ctx->d_G, ctx->d_Eatom, F || V ? ctx->d_dEdG : nullptr), "net_fwd");
+    if (F || V) RET_HIP(symm_bwd(ctx, form, ctx->d_dEdG, ctx->d_P), "symm_bwd");
+    if (F || V) RET_HIP(force_reduce(ctx, ctx->d_P, ctx->d_Eatom, with_evac, F, E, V), "force_reduce");
+    else if (E) RET_HIP(force_reduce(ctx, ctx->d_P, ctx->d_Eatom, with_evac, nullptr, E, nullptr), "energy");
+    return ANN_OK;
+}
+
+extern "C" int ann_symm_tangent(ann_ctx* ctx, int form, const float* u, float* v) {
+    int rc = need_nlist(ctx); if (rc) return rc;
+    if ((rc = check_form(form))) return rc;
+    if (!u || !v) return fail(ANN_ERR_ARG, "null argument");
+    RET_HIP(symm_tangent(ctx, form, u, v), "symm_tangent");
+    return ANN_OK;
+}
+
+extern "C" int ann_net_bwd2(ann_ctx* ctx, const float* G, const float* v, const double* gE, double* grad) {
+    int rc = need_nlist(ctx); if (rc) return rc;
+    if (!G || !gE || !grad) return fail(ANN_ERR_ARG, "null argument");
+    RET_HIP(net_bwd2(ctx, G, v, gE, grad), "net_bwd2");
+    return ANN_OK;
+}
+
+extern "C" int ann_train_step(ann_ctx* ctx, int form, const double* Eref, const double* Fref, double wE, double wF,
+                              double gamma, double* loss, double* grad) {
+    int rc = need_nlist(ctx); if (rc) return rc;
+    if ((rc = check_form(form))) return rc;
+    if (!Eref || !Fref) return fail(ANN_ERR_ARG, "null reference energies/forces");
+    RET_HIP(ensure_work(ctx), "allocation");
+    RET_HIP(train_step_impl(ctx, form, Eref, Fref, wE, wF, gamma, loss, grad), "train_step");
+    return ANN_OK;
+}
+
+extern "C" int ann_profile(ann_ctx* ctx, int enable) {
+    CHECK_CTX(ctx);
+    ctx->prof = enable != 0;
+    if (!enable) {
+        for (auto& kv : ctx->timers)
+            for (auto& e : kv.second.ev) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
+        ctx->timers.clear();
+    }
+    return ANN_OK;
+}
+
+extern "C" int ann_profile_get(ann_ctx* ctx, const char* name, double* total_ms, int* count) {
+    CHECK_CTX(ctx);
+    ANN_HIP(hipStreamSynchronize(ctx->stream));
+    auto it = ctx->timers.find(name ? name : "");
+    if (it == ctx->timers.end()) {
+        if (total_ms) *total_ms = 0;
+        if (count) *count = 0;
+        return ANN_OK;
+    }
+    Timer& T = it->second;
+    for (auto& e : T.ev) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess) {
+            T.total_ms += ms;
+            T.count += 1;
+        }
+        hipEventDestroy(e.first);
+        hipEventDestroy(e.second);
+    }
+    T.ev.clear();
+    if (total_ms) *total_ms = T.total_ms;
+    if (count) *count = T.count;
+    return ANN_OK;
+}

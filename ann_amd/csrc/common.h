@@ -1,0 +1,201 @@
+// common.h -- shared definitions of libann_amd (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "ann_gpu.h"
+
+namespace ann {
+
+constexpr int kWave = 64;
+constexpr int kMaxSpecies = 8;
+constexpr int kMaxNeigh = 512;     // neighbour-list capacity per atom
+constexpr int kMaxLayers = 4;      // hidden + output
+constexpr int kMaxWidth = 64;      // widest layer supported by the network kernels
+constexpr int kMaxFunc = 32;       // functions per basis block
+constexpr int kMaxUnionEta = 8;    // distinct angular widths per centre species
+constexpr int kGridG = 4;          // fast angular tile: widths per block
+constexpr int kGridK = 8;          // fast angular tile: (lambda, zeta) combinations per block
+
+// per (centre, neighbour) radial block
+struct RadBlock {
+    int name, n, goff, par;        // par: start in rad_rs / rad_eta
+};
+
+// per (centre, pair) angular block
+struct AngBlock {
+    int name, n, goff, par;        // par: start in ang_eta / ang_zeta / ang_lambda
+    int grid;                      // 1 if functions = {width g} x {(lambda, zeta) k}, GAUSS family
+    int ng, nk;
+    int gu[kGridG];                // union width index of group g
+    float lamh[kGridK];            // lambda/2 of combination k
+    float zeta[kGridK];
+    int zint[kGridK];              // integer zeta (>=0) or -1
+    int outm[kGridG * kGridK];     // function index of (g, k)
+};
+
+struct NetDesc {
+    int nin, nlayer, neuron;
+    int nodes[kMaxLayers];
+    int woff[kMaxLayers], boff[kMaxLayers];   // into the FP32 weight array
+    int poff;                                  // start of this species in the packed params
+    int nparam;
+    int inoff;                                 // into inpw / inpb
+    float c, outw, outb;
+    double evac;
+};
+
+// Everything the kernels read about the potential (device resident).
+struct DevModel {
+    int S, NP, cutoff, ldg;
+    float rc, rci, pirc;
+    int nin[kMaxSpecies];
+    int nunion[kMaxSpecies];
+    float uieta2[kMaxSpecies][kMaxUnionEta];
+    RadBlock rad[kMaxSpecies * kMaxSpecies];
+    AngBlock ang[kMaxSpecies * (kMaxSpecies * (kMaxSpecies + 1) / 2)];
+    NetDesc net[kMaxSpecies];
+    int all_grid;                               // every angular block is grid-shaped
+};
+
+// device pointers of the variable-size tables
+struct DevArrays {
+    float* rad_rs = nullptr;
+    float* rad_eta = nullptr;
+    float* ang_eta = nullptr;
+    float* ang_zeta = nullptr;
+    float* ang_lam = nullptr;
+    float* W = nullptr;          // FP32 copy of every species' parameters (packing order)
+    float* inpw = nullptr;
+    float* inpb = nullptr;
+    double* params = nullptr;    // FP64 master parameters
+    double* adam_m = nullptr;
+    double* adam_v = nullptr;
+};
+
+// neighbour list in CSR form
+struct NList {
+    int64_t npairs = 0;
+    int maxn = 0;
+    int* off = nullptr;           // [N+1]
+    int* idx = nullptr;           // [P]
+    uint32_t* img = nullptr;      // [P] packed image (3 x 10-bit, +512 bias)
+    int* rev = nullptr;           // [P]
+    float4* geo = nullptr;        // [P] (x, y, z, |r|)
+    uint8_t* nsp = nullptr;       // [P] species of the neighbour
+    int64_t cap = 0;
+};
+
+inline __host__ __device__ uint32_t pack_img(int n0, int n1, int n2) {
+    return ((uint32_t)(n0 + 512) << 20) | ((uint32_t)(n1 + 512) << 10) | (uint32_t)(n2 + 512);
+}
+inline __host__ __device__ void unpack_img(uint32_t p, int& n0, int& n1, int& n2) {
+    n0 = (int)((p >> 20) & 1023) - 512;
+    n1 = (int)((p >> 10) & 1023) - 512;
+    n2 = (int)(p & 1023) - 512;
+}
+
+struct Timer {
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    double total_ms = 0;
+    int count = 0;
+};
+
+}  // namespace ann
+
+struct ann_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // host copy of the model
+    int S = 0, ldg = 0, nparams = 0, max_nin = 0;
+    ann::DevModel hm{};
+    std::vector<double> params;
+    // device model
+    ann::DevModel* dm = nullptr;
+    ann::DevArrays da;
+    // batch
+    int nstruc = 0, natoms = 0;
+    std::vector<int> atom_off;
+    const int* d_atom_off = nullptr;
+    const double* d_pos = nullptr;
+    const double* d_cells = nullptr;
+    const int* d_species = nullptr;
+    int* d_struc_of = nullptr;       // structure index per atom
+    int struc_cap = 0, atom_cap = 0, e_cap = 0;
+    bool G_valid = false;             // G of the current batch is cached (training)
+    int* d_rank = nullptr;            // rank of each atom within its species
+    int* d_spcount = nullptr;         // atoms per species
+    bool rank_valid = false;
+    std::vector<int> hm_counts;       // atoms per species (host copy)
+    bool have_batch = false, have_nlist = false;
+    double nl_rc = 0;
+    ann::NList nl;
+    // neighbour-build scratch
+    int* d_cellinfo = nullptr;        // per structure grid description
+    double* d_cellmat = nullptr;      // per structure R^-1
+    int* d_bin_of = nullptr;          // per atom global bin
+    int* d_shift = nullptr;           // per atom integer shift [N][3]
+    int* d_bin_cnt = nullptr;
+    int* d_bin_start = nullptr;
+    int* d_bin_atoms = nullptr;
+    int bin_cap = 0;
+    uint64_t* d_keys = nullptr;       // [N][kMaxNeigh] sorted candidate keys
+    int* d_cnt = nullptr;             // [N] neighbour counts
+    int* d_flag = nullptr;            // error flags
+    int64_t keys_cap = 0;
+    // work buffers
+    float* d_G = nullptr;
+    float* d_dEdG = nullptr;
+    float* d_Eatom = nullptr;
+    float* d_P = nullptr;
+    float* d_u = nullptr;
+    float* d_v = nullptr;
+    double* d_F = nullptr;
+    double* d_E = nullptr;
+    double* d_gE = nullptr;
+    double* d_atomvir = nullptr;      // [N][9]
+    float* d_scr = nullptr;           // network second-backward per-atom vectors
+    int64_t scr_cap = 0;
+    double* d_gpart = nullptr;        // partial gradients
+    int64_t gpart_cap = 0;
+    int64_t work_atoms = 0, work_pairs = 0;
+    int adam_t = 0;
+    // profiling
+    bool prof = false;
+    std::map<std::string, ann::Timer> timers;
+};
+
+// ---- error handling ----
+namespace ann {
+void set_error(const std::string& msg);
+int fail(int code, const std::string& msg);
+}  // namespace ann
+
+#define ANN_HIP(call)                                                                    \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            return ann::fail(ANN_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// kernel launchers implemented in the .hip files (return a hipError_t)
+namespace ann {
+struct Launch {
+    ann_ctx* ctx;
+    const char* name;
+    Launch(ann_ctx* c, const char* n);
+    ~Launch();
+};
+hipError_t neighbors_build(ann_ctx* ctx, double rc);
+hipError_t symm_fwd(ann_ctx* ctx, int form, float* G);
+hipError_t symm_bwd(ann_ctx* ctx, int form, const float* dEdG, float* P);
+hipError_t symm_tangent(ann_ctx* ctx, int form, const float* u, float* v);
+hipError_t net_fwd(ann_ctx* ctx, int form, const float* G, float* Eatom, float* dEdG);
+hipError_t net_bwd2(ann_ctx* ctx, const float* G, const float* v, const double* gE, double* grad);
+hipError_t force_reduce(ann_ctx* ctx, const float* P, const float* Eatom, int with_evac, double* F,
+                        double* E, double* V);
+}  // namespace ann

@@ -1,0 +1,100 @@
+// reduce.hip -- atomic-free force assembly and FP64 per-structure sums.
+//
+// Forces: F_a = -dE/dr_a = sum_{s in N(a)} ( P[rev s] - P[s] ), with
+// P[s] = dE_i/dr_ij of slot s = (i -> j, n) and rev the reverse slot
+// (j -> i, -n).  This folds the reaction of every periodic image back onto
+// its owner, i.e. it is the exact adjoint of the reference force loops
+// (src/nnp/nnp.cpp:489-516, which apply reactions only to min() entries).
+// Each atom gathers its own sum in FP64 (no atomics, deterministic).
+// Energy per structure (NNP::energy src/nnp/nnp.cpp:436-447, vacuum energies
+// optional as in NNH::energy src/nnp/nnh.cpp:118-121) and the virial
+// W = -sum_slots P (x) r are FP64 tree reductions, one block per structure.
+// HBM-bound gather: 12 B (own P) + 12 B (reverse P) + 4 B (rev) + 16 B
+// (geometry, virial) per slot.
+#include "common.h"
+
+namespace ann {
+
+__global__ __launch_bounds__(256) void k_force_atoms(int N, const int* __restrict__ off,
+                                                     const int* __restrict__ rev, const float4* __restrict__ geo,
+                                                     const float* __restrict__ P, double* __restrict__ F,
+                                                     double* __restrict__ avir) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= N) return;
+    double fx = 0, fy = 0, fz = 0;
+    double v[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) v[q] = 0.0;
+    const int s0 = off[a], s1 = off[a + 1];
+    for (int s = s0; s < s1; ++s) {
+        const int r = rev[s];
+        const double px = P[3 * (size_t)s], py = P[3 * (size_t)s + 1], pz = P[3 * (size_t)s + 2];
+        fx += (double)P[3 * (size_t)r] - px;
+        fy += (double)P[3 * (size_t)r + 1] - py;
+        fz += (double)P[3 * (size_t)r + 2] - pz;
+        if (avir) {
+            const float4 g = geo[s];
+            const double rr[3] = {g.x, g.y, g.z}, pp[3] = {px, py, pz};
+#pragma unroll
+            for (int x = 0; x < 3; ++x)
+#pragma unroll
+                for (int y = 0; y < 3; ++y) v[3 * x + y] -= pp[x] * rr[y];
+        }
+    }
+    if (F) {
+        F[3 * (size_t)a] = fx;
+        F[3 * (size_t)a + 1] = fy;
+        F[3 * (size_t)a + 2] = fz;
+    }
+    if (avir)
+#pragma unroll
+        for (int q = 0; q < 9; ++q) avir[9 * (size_t)a + q] = v[q];
+}
+
+__global__ __launch_bounds__(256) void k_struc_sums(const int* __restrict__ aoff, const float* __restrict__ Eatom,
+                                                    const int* __restrict__ species, const DevModel* __restrict__ dm,
+                                                    int with_evac, const double* __restrict__ avir,
+                                                    double* __restrict__ E, double* __restrict__ V) {
+    __shared__ double red[256];
+    const int s = blockIdx.x, t = threadIdx.x;
+    const int a0 = aoff[s], a1 = aoff[s + 1];
+    const int nq = (V && avir) ? 10 : 1;
+    for (int q = 0; q < nq; ++q) {
+        if (q == 0 && !E) continue;
+        double acc = 0.0;
+        for (int a = a0 + t; a < a1; a += blockDim.x) {
+            if (q == 0) acc += (double)Eatom[a] + (with_evac ? dm->net[species[a]].evac : 0.0);
+            else acc += avir[9 * (size_t)a + (q - 1)];
+        }
+        red[t] = acc;
+        __syncthreads();
+        for (int w = 128; w > 0; w >>= 1) {
+            if (t < w) red[t] += red[t + w];
+            __syncthreads();
+        }
+        if (t == 0) {
+            if (q == 0) E[s] = red[0];
+            else V[9 * (size_t)s + (q - 1)] = red[0];
+        }
+        __syncthreads();
+    }
+}
+
+hipError_t force_reduce(ann_ctx* ctx, const float* P, const float* Eatom, int with_evac, double* F, double* E,
+                        double* V) {
+    const int N = ctx->natoms;
+    if (N == 0) return hipSuccess;
+    {
+        Launch L(ctx, "force_gather");
+        k_force_atoms<<<(N + 255) / 256, 256, 0, ctx->stream>>>(N, ctx->nl.off, ctx->nl.rev, ctx->nl.geo, P, F,
+                                                                 V ? ctx->d_atomvir : nullptr);
+    }
+    if (E || V) {
+        Launch L(ctx, "struc_sums");
+        k_struc_sums<<<ctx->nstruc, 256, 0, ctx->stream>>>(ctx->d_atom_off, Eatom, ctx->d_species, ctx->dm,
+                                                          with_evac, V ? ctx->d_atomvir : nullptr, E, V);
+    }
+    return hipGetLastError();
+}
+
+}  // namespace ann

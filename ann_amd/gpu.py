@@ -1,0 +1,229 @@
+"""Python binding of the C ABI (include/ann_gpu.h) with torch for device memory.
+
+The binding is thin: torch allocates HBM and provides the stream, every
+computation runs in libann_amd.so.  There is no CPU fallback; constructing an
+``Engine`` without the built library or without a GPU raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+import torch
+
+from .configs import Batch
+from .model import Model
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libann_amd.so")
+_LIB = None
+
+PLAIN, FAST = 0, 1
+P_D = C.POINTER(C.c_double)
+P_I = C.POINTER(C.c_int)
+
+ERRORS = {-1: "ANN_ERR_ARG", -2: "ANN_ERR_UNSUPPORTED", -3: "ANN_ERR_CAPACITY", -4: "ANN_ERR_HIP",
+          -5: "ANN_ERR_STATE"}
+
+
+class AnnError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class ModelDesc(C.Structure):
+    _fields_ = [("nspecies", C.c_int), ("rc", C.c_double), ("cutoff", C.c_int),
+                ("rad_name", P_I), ("rad_n", P_I), ("rad_off", P_I), ("rad_rs", P_D), ("rad_eta", P_D),
+                ("ang_name", P_I), ("ang_n", P_I), ("ang_off", P_I), ("ang_eta", P_D),
+                ("ang_zeta", P_D), ("ang_lambda", P_I),
+                ("nn_nlayer", P_I), ("nn_nodes_off", P_I), ("nn_nodes", P_I), ("nn_neuron", P_I),
+                ("nn_c", P_D), ("nn_par_off", P_I), ("params", P_D),
+                ("nn_inp_off", P_I), ("inpw", P_D), ("inpb", P_D),
+                ("outw", P_D), ("outb", P_D), ("evac", P_D)]
+
+
+class BatchDesc(C.Structure):
+    _fields_ = [("nstruc", C.c_int), ("atom_off", P_I), ("d_atom_off", C.c_void_p),
+                ("d_pos", C.c_void_p), ("d_cells", C.c_void_p), ("d_species", C.c_void_p)]
+
+
+EXPORTS = ["ann_last_error", "ann_create", "ann_destroy", "ann_set_stream", "ann_info", "ann_set_params",
+           "ann_set_scaling", "ann_set_batch", "ann_neighbors", "ann_nlist_size", "ann_nlist_copy",
+           "ann_symm_fwd", "ann_net_fwd", "ann_symm_bwd", "ann_force_reduce", "ann_energy_forces",
+           "ann_symm_tangent", "ann_net_bwd2", "ann_train_step", "ann_get_params", "ann_profile",
+           "ann_profile_get"]
+
+
+def lib():
+    """Load libann_amd.so (raises if it has not been built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python -m ann_amd.build` first")
+        L = C.CDLL(LIB_PATH)
+        L.ann_last_error.restype = C.c_char_p
+        _LIB = L
+    return _LIB
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise AnnError(rc, lib().ann_last_error().decode())
+
+
+def _ptr(t: Optional[torch.Tensor]):
+    return C.c_void_p(t.data_ptr()) if t is not None else C.c_void_p(0)
+
+
+class Engine:
+    """One context of the GPU path for a potential ``model`` on ``device``."""
+
+    def __init__(self, model: Model, device: str = "cuda:0"):
+        if not torch.cuda.is_available():
+            raise RuntimeError("ann_amd.gpu.Engine needs a GPU")
+        self.device = torch.device(device)
+        torch.cuda.set_device(self.device)
+        self.model = model
+        self.flat = model.flat()
+        f = self.flat
+        self.desc = ModelDesc(model.nspecies, float(model.rc), int(model.cutoff),
+                              *[f[k].ctypes.data_as(P_I if f[k].dtype == np.int32 else P_D) for k in (
+                                  "rad_name", "rad_n", "rad_off", "rad_rs", "rad_eta",
+                                  "ang_name", "ang_n", "ang_off", "ang_eta", "ang_zeta", "ang_lambda",
+                                  "nn_nlayer", "nn_nodes_off", "nn_nodes", "nn_neuron", "nn_c",
+                                  "nn_par_off", "params", "nn_inp_off", "inpw", "inpb", "outw", "outb",
+                                  "evac")])
+        h = C.c_void_p()
+        _check(lib().ann_create(C.byref(self.desc), C.byref(h)))
+        self.h = h
+        self.stream = torch.cuda.current_stream(self.device)
+        _check(lib().ann_set_stream(self.h, C.c_void_p(self.stream.cuda_stream)))
+        S, nin, ldg, npar = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        _check(lib().ann_info(self.h, C.byref(S), C.byref(nin), C.byref(ldg), C.byref(npar)))
+        self.ldg, self.max_nin, self.nparams = ldg.value, nin.value, npar.value
+        self.batch: Optional[Batch] = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                lib().ann_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------ set up
+    def set_params(self, p: np.ndarray) -> None:
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        _check(lib().ann_set_params(self.h, p.ctypes.data_as(P_D), len(p)))
+
+    def get_params(self) -> np.ndarray:
+        p = np.zeros(self.nparams)
+        _check(lib().ann_get_params(self.h, p.ctypes.data_as(P_D), self.nparams))
+        return p
+
+    def set_scaling(self, species: int, inpw, inpb, outw: float = 1.0, outb: float = 0.0) -> None:
+        w = np.ascontiguousarray(inpw, dtype=np.float64)
+        b = np.ascontiguousarray(inpb, dtype=np.float64)
+        _check(lib().ann_set_scaling(self.h, species, w.ctypes.data_as(P_D), b.ctypes.data_as(P_D),
+                                     C.c_double(outw), C.c_double(outb)))
+
+    def set_batch(self, batch: Batch) -> None:
+        dev = self.device
+        self.batch = batch
+        self._off_h = np.ascontiguousarray(batch.atom_off, dtype=np.int32)
+        self.d_off = torch.from_numpy(self._off_h).to(dev)
+        self.d_pos = torch.from_numpy(np.ascontiguousarray(batch.pos, dtype=np.float64)).to(dev)
+        self.d_cells = torch.from_numpy(np.ascontiguousarray(batch.cells, dtype=np.float64).reshape(-1, 9)).to(dev)
+        self.d_species = torch.from_numpy(np.ascontiguousarray(batch.species, dtype=np.int32)).to(dev)
+        self.N = batch.ntotal
+        self.S = batch.nstruc
+        self._bdesc = BatchDesc(self.S, self._off_h.ctypes.data_as(P_I), _ptr(self.d_off), _ptr(self.d_pos),
+                                _ptr(self.d_cells), _ptr(self.d_species))
+        _check(lib().ann_set_batch(self.h, C.byref(self._bdesc)))
+
+    def neighbors(self, rc: float = 0.0) -> None:
+        _check(lib().ann_neighbors(self.h, C.c_double(rc)))
+
+    def nlist_size(self):
+        P, m = C.c_int64(), C.c_int()
+        _check(lib().ann_nlist_size(self.h, C.byref(P), C.byref(m)))
+        return P.value, m.value
+
+    def nlist(self) -> dict:
+        P, _ = self.nlist_size()
+        dev = self.device
+        off = torch.empty(self.N + 1, dtype=torch.int32, device=dev)
+        idx = torch.empty(max(P, 1), dtype=torch.int32, device=dev)
+        img = torch.empty((max(P, 1), 3), dtype=torch.int32, device=dev)
+        rev = torch.empty(max(P, 1), dtype=torch.int32, device=dev)
+        geo = torch.empty((max(P, 1), 4), dtype=torch.float32, device=dev)
+        _check(lib().ann_nlist_copy(self.h, _ptr(off), _ptr(idx), _ptr(img), _ptr(rev), _ptr(geo)))
+        torch.cuda.synchronize(dev)
+        return {"off": off.cpu().numpy(), "idx": idx[:P].cpu().numpy(), "img": img[:P].cpu().numpy(),
+                "rev": rev[:P].cpu().numpy(), "geo": geo[:P].cpu().numpy()}
+
+    # ------------------------------------------------------------ forward / backward pairs
+    def symm(self, form: int = FAST) -> torch.Tensor:
+        G = torch.empty((self.N, self.ldg), dtype=torch.float32, device=self.device)
+        _check(lib().ann_symm_fwd(self.h, form, _ptr(G)))
+        return G
+
+    def net(self, G: torch.Tensor, form: int = FAST, grad: bool = True):
+        E = torch.empty(self.N, dtype=torch.float32, device=self.device)
+        dE = torch.empty((self.N, self.ldg), dtype=torch.float32, device=self.device) if grad else None
+        _check(lib().ann_net_fwd(self.h, form, _ptr(G.contiguous()), _ptr(E), _ptr(dE)))
+        return E, dE
+
+    def symm_bwd(self, dEdG: torch.Tensor, form: int = FAST) -> torch.Tensor:
+        P, _ = self.nlist_size()
+        out = torch.empty((max(P, 1), 3), dtype=torch.float32, device=self.device)
+        _check(lib().ann_symm_bwd(self.h, form, _ptr(dEdG.contiguous()), _ptr(out)))
+        return out[:P]
+
+    def force_reduce(self, P: torch.Tensor, Eatom: torch.Tensor, with_evac: bool = True):
+        F = torch.empty((self.N, 3), dtype=torch.float64, device=self.device)
+        E = torch.empty(self.S, dtype=torch.float64, device=self.device)
+        V = torch.empty((self.S, 3, 3), dtype=torch.float64, device=self.device)
+        _check(lib().ann_force_reduce(self.h, _ptr(P.contiguous()), _ptr(Eatom), int(with_evac), _ptr(F),
+                                      _ptr(E), _ptr(V)))
+        return E, F, V
+
+    def energy_forces(self, form: int = FAST, with_evac: bool = True, out=None):
+        if out is None:
+            out = (torch.empty(self.S, dtype=torch.float64, device=self.device),
+                   torch.empty((self.N, 3), dtype=torch.float64, device=self.device),
+                   torch.empty((self.S, 3, 3), dtype=torch.float64, device=self.device))
+        E, F, V = out
+        _check(lib().ann_energy_forces(self.h, form, int(with_evac), _ptr(E), _ptr(F), _ptr(V)))
+        return E, F, V
+
+    def tangent(self, u: torch.Tensor, form: int = FAST) -> torch.Tensor:
+        v = torch.empty((self.N, self.ldg), dtype=torch.float32, device=self.device)
+        _check(lib().ann_symm_tangent(self.h, form, _ptr(u.to(torch.float32).contiguous()), _ptr(v)))
+        return v
+
+    def net_bwd2(self, G: torch.Tensor, v: Optional[torch.Tensor], gE: torch.Tensor) -> torch.Tensor:
+        g = torch.empty(self.nparams, dtype=torch.float64, device=self.device)
+        _check(lib().ann_net_bwd2(self.h, _ptr(G.contiguous()), _ptr(v), _ptr(gE.to(torch.float64).contiguous()),
+                                  _ptr(g)))
+        return g
+
+    def train_step(self, Eref: torch.Tensor, Fref: torch.Tensor, wE: float = 1.0, wF: float = 0.1,
+                   gamma: float = 0.0, form: int = FAST):
+        loss = torch.empty(2, dtype=torch.float64, device=self.device)
+        g = torch.empty(self.nparams, dtype=torch.float64, device=self.device)
+        _check(lib().ann_train_step(self.h, form, _ptr(Eref.contiguous()), _ptr(Fref.contiguous()),
+                                    C.c_double(wE), C.c_double(wF), C.c_double(gamma), _ptr(loss), _ptr(g)))
+        return loss, g
+
+    # ------------------------------------------------------------ timing
+    def profile(self, enable: bool = True) -> None:
+        _check(lib().ann_profile(self.h, int(enable)))
+
+    def profile_get(self, name: str):
+        ms, n = C.c_double(), C.c_int()
+        _check(lib().ann_profile_get(self.h, name.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value

@@ -1,0 +1,167 @@
+/*
+ * ann_gpu.h -- C ABI of the MI355X-native Behler-Parrinello energy / force /
+ * weight-gradient path (libann_amd.so, gfx950).
+ *
+ * Conventions
+ *  - Every entry point returns 0 on success or a negative ANN_ERR_* code;
+ *    ann_last_error() gives the message of the last failure on the calling
+ *    thread.  No exception crosses the ABI.  The reference throws
+ *    std::invalid_argument / std::runtime_error from the same places
+ *    (src/nnp/cutoff.cpp:51-52, src/nnp/basis_radial.cpp:203, src/ml/nn.cpp:
+ *    1022-1023); the C++ shim rethrows the message as std::runtime_error.
+ *  - Array arguments named d_* are DEVICE pointers (HBM) on the context's
+ *    device; everything else is host memory.  Work is enqueued on the stream
+ *    set with ann_set_stream (default: the null stream) and is asynchronous
+ *    unless stated otherwise.
+ *  - One context per host thread; a context holds its own device buffers.
+ *  - Enumerations use the reference numbering: Cutoff::Name
+ *    (src/nnp/cutoff.hpp:22-28), BasisR::Name (src/nnp/basis_radial.hpp:27-35),
+ *    BasisA::Name (src/nnp/basis_angular.hpp:27-35), NN::Neuron
+ *    (src/ml/nn.hpp:83-107).
+ *  - Units and formulas are the reference's (radial eta in 1/A, angular eta in
+ *    A, cosine with the vectors pointing towards the centre atom).
+ */
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    ANN_OK = 0,
+    ANN_ERR_ARG = -1,          /* invalid argument or descriptor */
+    ANN_ERR_UNSUPPORTED = -2,  /* basis / neuron / network shape not supported */
+    ANN_ERR_CAPACITY = -3,     /* an atom has more neighbours than the list capacity */
+    ANN_ERR_HIP = -4,          /* HIP runtime error */
+    ANN_ERR_STATE = -5         /* call out of order (e.g. no batch / neighbour list) */
+};
+
+/* computation forms */
+enum {
+    ANN_FORM_PLAIN = 0,  /* FP32, reference-shaped loops: per-triplet function
+                            evaluation, LDS-atomic accumulation, FP32 GEMV network */
+    ANN_FORM_FAST = 1    /* species-blocked balanced triplet tiles, pair-separable
+                            exponentials, integer powers, atomic-free adjoint,
+                            network layers on BF16 MFMA with split hi+lo products */
+};
+
+/* Potential description, flat arrays (host memory, copied by ann_create).
+ * S species, NP = S(S+1)/2 neighbour pairs in TensorTri order
+ * p(j<=k) = k(k+1)/2 + j (src/mem/tensortri.hpp:196-201).  Radial tables are
+ * indexed [centre*S + neighbour], angular tables [centre*NP + p]; *_off give
+ * where each basis starts in the value arrays.  Network of species a has
+ * nn_nlayer[a] layers (hidden + output, output size 1) of sizes
+ * nn_nodes[nn_nodes_off[a] + l]; parameters in the reference packing (all
+ * biases layer by layer, then each weight matrix column-major,
+ * src/ml/nn.cpp:846-865) at params[nn_par_off[a]]; input scaling at
+ * inpw/inpb[nn_inp_off[a]]; outw/outb/evac per species.
+ * Replaces NNP::read + NNH::init_input (src/nnp/nnp.cpp:243-307,
+ * src/nnp/nnh.cpp:94-112) as the way a potential reaches the device. */
+typedef struct ann_model_desc {
+    int nspecies;
+    double rc;
+    int cutoff;
+    const int* rad_name; const int* rad_n; const int* rad_off;
+    const double* rad_rs; const double* rad_eta;
+    const int* ang_name; const int* ang_n; const int* ang_off;
+    const double* ang_eta; const double* ang_zeta; const int* ang_lambda;
+    const int* nn_nlayer; const int* nn_nodes_off; const int* nn_nodes;
+    const int* nn_neuron; const double* nn_c;
+    const int* nn_par_off; const double* params;
+    const int* nn_inp_off; const double* inpw; const double* inpb;
+    const double* outw; const double* outb; const double* evac;
+} ann_model_desc;
+
+/* A batch of periodic structures (Structure, src/struc absent; contract from
+ * its call sites src/nnp/nnp.cpp:316-427): atoms of structure s are
+ * [atom_off[s], atom_off[s+1]); positions Cartesian (any image), cells
+ * row-major 3x3 with the lattice vectors as COLUMNS. */
+typedef struct ann_batch {
+    int nstruc;
+    const int* atom_off;     /* HOST [nstruc+1] */
+    const int* d_atom_off;   /* device copy of atom_off */
+    const double* d_pos;     /* device [N][3] */
+    const double* d_cells;   /* device [nstruc][9] */
+    const int* d_species;    /* device [N], species index in [0,S) */
+} ann_batch;
+
+typedef struct ann_ctx ann_ctx;
+
+/* ---- context ---- */
+const char* ann_last_error(void);
+int ann_create(const ann_model_desc* desc, ann_ctx** out);
+int ann_destroy(ann_ctx* ctx);
+int ann_set_stream(ann_ctx* ctx, void* hip_stream);
+/* number of inputs of species a (NNH::nInput) and the G row stride */
+int ann_info(ann_ctx* ctx, int* nspecies, int* max_ninput, int* ldg, int* nparams_total);
+/* parameters of all species concatenated, reference packing (replaces
+ * NN::ANN operator<< , src/ml/nn.cpp:867-885, as used by
+ * NNPTEFR::error_cost_o1, src/nnp/nnptefr2.cpp:1293-1303) */
+int ann_set_params(ann_ctx* ctx, const double* params, int n);
+/* input / output scaling of one species (ANN::inpw/inpb/outw/outb,
+ * src/ml/nn.hpp:299-306) */
+int ann_set_scaling(ann_ctx* ctx, int species, const double* inpw, const double* inpb,
+                    double outw, double outb);
+
+/* ---- neighbour list (NeighborList::build; src/nnp/nnpce.cpp:282) ----
+ * Cell-list build with periodic image vectors: every (j, n) with
+ * 0 < |r_i - r_j - R n| < rc (rc <= 0: the model cutoff), canonical order
+ * j ascending then n lexicographic, plus the reverse-slot map
+ * rev: (i->j, n) <-> (j->i, -n).  The pointers of the batch must stay valid
+ * until the next ann_set_batch. */
+int ann_set_batch(ann_ctx* ctx, const ann_batch* batch);
+int ann_neighbors(ann_ctx* ctx, double rc);
+/* synchronous: total pairs and the largest per-atom count */
+int ann_nlist_size(ann_ctx* ctx, int64_t* npairs, int* maxn);
+/* device-to-device copies of the list (any pointer may be NULL):
+ * off [N+1], idx [P], img [P][3], rev [P], geo [P][4] = (x, y, z, |r|) of
+ * r_i - r_j - R n in FP32 */
+int ann_nlist_copy(ann_ctx* ctx, int* d_off, int* d_idx, int* d_img, int* d_rev, float* d_geo);
+
+/* ---- forward / backward pairs (shaped for a torch.autograd.Function) ---- */
+/* NNP::symm (src/nnp/nnp.cpp:333-375): G [N][ldg] FP32 */
+int ann_symm_fwd(ann_ctx* ctx, int form, float* d_G);
+/* ANN::fpbp + DODZ::grad (src/ml/nn.cpp:1367-1389, 1801-1830): atom energies
+ * (network output, no vacuum energy) and dE/dG [N][ldg] */
+int ann_net_fwd(ann_ctx* ctx, int form, const float* d_G, float* d_Eatom, float* d_dEdG);
+/* force adjoint of NNP::force (src/nnp/nnp.cpp:461-520) per slot:
+ * P[s] = dE_i / d r_ij, [P][3] FP32 */
+int ann_symm_bwd(ann_ctx* ctx, int form, const float* d_dEdG, float* d_P);
+/* F_a = sum_{s in N(a)} (P[rev s] - P[s]) (FP64 accumulation), per-structure
+ * energy E_s = sum_i Eatom_i (+ vacuum energies if with_evac, as
+ * NNP::energy), virial W = -sum P (x) r; any output may be NULL */
+int ann_force_reduce(ann_ctx* ctx, const float* d_P, const float* d_Eatom, int with_evac,
+                     double* d_F, double* d_E, double* d_virial);
+/* fused NNP::compute (src/nnp/nnp.cpp:601-665): energies, forces, virials */
+int ann_energy_forces(ann_ctx* ctx, int form, int with_evac, double* d_E, double* d_F,
+                      double* d_virial);
+
+/* ---- second backward (weight gradient of a force loss) ---- */
+/* v_i = sum_s dG_i/dr_ij . (u_i - u_j): forward-mode tangent of the basis
+ * along the per-atom field u [N][3]; v [N][ldg] */
+int ann_symm_tangent(ann_ctx* ctx, int form, const float* d_u, float* d_v);
+/* grad = sum_i d/dp [ gE[s(i)] E_i + dE_i/dG . v_i ] for every parameter p
+ * (species blocks concatenated, reference packing), FP64 [nparams_total];
+ * replaces Cost::grad + D2ODZDI::grad (src/ml/nn.cpp:1712-1776, 2012-2103) */
+int ann_net_bwd2(ann_ctx* ctx, const float* d_G, const float* d_v, const double* d_gE,
+                 double* d_grad);
+/* one training step on the current batch: loss
+ * L = (1/B) sum_s [ wE/2 ((E_s-Eref_s)/N_s)^2 + wF/(3N_s) sum |F-Fref|^2 ],
+ * gradient, then the Adam update of the FP64 master parameters (ADAM::step,
+ * src/opt/algo.cpp:280-296) when gamma > 0.  d_loss [2] = (energy, force) */
+int ann_train_step(ann_ctx* ctx, int form, const double* d_Eref, const double* d_Fref, double wE,
+                   double wF, double gamma, double* d_loss, double* d_grad);
+/* copy the FP64 master parameters (device) to host */
+int ann_get_params(ann_ctx* ctx, double* params, int n);
+
+/* ---- kernel timing with HIP events on the context stream ---- */
+int ann_profile(ann_ctx* ctx, int enable);
+/* accumulated time (ms) and launch count of the kernel family `name`;
+ * synchronises the stream */
+int ann_profile_get(ann_ctx* ctx, const char* name, double* total_ms, int* count);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,135 @@
+/*
+ * oracle.h -- FP64 CPU restatement of the AtomicNN (markdellostritto/ann)
+ * Behler-Parrinello energy / force / weight-gradient path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: it is loaded by
+ * tests/, by __graft_entry__.smoke() and by the cpu_baseline leg of bench.py,
+ * and by nothing else.  The product (ann_amd/, libann_amd.so) never links,
+ * imports or calls it.
+ *
+ * Every function restates the reference algorithm in plain C++17 with no
+ * Eigen; the reference file:line each one follows is given at its definition.
+ * Enumerations use the reference numbering so that one integer means the same
+ * thing in the reference, the oracle and the GPU library:
+ *   Cutoff::Name   src/nnp/cutoff.hpp:22-28
+ *   BasisR::Name   src/nnp/basis_radial.hpp:27-35
+ *   BasisA::Name   src/nnp/basis_angular.hpp:27-35
+ *   NN::Neuron     src/ml/nn.hpp:83-107
+ *
+ * Parity status of each function (how it is pinned) is listed in the header
+ * comment of the source file that defines it.
+ */
+#pragma once
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_CUT_NONE = 0, ORC_CUT_STEP = 1, ORC_CUT_COS = 2, ORC_CUT_TANH = 3, ORC_CUT_POLY3 = 4 };
+enum { ORC_RAD_NONE = 0, ORC_RAD_GAUSSIAN = 1, ORC_RAD_SECH = 2, ORC_RAD_LOGISTIC = 3,
+       ORC_RAD_TANH = 4, ORC_RAD_LOGCOSH = 5, ORC_RAD_LOGCOSH2 = 6 };
+enum { ORC_ANG_NONE = 0, ORC_ANG_GAUSS = 1, ORC_ANG_GAUSS2 = 2, ORC_ANG_SECH = 3,
+       ORC_ANG_STUDENT3 = 4, ORC_ANG_STUDENT4 = 5, ORC_ANG_STUDENT5 = 6 };
+enum { ORC_NEU_LINEAR = 1, ORC_NEU_SIGMOID = 2, ORC_NEU_TANH = 3, ORC_NEU_ARCTAN = 5,
+       ORC_NEU_RELU = 6, ORC_NEU_ELU = 7, ORC_NEU_SWISH = 10, ORC_NEU_SOFTPLUS = 16,
+       ORC_NEU_SQPLUS = 17 };
+
+/* Potential description.  S species; NP = S(S+1)/2 neighbour pairs stored in
+ * TensorTri order p(j<=k) = k(k+1)/2 + j (src/mem/tensortri.hpp:196-201).
+ * Radial tables are indexed [centre*S + neighbour], angular tables
+ * [centre*NP + p]; *_off give the start of each basis in the flat arrays.
+ * Network of species a: nn_nlayer[a] layers (hidden + output), sizes
+ * nn_nodes[nn_nodes_off[a] + l]; parameters in the reference packing order
+ * (all biases layer by layer, then every weight matrix column-major,
+ * src/ml/nn.cpp:846-865) starting at params[nn_par_off[a]]. */
+typedef struct orc_model {
+    int nspecies;
+    double rc;
+    int cutoff;
+    const int* rad_name; const int* rad_n; const int* rad_off;
+    const double* rad_rs; const double* rad_eta;
+    const int* ang_name; const int* ang_n; const int* ang_off;
+    const double* ang_eta; const double* ang_zeta; const int* ang_lambda;
+    const int* nn_nlayer; const int* nn_nodes_off; const int* nn_nodes;
+    const int* nn_neuron; const double* nn_c;
+    const int* nn_par_off; const double* params;
+    const int* nn_inp_off; const double* inpw; const double* inpb;
+    const double* outw; const double* outb; const double* evac;
+} orc_model;
+
+/* One periodic structure: positions [N][3] (Cartesian, not necessarily
+ * wrapped), cell [9] row-major 3x3 whose COLUMNS are the lattice vectors
+ * (src/format/xyz_struc.cpp:123-131), species index per atom. */
+typedef struct orc_struc {
+    int natoms;
+    const double* pos;
+    const double* cell;
+    const int* species;
+} orc_struc;
+
+/* ---- special functions, cutoffs, basis (per pair / per triplet) ---- */
+double orc_fmexp(double x);
+double orc_coscut(double x);
+double orc_sincut(double x);
+void   orc_cutoff(int name, double rc, double r, double* f, double* g);
+/* BasisR::symm / BasisR::force for one pair distance: G[n], dGdr[n], amp */
+void   orc_radial(int name, int cut, double rc, int n, const double* rs, const double* eta,
+                  double r, double* G, double* dGdr);
+double orc_radial_force(int name, int cut, double rc, int n, const double* rs, const double* eta,
+                        double r, const double* dEdG);
+/* BasisA::symm / symmd / force for one triplet: G[n], dGdcos[n], dGdr0[n], dGdr1[n] */
+void   orc_angular(int name, int cut, double rc, int n, const double* eta, const double* zeta,
+                   const int* lambda, double r0, double r1, double cos, double* G,
+                   double* dGdcos, double* dGdr0, double* dGdr1);
+void   orc_angular_force(int name, int cut, double rc, int n, const double* eta, const double* zeta,
+                         const int* lambda, double r0, double r1, double cos, const double* dEdG,
+                         double* phi, double* eta_out);
+
+/* ---- model layout ---- */
+int    orc_ninput(const orc_model* m, int species);
+int    orc_nparams(const orc_model* m, int species);
+
+/* ---- neighbour list ----
+ * method 0 = brute-force image enumeration, 1 = cell list.  Both produce the
+ * canonical list: entries (j, n) with 0 < |r_i - r_j - R n| < rc, sorted by j
+ * then n lexicographic.  Returns the total number of pairs. */
+int64_t orc_nlist_build(const orc_struc* s, double rc, int method, void** handle);
+void    orc_nlist_get(void* h, int* off, int* idx, int* img, double* r, double* d, int* rev, int* minimg);
+void    orc_nlist_free(void* h);
+
+/* ---- NNP path (NNP::symm, NNP::energy, NNP::force, NNP::compute) ---- */
+void   orc_symm(const orc_model* m, const orc_struc* s, void* nl, double* G, int ldg);
+void   orc_net(const orc_model* m, int natoms, const int* species, const double* G, int ldg,
+               double* Eatom, double* dEdG);
+/* mode 0 = exact adjoint (every image reaction folded back),
+ * mode 1 = reference gating (reaction only on the minimum-image entry). */
+void   orc_force(const orc_model* m, const orc_struc* s, void* nl, const double* dEdG, int ldg,
+                 int mode, double* F, double* virial, double* P);
+double orc_compute(const orc_model* m, const orc_struc* s, int mode, double* Eatom, double* F,
+                   double* virial);
+void   orc_compute_batch(const orc_model* m, int nstruc, const int* atom_off, const double* pos,
+                         const double* cells, const int* species, int nthreads,
+                         double* E, double* F, double* virial);
+void   orc_symm_tangent(const orc_model* m, const orc_struc* s, void* nl, const double* u,
+                        double* v, int ldg);
+
+/* ---- network primitives (src/ml/nn.cpp) ---- */
+void   orc_activation(int neuron, double c, double z, double* a, double* d, double* d2);
+double orc_nn_forward(const orc_model* m, int species, const double* G, double* dEdG);
+void   orc_nn_grad_energy(const orc_model* m, int species, const double* G, double dcdo,
+                          double* grad);
+void   orc_nn_grad_rop(const orc_model* m, int species, const double* G, const double* v,
+                       double gE, double* grad);
+
+/* ---- training (energy + force loss, weight gradient, Adam) ---- */
+double orc_train_loss_grad(const orc_model* m, int nstruc, const int* atom_off, const double* pos,
+                           const double* cells, const int* species, const double* Eref,
+                           const double* Fref, double wE, double wF, double* grad,
+                           double* E, double* F, double* lossparts);
+void   orc_adam_step(int n, double* p, double* mom1, double* mom2, const double* g, double gamma,
+                     double eps, int t);
+
+#ifdef __cplusplus
+}
+#endif

@@ -1,0 +1,148 @@
+"""GPU parity: the HIP path (through the C ABI) against the FP64 oracle on the
+same seeded inputs.
+
+Tolerances (stated, per quantity):
+  neighbour list (set, order, image vectors, reverse slots): bit-exact;
+  pair geometry: FP32 rounding of the FP64 vector (<= 2 ulp);
+  symmetry functions: max |dG| <= 2e-5 * max |G| per structure;
+  energies: <= 1e-6 eV/atom;
+  forces: relative RMS <= 1e-5 over the batch;
+  per-slot adjoint P: relative RMS <= 1e-5.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+FORMS = [0, 1]
+
+
+def _engine(model, batch):
+    from ann_amd.gpu import Engine
+    eng = Engine(model, "cuda:0")
+    eng.set_batch(batch)
+    eng.neighbors()
+    return eng
+
+
+def _cases():
+    from ann_amd import configs
+    return {
+        "ar256": (configs.model_ar256(), configs.struc_ar256()),
+        "water4": (configs.model_water(), configs.batch_water(4, seed=21)),
+        "uneven6": (configs.model_water(), configs.batch_uneven(6, seed=22, nmax=300)),
+        "lips3": (configs.model_lips(), configs.batch_lips(3, seed=23)),
+        "random20": (configs.model_lips(), configs.small_random(24, natoms=20, a0=9.0)),
+    }
+
+
+CASES = ["ar256", "water4", "uneven6", "lips3", "random20"]
+
+
+@pytest.fixture(scope="module")
+def cases():
+    return _cases()
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_neighbor_list_bit_exact(cases, oracle_mod, name):
+    model, batch = cases[name]
+    print(f"case {name}", flush=True)
+    eng = _engine(model, batch)
+    nl = eng.nlist()
+    orc = oracle_mod.Oracle(model)
+    off = batch.atom_off
+    base = 0
+    for k in range(batch.nstruc):
+        pos, cell, sp = batch.struc(k)
+        ref = orc.nlist(pos, cell, sp)
+        a0, a1 = off[k], off[k + 1]
+        s0, s1 = nl["off"][a0], nl["off"][a1]
+        assert np.array_equal(nl["off"][a0:a1 + 1] - s0, ref.off)
+        assert np.array_equal(nl["idx"][s0:s1] - a0, ref.idx)
+        assert np.array_equal(nl["img"][s0:s1], ref.img)
+        assert np.array_equal(nl["rev"][s0:s1] - s0, ref.rev)
+        geo = nl["geo"][s0:s1].astype(np.float64)
+        assert np.all(np.abs(geo[:, :3] - ref.r) <= 2 * np.spacing(np.abs(ref.r).astype(np.float32)) + 1e-30)
+        assert np.all(np.abs(geo[:, 3] - ref.d) <= 2 * np.spacing(ref.d.astype(np.float32)))
+        base += s1 - s0
+    assert base == len(nl["idx"])
+
+
+@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("name", CASES)
+def test_symm_parity(cases, oracle_mod, name, form):
+    model, batch = cases[name]
+    print(f"case {name} form {form}", flush=True)
+    eng = _engine(model, batch)
+    G = eng.symm(form).cpu().numpy().astype(np.float64)
+    orc = oracle_mod.Oracle(model)
+    off = batch.atom_off
+    for k in range(batch.nstruc):
+        pos, cell, sp = batch.struc(k)
+        Gr = orc.symm(pos, cell, sp)
+        Gg = G[off[k]:off[k + 1], :Gr.shape[1]]
+        err = np.abs(Gg - Gr).max() / max(np.abs(Gr).max(), 1e-30)
+        assert err <= 2e-5, (k, err)
+
+
+@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("name", CASES)
+def test_energy_force_parity(cases, oracle_mod, name, form):
+    model, batch = cases[name]
+    print(f"case {name} form {form}", flush=True)
+    eng = _engine(model, batch)
+    E, F, V = eng.energy_forces(form)
+    E, F, V = E.cpu().numpy(), F.cpu().numpy(), V.cpu().numpy()
+    orc = oracle_mod.Oracle(model)
+    Eo, Fo, Vo = orc.compute_batch(batch, nthreads=4)
+    de = np.max(np.abs(E - Eo) / batch.natoms)
+    rf = np.sqrt(((F - Fo) ** 2).sum() / (Fo ** 2).sum())
+    rv = np.abs(V - Vo).max() / max(np.abs(Vo).max(), 1e-30)
+    print(f"  dE/atom={de:.3e} force relRMS={rf:.3e} virial rel={rv:.3e}", flush=True)
+    assert de <= 1e-6
+    assert rf <= 1e-5
+    assert rv <= 1e-4
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_slot_adjoint_parity(cases, oracle_mod, form):
+    model, batch = cases["water4"]
+    eng = _engine(model, batch)
+    G = eng.symm(form)
+    _, dE = eng.net(G, form)
+    P = eng.symm_bwd(dE, form).cpu().numpy()
+    orc = oracle_mod.Oracle(model)
+    off = batch.atom_off
+    nl = eng.nlist()
+    dEc = dE.cpu().numpy().astype(np.float64)
+    for k in range(batch.nstruc):
+        pos, cell, sp = batch.struc(k)
+        _, _, Pr = orc.force(pos, cell, sp, dEc[off[k]:off[k + 1], :orc.ld])
+        s0, s1 = nl["off"][off[k]], nl["off"][off[k + 1]]
+        err = np.sqrt(((P[s0:s1] - Pr) ** 2).sum() / (Pr ** 2).sum())
+        assert err <= 1e-5, (k, err)
+
+
+def test_forms_agree_and_deterministic(cases):
+    model, batch = cases["water4"]
+    eng = _engine(model, batch)
+    E0, F0, _ = eng.energy_forces(1)
+    E1, F1, _ = eng.energy_forces(1)
+    assert np.array_equal(E0.cpu().numpy(), E1.cpu().numpy())
+    assert np.array_equal(F0.cpu().numpy(), F1.cpu().numpy())
+
+
+def test_capacity_error_is_reported():
+    from ann_amd import configs
+    from ann_amd.gpu import AnnError, Engine
+    model = configs.model_water()
+    # 600 atoms inside a sphere of radius < rc: every atom has > 512 neighbours
+    rng = np.random.default_rng(3)
+    pos = rng.uniform(0, 5.0, size=(600, 3))
+    b = configs.single(pos, np.eye(3) * 40.0, np.zeros(600, np.int32))
+    eng = Engine(model, "cuda:0")
+    eng.set_batch(b)
+    with pytest.raises(AnnError) as ei:
+        eng.neighbors()
+    assert ei.value.code == -3
