@@ -18,18 +18,44 @@ int fail(int code, const std::string& msg) {
     return code;
 }
 
+static int pool_event(ann_ctx* ctx) {
+    if (ctx->evused == (int)ctx->evpool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return -1;
+        ctx->evpool.push_back(e);
+    }
+    return ctx->evused++;
+}
+
 Launch::Launch(ann_ctx* c, const char* n) : ctx(c), name(n) {
     if (!ctx->prof) return;
-    hipEvent_t a, b;
-    if (hipEventCreate(&a) != hipSuccess) return;
-    if (hipEventCreate(&b) != hipSuccess) { hipEventDestroy(a); return; }
-    hipEventRecord(a, ctx->stream);
+    const int a = pool_event(ctx), b = pool_event(ctx);
+    if (a < 0 || b < 0) return;
+    hipEventRecord(ctx->evpool[a], ctx->stream);
     ctx->timers[name].ev.push_back({a, b});
 }
 Launch::~Launch() {
     if (!ctx->prof) return;
     auto& v = ctx->timers[name].ev;
-    if (!v.empty()) hipEventRecord(v.back().second, ctx->stream);
+    if (!v.empty()) hipEventRecord(ctx->evpool[v.back().second], ctx->stream);
+}
+
+// resolve every recorded launch into the per-name totals and recycle the events
+static int drain_timers(ann_ctx* ctx) {
+    ANN_HIP(hipStreamSynchronize(ctx->stream));
+    for (auto& kv : ctx->timers) {
+        Timer& T = kv.second;
+        for (auto& e : T.ev) {
+            float ms = 0;
+            if (hipEventElapsedTime(&ms, ctx->evpool[e.first], ctx->evpool[e.second]) == hipSuccess) {
+                T.total_ms += ms;
+                T.count += 1;
+            }
+        }
+        T.ev.clear();
+    }
+    ctx->evused = 0;
+    return ANN_OK;
 }
 
 hipError_t struc_index(ann_ctx* ctx);
@@ -254,11 +280,10 @@ extern "C" int ann_destroy(ann_ctx* ctx) {
                     ctx->d_bin_start, ctx->d_bin_atoms, ctx->d_keys, ctx->d_cnt, ctx->d_flag, ctx->nl.off,
                     ctx->nl.idx, ctx->nl.img, ctx->nl.rev, ctx->nl.geo, ctx->nl.nsp, ctx->d_G, ctx->d_dEdG,
                     ctx->d_Eatom, ctx->d_P, ctx->d_u, ctx->d_v, ctx->d_F, ctx->d_E, ctx->d_gE, ctx->d_atomvir,
-                    ctx->d_scr, ctx->d_gpart, ctx->d_rank, ctx->d_spcount};
+                    ctx->d_scr, ctx->d_gpart, ctx->d_rank, ctx->d_spcount, ctx->d_perm, ctx->d_tiles, ctx->d_spend};
     for (void* p : ptrs)
         if (p) hipFree(p);
-    for (auto& kv : ctx->timers)
-        for (auto& e : kv.second.ev) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
+    for (hipEvent_t e : ctx->evpool) hipEventDestroy(e);
     delete ctx;
     return ANN_OK;
 }
@@ -346,6 +371,7 @@ extern "C" int ann_set_batch(ann_ctx* ctx, const ann_batch* b) {
     ctx->have_nlist = false;
     ctx->G_valid = false;
     ctx->rank_valid = false;
+    ctx->perm_valid = false;
     return ANN_OK;
 }
 
@@ -522,36 +548,19 @@ extern "C" int ann_train_step(ann_ctx* ctx, int form, const double* Eref, const 
 
 extern "C" int ann_profile(ann_ctx* ctx, int enable) {
     CHECK_CTX(ctx);
+    int rc = drain_timers(ctx);
+    if (rc) return rc;
     ctx->prof = enable != 0;
-    if (!enable) {
-        for (auto& kv : ctx->timers)
-            for (auto& e : kv.second.ev) { hipEventDestroy(e.first); hipEventDestroy(e.second); }
-        ctx->timers.clear();
-    }
+    ctx->timers.clear();
     return ANN_OK;
 }
 
 extern "C" int ann_profile_get(ann_ctx* ctx, const char* name, double* total_ms, int* count) {
     CHECK_CTX(ctx);
-    ANN_HIP(hipStreamSynchronize(ctx->stream));
+    int rc = drain_timers(ctx);
+    if (rc) return rc;
     auto it = ctx->timers.find(name ? name : "");
-    if (it == ctx->timers.end()) {
-        if (total_ms) *total_ms = 0;
-        if (count) *count = 0;
-        return ANN_OK;
-    }
-    Timer& T = it->second;
-    for (auto& e : T.ev) {
-        float ms = 0;
-        if (hipEventElapsedTime(&ms, e.first, e.second) == hipSuccess) {
-            T.total_ms += ms;
-            T.count += 1;
-        }
-        hipEventDestroy(e.first);
-        hipEventDestroy(e.second);
-    }
-    T.ev.clear();
-    if (total_ms) *total_ms = T.total_ms;
-    if (count) *count = T.count;
+    if (total_ms) *total_ms = it == ctx->timers.end() ? 0.0 : it->second.total_ms;
+    if (count) *count = it == ctx->timers.end() ? 0 : it->second.count;
     return ANN_OK;
 }

@@ -100,7 +100,7 @@ inline __host__ __device__ void unpack_img(uint32_t p, int& n0, int& n1, int& n2
 }
 
 struct Timer {
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    std::vector<std::pair<int, int>> ev;   // indices into the context's event pool
     double total_ms = 0;
     int count = 0;
 };
@@ -131,6 +131,12 @@ struct ann_ctx {
     int* d_spcount = nullptr;         // atoms per species
     bool rank_valid = false;
     std::vector<int> hm_counts;       // atoms per species (host copy)
+    int* d_perm = nullptr;            // species-sorted atom order
+    int* d_tiles = nullptr;           // (species, start) per 16-atom tile
+    int* d_spend = nullptr;           // [2S]: start then end of each species block in d_perm
+    int* d_spend_end = nullptr;       // = d_spend + S
+    int ntiles = 0;
+    bool perm_valid = false;
     bool have_batch = false, have_nlist = false;
     double nl_rc = 0;
     ann::NList nl;
@@ -167,6 +173,8 @@ struct ann_ctx {
     // profiling
     bool prof = false;
     std::map<std::string, ann::Timer> timers;
+    std::vector<hipEvent_t> evpool;   // created once, reused between drains
+    int evused = 0;
 };
 
 // ---- error handling ----

@@ -33,7 +33,7 @@ __device__ __forceinline__ void act_f(int neuron, float c, float z, float& a, fl
             const float zz = c * z * 1.5707963267948966f, f = 1.0f / (1.0f + zz * zz);
             a = 0.6366197723675814f * atanf(zz);
             d = c * f;
-            if (D2) d2 = -2.0f * c * zz * f * f;
+            if (D2) d2 = -2.0f * c * zz * f * f * (c * 1.5707963267948966f);  // true d2 (see oracle)
         } break;
         case NEU_RELU:
             a = z > 0.0f ? z : 0.0f;

@@ -1,0 +1,251 @@
+// net_mfma.hip -- per-species networks as batched GEMMs on the matrix cores.
+//
+// Same mathematics as ANN::fpbp + DODZ::grad (src/ml/nn.cpp:1367-1389,
+// 1801-1830).  Atoms are grouped by species (a species-sorted permutation of
+// the batch) into tiles of 16 rows; one wavefront owns a tile and runs every
+// layer as 16 x N x K products with v_mfma_f32_16x16x32_bf16 (BF16 operands,
+// FP32 accumulation).  BF16 alone carries 8 mantissa bits, so every operand
+// is split into three BF16 terms x = x0 + x1 + x2 (x0 = bf16(x),
+// x1 = bf16(x - x0), x2 = bf16(x - x0 - x1)) and the six products with
+// i + j <= 2 are accumulated: the truncation error is ~2^-24 relative, the
+// FP32 level that the 1e-5 force bar needs.
+//   forward:  Z_l = A_{l-1} W_l^T (+ b_l), activation on the accumulator;
+//   backward: g_{L-1} = outw W_L^T . sigma'_{L-1}, g_{l-1} = (g_l W_l) . sigma'_{l-1},
+//             dE/dG = (g_0 W_0) . inpw.
+// Activations move between the accumulator layout and the operand layout
+// through LDS.  Weights are read as FP32 and split on the fly.
+//
+// MFMA 16x16x32 lane layout used here: lane l supplies row (l % 16) of A and
+// column (l % 16) of B for the k-block 8*(l / 16) .. +7; it receives D rows
+// 4*(l / 16) .. +3 of column (l % 16).
+#include "net_dev.h"
+
+namespace ann {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+hipError_t species_perm(ann_ctx* ctx);
+
+constexpr int kTile = 16;
+constexpr int kMaxK = 128;     // padded input width supported (n_in <= 128)
+
+struct MfmaArgs {
+    const DevModel* dm;
+    const float* W;
+    const float* inpw;
+    const float* inpb;
+    int ldg;
+    const int* perm;       // species-sorted atom order
+    const int2* tiles;     // (species, start in perm); count from the species end
+    const int* spend;      // end of each species block in perm
+    int ntiles;
+    const float* G;
+    float* Eatom;
+    float* dEdG;
+};
+
+__device__ __forceinline__ void split3(float x, __bf16& a, __bf16& b, __bf16& c) {
+    a = (__bf16)x;
+    const float r1 = x - (float)a;
+    b = (__bf16)r1;
+    c = (__bf16)(r1 - (float)b);
+}
+
+// acc += A (16 x K, operand layout from fa(row, k)) * B (K x 16 col tile from fb(k, col))
+template <typename FA, typename FB>
+__device__ __forceinline__ f32x4 mm_tile(int K, FA fa, FB fb, f32x4 acc) {
+    const int lane = threadIdx.x % kWave;
+    const int row = lane % 16, kb = 8 * (lane / 16);
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        bf16x8 a0, a1, a2, b0, b1, b2;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            __bf16 x0, x1, x2, y0, y1, y2;
+            split3(fa(row, k0 + kb + j), x0, x1, x2);
+            split3(fb(k0 + kb + j, row), y0, y1, y2);
+            a0[j] = x0; a1[j] = x1; a2[j] = x2;
+            b0[j] = y0; b1[j] = y1; b2[j] = y2;
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, b0, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b2, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+// one wavefront per 16-atom single-species tile
+__global__ __launch_bounds__(64) void k_net_mfma(MfmaArgs A) {
+    __shared__ float act[kTile][kMaxK + 1];     // activations / gradients, operand side
+    __shared__ float sdz[kMaxLayers - 1][kTile][kMaxWidth + 1];
+    __shared__ int atoms[kTile];
+    const int t = blockIdx.x, lane = threadIdx.x;
+    if (t >= A.ntiles) return;
+    const int2 tl = A.tiles[t];
+    const int a = tl.x;
+    const int nrow = min(kTile, A.spend[a] - tl.y);
+    const NetDesc& D = A.dm->net[a];
+    const int L = D.nlayer;
+    const float* iw = A.inpw + D.inoff;
+    const float* ib = A.inpb + D.inoff;
+    if (lane < kTile) atoms[lane] = lane < nrow ? A.perm[tl.y + lane] : -1;
+    // scaled input s = inpw (G + inpb) into LDS
+    for (int q = lane; q < kTile * kMaxK; q += kWave) {
+        const int r = q / kMaxK, k = q % kMaxK;
+        const int ai = r < nrow ? A.perm[tl.y + r] : -1;
+        act[r][k] = (ai >= 0 && k < D.nin) ? iw[k] * (A.G[(size_t)ai * A.ldg + k] + ib[k]) : 0.0f;
+    }
+    __syncthreads();
+    const int col = lane % 16, rb = 4 * (lane / 16);
+    int ni = D.nin;
+    for (int l = 0; l < L - 1; ++l) {
+        const int no = D.nodes[l];
+        const float* Wl = A.W + D.woff[l];
+        const float* bl = A.W + D.boff[l];
+        const int Kp = (ni + 31) / 32 * 32;
+        float out[kMaxWidth / 16][4];
+#pragma unroll
+        for (int ct = 0; ct < kMaxWidth / 16; ++ct) {
+            if (ct * 16 >= no) break;
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            acc = mm_tile(Kp, [&](int r, int k) { return act[r][k]; },
+                          [&](int k, int c) {
+                              const int n = ct * 16 + c;
+                              return (k < ni && n < no) ? Wl[k * no + n] : 0.0f;
+                          }, acc);
+            const int n = ct * 16 + col;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float av = 0.f, dv = 0.f, d2;
+                if (n < no) act_f<false>(D.neuron, D.c, acc[i] + bl[n], av, dv, d2);
+                out[ct][i] = av;
+                sdz[l][rb + i][n < kMaxWidth ? n : kMaxWidth] = dv;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ct = 0; ct < kMaxWidth / 16; ++ct) {
+            if (ct * 16 >= no) break;
+            const int n = ct * 16 + col;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) act[rb + i][n] = n < no ? out[ct][i] : 0.0f;
+        }
+        // zero the padding up to the next multiple of 32
+        for (int q = lane; q < kTile * 32; q += kWave) {
+            const int r = q / 32, k = no + q % 32;
+            if (k < kMaxK) act[r][k] = 0.0f;
+        }
+        __syncthreads();
+        ni = no;
+    }
+    // linear output layer (1 neuron) on the vector ALU
+    const float* WL = A.W + D.woff[L - 1];
+    const float bL = A.W[D.boff[L - 1]];
+    if (lane < nrow) {
+        float z = bL;
+        for (int m = 0; m < ni; ++m) z += WL[m] * act[lane][m];
+        A.Eatom[atoms[lane]] = D.outb + D.outw * z;
+    }
+    if (!A.dEdG) return;
+    if (L == 1) {
+        for (int q = lane; q < kTile * A.ldg; q += kWave) {
+            const int r = q / A.ldg, m = q % A.ldg;
+            if (r < nrow) A.dEdG[(size_t)atoms[r] * A.ldg + m] = m < D.nin ? D.outw * WL[m] * iw[m] : 0.0f;
+        }
+        return;
+    }
+    __syncthreads();
+    // g_{L-1} = outw W_L . sigma'_{L-1}  (operand layout, rows = atoms)
+    for (int q = lane; q < kTile * kMaxK; q += kWave) {
+        const int r = q / kMaxK, m = q % kMaxK;
+        act[r][m] = m < ni ? D.outw * WL[m] * sdz[L - 2][r][m] : 0.0f;
+    }
+    __syncthreads();
+    for (int l = L - 2; l >= 0; --l) {
+        const int no = D.nodes[l];
+        const int nprev = l == 0 ? D.nin : D.nodes[l - 1];
+        const float* Wl = A.W + D.woff[l];
+        const int Kp = (no + 31) / 32 * 32;
+        float out[kMaxK / 16][4];
+#pragma unroll
+        for (int ct = 0; ct < kMaxK / 16; ++ct) {
+            if (ct * 16 >= nprev) break;
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            acc = mm_tile(Kp, [&](int r, int k) { return act[r][k]; },
+                          [&](int k, int c) {
+                              const int m = ct * 16 + c;
+                              return (k < no && m < nprev) ? Wl[m * no + k] : 0.0f;
+                          }, acc);
+            const int m = ct * 16 + col;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float v = acc[i];
+                if (l > 0) v *= (m < nprev) ? sdz[l - 1][rb + i][m] : 0.0f;
+                out[ct][i] = v;
+            }
+        }
+        __syncthreads();
+        if (l > 0) {
+#pragma unroll
+            for (int ct = 0; ct < kMaxK / 16; ++ct) {
+                if (ct * 16 >= nprev) break;
+                const int m = ct * 16 + col;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) act[rb + i][m] = m < nprev ? out[ct][i] : 0.0f;
+            }
+            for (int q = lane; q < kTile * 32; q += kWave) {
+                const int r = q / 32, k = nprev + q % 32;
+                if (k < kMaxK) act[r][k] = 0.0f;
+            }
+        } else {
+#pragma unroll
+            for (int ct = 0; ct < kMaxK / 16; ++ct) {
+                if (ct * 16 >= A.ldg) break;
+                const int m = ct * 16 + col;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int r = rb + i;
+                    if (r < nrow && m < A.ldg)
+                        A.dEdG[(size_t)atoms[r] * A.ldg + m] = (m < D.nin && ct * 16 < nprev) ? out[ct][i] * iw[m] : 0.0f;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+hipError_t net_fwd_mfma(ann_ctx* ctx, const float* G, float* Eatom, float* dEdG) {
+    hipError_t e = species_perm(ctx);
+    if (e != hipSuccess) return e;
+    MfmaArgs A{};
+    A.dm = ctx->dm;
+    A.W = ctx->da.W;
+    A.inpw = ctx->da.inpw;
+    A.inpb = ctx->da.inpb;
+    A.ldg = ctx->ldg;
+    A.perm = ctx->d_perm;
+    A.tiles = reinterpret_cast<const int2*>(ctx->d_tiles);
+    A.spend = ctx->d_spend_end;
+    A.ntiles = ctx->ntiles;
+    A.G = G;
+    A.Eatom = Eatom;
+    A.dEdG = dEdG;
+    if (A.ntiles == 0) return hipSuccess;
+    Launch L(ctx, "net_mfma");
+    k_net_mfma<<<A.ntiles, 64, 0, ctx->stream>>>(A);
+    return hipGetLastError();
+}
+
+bool mfma_net_supported(const DevModel& M) {
+    for (int a = 0; a < M.S; ++a) {
+        if (M.net[a].nin > kMaxK) return false;
+        for (int l = 0; l < M.net[a].nlayer; ++l)
+            if (M.net[a].nodes[l] > kMaxWidth) return false;
+    }
+    return true;
+}
+
+}  // namespace ann

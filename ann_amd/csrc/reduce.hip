@@ -5,7 +5,8 @@
 // (j -> i, -n).  This folds the reaction of every periodic image back onto
 // its owner, i.e. it is the exact adjoint of the reference force loops
 // (src/nnp/nnp.cpp:489-516, which apply reactions only to min() entries).
-// Each atom gathers its own sum in FP64 (no atomics, deterministic).
+// Each atom gathers its own sum in FP64 (a wavefront per atom, lanes over
+// slots, fixed butterfly; no atomics, deterministic).
 // Energy per structure (NNP::energy src/nnp/nnp.cpp:436-447, vacuum energies
 // optional as in NNH::energy src/nnp/nnh.cpp:118-121) and the virial
 // W = -sum_slots P (x) r are FP64 tree reductions, one block per structure.
@@ -15,18 +16,26 @@
 
 namespace ann {
 
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// one wavefront per atom: lanes over its slots, fixed-order FP64 butterfly
 __global__ __launch_bounds__(256) void k_force_atoms(int N, const int* __restrict__ off,
                                                      const int* __restrict__ rev, const float4* __restrict__ geo,
                                                      const float* __restrict__ P, double* __restrict__ F,
                                                      double* __restrict__ avir) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
     if (a >= N) return;
     double fx = 0, fy = 0, fz = 0;
     double v[9];
 #pragma unroll
     for (int q = 0; q < 9; ++q) v[q] = 0.0;
     const int s0 = off[a], s1 = off[a + 1];
-    for (int s = s0; s < s1; ++s) {
+    for (int s = s0 + lane; s < s1; s += kWave) {
         const int r = rev[s];
         const double px = P[3 * (size_t)s], py = P[3 * (size_t)s + 1], pz = P[3 * (size_t)s + 2];
         fx += (double)P[3 * (size_t)r] - px;
@@ -41,14 +50,21 @@ __global__ __launch_bounds__(256) void k_force_atoms(int N, const int* __restric
                 for (int y = 0; y < 3; ++y) v[3 * x + y] -= pp[x] * rr[y];
         }
     }
-    if (F) {
+    fx = wave_sum_d(fx);
+    fy = wave_sum_d(fy);
+    fz = wave_sum_d(fz);
+    if (F && lane == 0) {
         F[3 * (size_t)a] = fx;
         F[3 * (size_t)a + 1] = fy;
         F[3 * (size_t)a + 2] = fz;
     }
-    if (avir)
+    if (avir) {
 #pragma unroll
-        for (int q = 0; q < 9; ++q) avir[9 * (size_t)a + q] = v[q];
+        for (int q = 0; q < 9; ++q) {
+            const double t = wave_sum_d(v[q]);
+            if (lane == 0) avir[9 * (size_t)a + q] = t;
+        }
+    }
 }
 
 __global__ __launch_bounds__(256) void k_struc_sums(const int* __restrict__ aoff, const float* __restrict__ Eatom,
@@ -86,7 +102,7 @@ hipError_t force_reduce(ann_ctx* ctx, const float* P, const float* Eatom, int wi
     if (N == 0) return hipSuccess;
     {
         Launch L(ctx, "force_gather");
-        k_force_atoms<<<(N + 255) / 256, 256, 0, ctx->stream>>>(N, ctx->nl.off, ctx->nl.rev, ctx->nl.geo, P, F,
+        k_force_atoms<<<(N + 3) / 4, 256, 0, ctx->stream>>>(N, ctx->nl.off, ctx->nl.rev, ctx->nl.geo, P, F,
                                                                  V ? ctx->d_atomvir : nullptr);
     }
     if (E || V) {

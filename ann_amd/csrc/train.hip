@@ -228,7 +228,7 @@ __global__ void k_take_rank(int N, const int* __restrict__ species, int s, const
     if (i == N) count[s] = pre[N];
 }
 
-static hipError_t species_ranks(ann_ctx* ctx) {
+hipError_t species_ranks(ann_ctx* ctx) {
     if (ctx->rank_valid) return hipSuccess;
     const int N = ctx->natoms;
     hipError_t e;
@@ -264,6 +264,48 @@ static hipError_t species_ranks(ann_ctx* ctx) {
 }
 
 int max_width(const DevModel& M);
+
+__global__ void k_perm(int N, const int* __restrict__ species, const int* __restrict__ rank,
+                       const int* __restrict__ base, int* __restrict__ perm) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) perm[base[species[i]] + rank[i]] = i;
+}
+
+// species-sorted permutation of the batch and its 16-atom single-species tiles
+hipError_t species_perm(ann_ctx* ctx) {
+    if (ctx->perm_valid) return hipSuccess;
+    hipError_t e = species_ranks(ctx);
+    if (e != hipSuccess) return e;
+    const int S = ctx->S, N = ctx->natoms;
+    std::vector<int> base(S), end(S), tiles;
+    int acc = 0;
+    for (int s = 0; s < S; ++s) {
+        base[s] = acc;
+        for (int t = 0; t < ctx->hm_counts[s]; t += 16) { tiles.push_back(s); tiles.push_back(acc + t); }
+        acc += ctx->hm_counts[s];
+        end[s] = acc;
+    }
+    for (int* p : {ctx->d_perm, ctx->d_tiles, ctx->d_spend})
+        if (p) hipFree(p);
+    ctx->d_perm = ctx->d_tiles = ctx->d_spend = nullptr;
+    if ((e = hipMalloc(&ctx->d_perm, sizeof(int) * std::max(N, 1))) != hipSuccess) return e;
+    if ((e = hipMalloc(&ctx->d_tiles, sizeof(int) * std::max<size_t>(tiles.size(), 2))) != hipSuccess) return e;
+    if ((e = hipMalloc(&ctx->d_spend, sizeof(int) * 2 * S)) != hipSuccess) return e;
+    std::vector<int> be(base);
+    be.insert(be.end(), end.begin(), end.end());
+    if ((e = hipMemcpyAsync(ctx->d_spend, be.data(), sizeof(int) * 2 * S, hipMemcpyHostToDevice, ctx->stream)) != hipSuccess) return e;
+    if (!tiles.empty() &&
+        (e = hipMemcpyAsync(ctx->d_tiles, tiles.data(), sizeof(int) * tiles.size(), hipMemcpyHostToDevice, ctx->stream)) != hipSuccess)
+        return e;
+    k_perm<<<(N + 255) / 256, 256, 0, ctx->stream>>>(N, ctx->d_species, ctx->d_rank, ctx->d_spend, ctx->d_perm);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    // spend[] must hold the END of each species block: shift the uploaded (base, end) pair
+    ctx->d_spend_end = ctx->d_spend + S;
+    if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return e;
+    ctx->ntiles = (int)tiles.size() / 2;
+    ctx->perm_valid = true;
+    return hipSuccess;
+}
 
 hipError_t net_bwd2(ann_ctx* ctx, const float* G, const float* v, const double* gE, double* grad) {
     hipError_t e = species_ranks(ctx);

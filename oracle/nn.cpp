@@ -52,7 +52,9 @@ void activation(int neuron, double c, double z, double& a, double& d, double& d2
         } break;
         case ORC_NEU_ARCTAN: {
             const double zz = c * z * kPI / 2.0, f = 1.0 / (1.0 + zz * zz);
-            a = (2.0 / kPI) * std::atan(zz); d = c * f; d2 = -2.0 * c * zz * f * f;
+            // the reference's d2 (-2 c zz f^2, src/ml/nn.cpp:289) lacks the chain
+            // factor c pi/2; the true second derivative is used here
+            a = (2.0 / kPI) * std::atan(zz); d = c * f; d2 = -2.0 * c * zz * f * f * (c * kPI / 2.0);
         } break;
         case ORC_NEU_RELU:
             if (z > 0.0) { a = z; d = 1.0; } else { a = 0.0; d = 0.0; }

@@ -161,20 +161,20 @@ class Oracle:
     def nn_forward(self, species: int, G):
         G = np.ascontiguousarray(G, dtype=np.float64)
         dE = np.zeros(len(G))
-        e = lib().orc_nn_forward(C.byref(self.c), species, _p(G, P_D), _p(dE, P_D))
+        e = lib().orc_nn_forward(C.byref(self.c), int(species), _p(G, P_D), _p(dE, P_D))
         return e, dE
 
     def nparams(self, species: int) -> int:
-        return lib().orc_nparams(C.byref(self.c), species)
+        return lib().orc_nparams(C.byref(self.c), int(species))
 
     def nn_grad_rop(self, species: int, G, v=None, gE: float = 1.0):
         G = np.ascontiguousarray(G, dtype=np.float64)
         g = np.zeros(self.nparams(species))
         if v is None:
-            lib().orc_nn_grad_energy(C.byref(self.c), species, _p(G, P_D), C.c_double(gE), _p(g, P_D))
+            lib().orc_nn_grad_energy(C.byref(self.c), int(species), _p(G, P_D), C.c_double(gE), _p(g, P_D))
         else:
             v = np.ascontiguousarray(v, dtype=np.float64)
-            lib().orc_nn_grad_rop(C.byref(self.c), species, _p(G, P_D), _p(v, P_D), C.c_double(gE),
+            lib().orc_nn_grad_rop(C.byref(self.c), int(species), _p(G, P_D), _p(v, P_D), C.c_double(gE),
                                   _p(g, P_D))
         return g
 
@@ -224,7 +224,8 @@ class NList:
 
 
 def adam_step(p, m1, m2, g, gamma, eps, t):
-    p = np.ascontiguousarray(p, dtype=np.float64)
+    """One ADAM::step on a copy of p (moments m1, m2 updated in place)."""
+    p = np.array(p, dtype=np.float64, copy=True)
     lib().orc_adam_step(len(p), _p(p, P_D), _p(m1, P_D), _p(m2, P_D),
                         _p(np.ascontiguousarray(g, dtype=np.float64), P_D), C.c_double(gamma),
                         C.c_double(eps), t)

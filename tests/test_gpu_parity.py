@@ -146,3 +146,23 @@ def test_capacity_error_is_reported():
     with pytest.raises(AnnError) as ei:
         eng.neighbors()
     assert ei.value.code == -3
+
+
+@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("name", ["ar256", "water4", "lips3"])
+def test_network_parity(cases, oracle_mod, name, form):
+    """Atom energies and dE/dG of both network kernels against the FP64
+    network on the same (GPU-computed) G."""
+    model, batch = cases[name]
+    print(f"case {name} form {form}", flush=True)
+    eng = _engine(model, batch)
+    G = eng.symm(form)
+    E, dE = eng.net(G, form)
+    E, dE = E.cpu().numpy(), dE.cpu().numpy()
+    orc = oracle_mod.Oracle(model)
+    Gc = G.cpu().numpy().astype(np.float64)[:, :orc.ld]
+    Eo, dEo = orc.net(batch.species, Gc)
+    assert np.max(np.abs(E - Eo)) <= 1e-6 * max(1.0, np.abs(Eo).max())
+    err = np.sqrt(((dE[:, :orc.ld] - dEo) ** 2).sum() / (dEo ** 2).sum())
+    assert err <= 1e-5, err
+    assert np.all(dE[:, orc.ld:] == 0)
