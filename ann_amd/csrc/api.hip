@@ -175,6 +175,8 @@ static int upload_model(ann_ctx* ctx, const ann_model_desc* d) {
                         it = uni.end() - 1;
                     }
                     ab.gu[g] = (int)(it - uni.begin());
+                    // the fast kernels address width g of a block as union width g
+                    if (ab.gu[g] != g) grid = false;
                 }
             }
             ab.grid = grid ? 1 : 0;
@@ -186,6 +188,15 @@ static int upload_model(ann_ctx* ctx, const ann_model_desc* d) {
         for (size_t u = 0; u < uni.size(); ++u) M.uieta2[a][u] = (float)uni[u];
         M.nin[a] = nin[a];
     }
+    // the fast kernels are compiled per grid shape: every non-empty block must share it
+    M.fast_ng = M.fast_nk = 0;
+    for (int t = 0; t < S * NP; ++t) {
+        const AngBlock& ab = M.ang[t];
+        if (ab.n == 0) continue;
+        if (M.fast_ng == 0) { M.fast_ng = ab.ng; M.fast_nk = ab.nk; }
+        else if (ab.ng != M.fast_ng || ab.nk != M.fast_nk) M.all_grid = 0;
+    }
+    if (M.fast_ng == 0) { M.fast_ng = 1; M.fast_nk = 1; }
     int maxin = 1;
     for (int a = 0; a < S; ++a) maxin = std::max(maxin, nin[a]);
     ctx->max_nin = maxin;

@@ -17,7 +17,7 @@ constexpr int kMaxNeigh = 512;     // neighbour-list capacity per atom
 constexpr int kMaxLayers = 4;      // hidden + output
 constexpr int kMaxWidth = 64;      // widest layer supported by the network kernels
 constexpr int kMaxFunc = 32;       // functions per basis block
-constexpr int kMaxUnionEta = 8;    // distinct angular widths per centre species
+constexpr int kMaxUnionEta = 4;    // distinct angular widths per centre species (fast form)
 constexpr int kGridG = 4;          // fast angular tile: widths per block
 constexpr int kGridK = 8;          // fast angular tile: (lambda, zeta) combinations per block
 
@@ -60,6 +60,7 @@ struct DevModel {
     AngBlock ang[kMaxSpecies * (kMaxSpecies * (kMaxSpecies + 1) / 2)];
     NetDesc net[kMaxSpecies];
     int all_grid;                               // every angular block is grid-shaped
+    int fast_ng, fast_nk;                       // the common grid shape (0 if blocks differ)
 };
 
 // device pointers of the variable-size tables

@@ -54,6 +54,35 @@ struct SymmArgs {
     float* v;
 };
 
+// Sum N per-lane values over the 64 lanes with a transposing butterfly: at
+// each halving step a lane keeps half of its values and receives the partner's
+// copy of that half (N - 1 exchanges instead of 6N), then the lanes that hold
+// the same value index finish with plain butterflies.  Afterwards lane l holds
+// the total of value index vidx(l); returns it.
+template <int N>
+__device__ __forceinline__ float wave_sum_multi(float (&v)[N], int& idx) {
+    const int lane = threadIdx.x % kWave;
+    int base = 0;
+#pragma unroll
+    for (int h = N / 2, bit = 32; h >= 1; h >>= 1, bit >>= 1) {
+        const bool upper = (lane & bit) != 0;
+#pragma unroll
+        for (int q = 0; q < h; ++q) {
+            const float send = upper ? v[q] : v[q + h];
+            const float keep = upper ? v[q + h] : v[q];
+            v[q] = keep + __shfl_xor(send, bit);
+        }
+        if (upper) base += h;
+    }
+    float s = v[0];
+    constexpr int steps = (N >= 64) ? 0 : 6 - __builtin_ctz(N);
+    int bit = 1;
+#pragma unroll
+    for (int q = 0; q < steps; ++q, bit <<= 1) s += __shfl_xor(s, bit);
+    idx = base;
+    return s;
+}
+
 __device__ __forceinline__ float wave_sum(float v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -191,22 +220,54 @@ __global__ __launch_bounds__(64) void k_plain(SymmArgs A) {
 
 // ============================================================ FAST form ====
 
+constexpr int kFastU = 4;   // distinct angular widths per centre species in the fast form
+
 struct FastSmem {
-    float4* g;      // [maxn] (hx, hy, hz, r), species-sorted
-    float* ef;      // [U][maxn] exp(-r^2 ieta2_u) fc
-    float* efd;     // [U][maxn] d/dr of ef
-    float* fc;      // [maxn]
-    float* gc;      // [maxn]
+    float4* g;      // [maxn] (hx, hy, hz, 1/r), species-sorted
+    float4* ef;     // [maxn] exp(-r^2 ieta2_u) fc for u = 0..3
+    float4* efd;    // [maxn] d/dr of ef
+    float* r;       // [maxn]
     int* canon;     // [maxn] canonical slot of each sorted entry
     int* seg;       // [kMaxSpecies + 1]
     float* extra;   // kernel specific
 };
 
+__device__ __forceinline__ float f4get(const float4& v, int u) {
+    return u == 0 ? v.x : (u == 1 ? v.y : (u == 2 ? v.z : v.w));
+}
+
+// w^e for a wave-uniform integer e >= 0 (common exponents unrolled)
+__device__ __forceinline__ float ipow(float w, int e) {
+    switch (e) {
+        case 0: return 1.0f;
+        case 1: return w;
+        case 2: return w * w;
+        case 3: return w * w * w;
+        case 4: { const float q = w * w; return q * q; }
+        case 7: { const float q = w * w, q4 = q * q; return q4 * q * w; }
+        case 8: { const float q = w * w, q4 = q * q; return q4 * q4; }
+        default: {
+            float r = 1.0f, b = w;
+            for (int x = e; x; x >>= 1) {
+                if (x & 1) r *= b;
+                b *= b;
+            }
+            return r;
+        }
+    }
+}
+
+// w^(zeta-1) for integer (zi >= 1) or general zeta (zi < 0)
+__device__ __forceinline__ float pow_m1(float w, int zi, float zeta) {
+    if (zi >= 1) return ipow(w, zi - 1);
+    return w > 0.0f ? exp2f((zeta - 1.0f) * __log2f(w)) : 0.0f;
+}
+
 // Regroup the neighbours of atom i by species (stable) and precompute the
-// per-neighbour factors.  Returns nothing; seg[] gives the species segments.
+// per-neighbour factors; seg[] gives the species segments.
 __device__ void fast_load(const SymmArgs& A, const DevModel& M, int i, int a, int o, int n, FastSmem& sm,
                           bool need_deriv) {
-    const int lane = threadIdx.x, S = M.S, maxn = A.maxn, U = M.nunion[a];
+    const int lane = threadIdx.x, S = M.S, U = M.nunion[a];
     int cnt[kMaxSpecies];
 #pragma unroll
     for (int b = 0; b < kMaxSpecies; ++b) cnt[b] = 0;
@@ -228,6 +289,9 @@ __device__ void fast_load(const SymmArgs& A, const DevModel& M, int i, int a, in
     for (int b = 0; b < kMaxSpecies; ++b)
         if (b < S && lane == b) sm.seg[b] = base[b];
     if (lane == S) sm.seg[S] = acc;
+    float ie[kFastU];
+#pragma unroll
+    for (int u = 0; u < kFastU; ++u) ie[u] = u < U ? M.uieta2[a][u] : 0.0f;
     const uint64_t lt = (1ull << lane) - 1ull;
     for (int q = 0; q < n; q += kWave) {
         const int t = q + lane;
@@ -244,33 +308,35 @@ __device__ void fast_load(const SymmArgs& A, const DevModel& M, int i, int a, in
         if (t < n) {
             const float4 gq = A.geo[o + t];
             const float r = gq.w, ir = 1.0f / r;
-            sm.g[pos] = make_float4(gq.x * ir, gq.y * ir, gq.z * ir, r);
+            sm.g[pos] = make_float4(gq.x * ir, gq.y * ir, gq.z * ir, ir);
+            sm.r[pos] = r;
             sm.canon[pos] = t;
             float f, gcut;
             cutoff_fg(M.cutoff, M.rc, M.rci, M.pirc, r, f, gcut);
-            sm.fc[pos] = f;
-            sm.gc[pos] = gcut;
-            for (int uu = 0; uu < U; ++uu) {
-                const float ie = M.uieta2[a][uu];
-                const float e = __expf(-ie * r * r);
-                sm.ef[uu * maxn + pos] = e * f;
-                if (need_deriv) sm.efd[uu * maxn + pos] = e * (-2.0f * r * ie * f + gcut);
+            float e[kFastU], d[kFastU];
+#pragma unroll
+            for (int u = 0; u < kFastU; ++u) {
+                const float x = u < U ? __expf(-ie[u] * r * r) : 0.0f;
+                e[u] = x * f;
+                d[u] = x * (-2.0f * r * ie[u] * f + gcut);
             }
+            sm.ef[pos] = make_float4(e[0], e[1], e[2], e[3]);
+            if (need_deriv) sm.efd[pos] = make_float4(d[0], d[1], d[2], d[3]);
         }
     }
 }
 
 // ---- forward / tangent over one grid block (unordered pairs, flattened) ----
 template <int NG, int NK, int MODE>
-__device__ void fast_block_flat(const AngBlock& B, const FastSmem& sm, int maxn, int sb, int nb, int sc,
-                                int nc, bool tri, const float4* tv, float* out) {
+__device__ void fast_block_flat(const AngBlock& B, const FastSmem& sm, int sb, int nb, int sc, int nc, bool tri,
+                                const float4* tv, float* out) {
     const int lane = threadIdx.x;
     float lamh[NK], zeta[NK];
     int zi[NK], gu[NG];
 #pragma unroll
     for (int k = 0; k < NK; ++k) { lamh[k] = B.lamh[k]; zeta[k] = B.zeta[k]; zi[k] = B.zint[k]; }
 #pragma unroll
-    for (int g = 0; g < NG; ++g) gu[g] = B.gu[g] * maxn;
+    for (int g = 0; g < NG; ++g) gu[g] = B.gu[g];
     float acc[NG][NK];
 #pragma unroll
     for (int g = 0; g < NG; ++g)
@@ -296,33 +362,38 @@ __device__ void fast_block_flat(const AngBlock& B, const FastSmem& sm, int maxn,
         jj += sb;
         kk += sc;
         const float4 gj = sm.g[jj], gk = sm.g[kk];
+        const float4 ej4 = sm.ef[jj], ek4 = sm.ef[kk];
         const float c = gj.x * gk.x + gj.y * gk.y + gj.z * gk.z;
         if (MODE == 0) {
             float p[NK];
 #pragma unroll
-            for (int k = 0; k < NK; ++k) p[k] = upow(fabsf(0.5f + lamh[k] * c), zi[k], zeta[k]);
+            for (int k = 0; k < NK; ++k) {
+                const float cw = fabsf(0.5f + lamh[k] * c);
+                p[k] = zi[k] >= 0 ? ipow(cw, zi[k]) : pow_m1(cw, -1, zeta[k]) * cw;
+            }
 #pragma unroll
             for (int g = 0; g < NG; ++g) {
-                const float D = sm.ef[gu[g] + jj] * sm.ef[gu[g] + kk];
+                const float D = f4get(ej4, g) * f4get(ek4, g);
 #pragma unroll
                 for (int k = 0; k < NK; ++k) acc[g][k] += p[k] * D;
             }
         } else {
             const float4 tj = tv[jj], tk = tv[kk];
-            const float cdot = ((gk.x * tj.x + gk.y * tj.y + gk.z * tj.z) - c * tj.w) / gj.w +
-                               ((gj.x * tk.x + gj.y * tk.y + gj.z * tk.z) - c * tk.w) / gk.w;
+            const float4 dj4 = sm.efd[jj], dk4 = sm.efd[kk];
+            const float cdot = ((gk.x * tj.x + gk.y * tj.y + gk.z * tj.z) - c * tj.w) * gj.w +
+                               ((gj.x * tk.x + gj.y * tk.y + gj.z * tk.z) - c * tk.w) * gk.w;
             float p[NK], dp[NK];
 #pragma unroll
             for (int k = 0; k < NK; ++k) {
                 const float cw = fabsf(0.5f + lamh[k] * c);
-                const float w1 = upow(cw, zi[k] >= 1 ? zi[k] - 1 : -1, zeta[k] - 1.0f);
+                const float w1 = pow_m1(cw, zi[k], zeta[k]);
                 p[k] = w1 * cw;
                 dp[k] = zeta[k] * lamh[k] * w1;
             }
 #pragma unroll
             for (int g = 0; g < NG; ++g) {
-                const float ej = sm.ef[gu[g] + jj], ek = sm.ef[gu[g] + kk];
-                const float rdot = sm.efd[gu[g] + jj] * ek * tj.w + ej * sm.efd[gu[g] + kk] * tk.w;
+                const float ej = f4get(ej4, g), ek = f4get(ek4, g);
+                const float rdot = f4get(dj4, g) * ek * tj.w + ej * f4get(dk4, g) * tk.w;
                 const float D = ej * ek * cdot;
 #pragma unroll
                 for (int k = 0; k < NK; ++k) acc[g][k] += dp[k] * D + p[k] * rdot;
@@ -332,19 +403,29 @@ __device__ void fast_block_flat(const AngBlock& B, const FastSmem& sm, int maxn,
         j += dj;
         if (t >= cols) { t -= cols; j += 1; }
     }
+    float flat[NG * NK];
 #pragma unroll
     for (int g = 0; g < NG; ++g)
 #pragma unroll
-        for (int k = 0; k < NK; ++k) {
-            const float s = wave_sum(acc[g][k]);
-            if (lane == 0) out[B.goff + B.outm[g * NK + k]] = s;
+        for (int k = 0; k < NK; ++k) flat[g * NK + k] = acc[g][k];
+    if constexpr ((NG * NK & (NG * NK - 1)) == 0) {
+        int idx;
+        const float s = wave_sum_multi<NG * NK>(flat, idx);
+        constexpr int lpv = kWave / (NG * NK);   // lanes holding each value
+        if ((lane & (lpv - 1)) == 0 && lane < kWave) out[B.goff + B.outm[idx]] = s;
+    } else {
+#pragma unroll
+        for (int q = 0; q < NG * NK; ++q) {
+            const float s = wave_sum(flat[q]);
+            if (lane == 0) out[B.goff + B.outm[q]] = s;
         }
+    }
 }
 
 // ---- adjoint over one grid block: skewed owner/partner schedule ----
 template <int NG, int NK>
-__device__ void fast_block_adj(const AngBlock& B, const FastSmem& sm, int maxn, int sb, int nb, int sc,
-                               int nc, bool tri, const float* sd, float* sP) {
+__device__ void fast_block_adj(const AngBlock& B, const FastSmem& sm, int sb, int nb, int sc, int nc, bool tri,
+                               const float* sd, float* sP) {
     const int lane = threadIdx.x;
     float lamh[NK], zeta[NK], dl[NG][NK];
     int zi[NK], gu[NG];
@@ -352,7 +433,7 @@ __device__ void fast_block_adj(const AngBlock& B, const FastSmem& sm, int maxn, 
     for (int k = 0; k < NK; ++k) { lamh[k] = B.lamh[k]; zeta[k] = B.zeta[k]; zi[k] = B.zint[k]; }
 #pragma unroll
     for (int g = 0; g < NG; ++g) {
-        gu[g] = B.gu[g] * maxn;
+        gu[g] = B.gu[g];
 #pragma unroll
         for (int k = 0; k < NK; ++k) dl[g][k] = sd[B.goff + B.outm[g * NK + k]];
     }
@@ -369,31 +450,33 @@ __device__ void fast_block_adj(const AngBlock& B, const FastSmem& sm, int maxn, 
         const int q = act ? id / no : 0;
         const int J = so + j;
         const float4 gj = sm.g[J];
+        const float4 ej4 = sm.ef[J], dj4 = sm.efd[J];
         float ej[NG], edj[NG];
 #pragma unroll
-        for (int g = 0; g < NG; ++g) { ej[g] = sm.ef[gu[g] + J]; edj[g] = sm.efd[gu[g] + J]; }
+        for (int g = 0; g < NG; ++g) { ej[g] = f4get(ej4, g); edj[g] = f4get(dj4, g); }
         float px = 0.0f, py = 0.0f, pz = 0.0f;
         int len, koff;
         if (tri) {
             len = h + ((no & 1) == 0 && j < (no >> 1) ? 1 : 0);
-            koff = 1;
+            koff = j + 1;
         } else {
             len = (q < Q - 1) ? no : np - (Q - 1) * no;
             koff = j + q * no;
         }
-        int lmax = tri ? h + ((no & 1) == 0 ? 1 : 0) : (np - (Q - 1) * no);
+        const int lmax = tri ? h + ((no & 1) == 0 ? 1 : 0) : (np - (Q - 1) * no);
         for (int t = 0; t < lmax; ++t) {
             if (act && t < len) {
-                int k = tri ? j + koff + t : koff + t;
+                int k = koff + t;
                 if (k >= np) k -= np;
                 const int K = spp + k;
                 const float4 gk = sm.g[K];
+                const float4 ek4 = sm.ef[K], dk4 = sm.efd[K];
                 const float c = gj.x * gk.x + gj.y * gk.y + gj.z * gk.z;
                 float p[NK], dp[NK];
 #pragma unroll
                 for (int kk = 0; kk < NK; ++kk) {
                     const float cw = fabsf(0.5f + lamh[kk] * c);
-                    const float w1 = upow(cw, zi[kk] >= 1 ? zi[kk] - 1 : -1, zeta[kk] - 1.0f);
+                    const float w1 = pow_m1(cw, zi[kk], zeta[kk]);
                     p[kk] = w1 * cw;
                     dp[kk] = zeta[kk] * lamh[kk] * w1;
                 }
@@ -406,12 +489,12 @@ __device__ void fast_block_adj(const AngBlock& B, const FastSmem& sm, int maxn, 
                         Sg += dl[g][kk] * p[kk];
                         Tg += dl[g][kk] * dp[kk];
                     }
-                    const float ek = sm.ef[gu[g] + K], edk = sm.efd[gu[g] + K];
+                    const float ek = f4get(ek4, g), edk = f4get(dk4, g);
                     Ac += Tg * ej[g] * ek;
                     Bj += Sg * edj[g] * ek;
                     Bk += Sg * ej[g] * edk;
                 }
-                const float aj = Ac / gj.w, ak = Ac / gk.w;
+                const float aj = Ac * gj.w, ak = Ac * gk.w;
                 px += aj * (gk.x - c * gj.x) + Bj * gj.x;
                 py += aj * (gk.y - c * gj.y) + Bj * gj.y;
                 pz += aj * (gk.z - c * gj.z) + Bj * gj.z;
@@ -428,48 +511,18 @@ __device__ void fast_block_adj(const AngBlock& B, const FastSmem& sm, int maxn, 
     }
 }
 
-template <int MODE>
-__device__ void dispatch_flat(const AngBlock& B, const FastSmem& sm, int maxn, int sb, int nb, int sc, int nc,
-                              bool tri, const float4* tv, float* out) {
-    if (B.ng == 1 && B.nk == 8) fast_block_flat<1, 8, MODE>(B, sm, maxn, sb, nb, sc, nc, tri, tv, out);
-    else if (B.ng == 4 && B.nk == 4) fast_block_flat<4, 4, MODE>(B, sm, maxn, sb, nb, sc, nc, tri, tv, out);
-    else if (B.ng == 3 && B.nk == 4) fast_block_flat<3, 4, MODE>(B, sm, maxn, sb, nb, sc, nc, tri, tv, out);
-    else if (B.ng == 2 && B.nk == 4) fast_block_flat<2, 4, MODE>(B, sm, maxn, sb, nb, sc, nc, tri, tv, out);
-    else if (B.ng == 1 && B.nk == 4) fast_block_flat<1, 4, MODE>(B, sm, maxn, sb, nb, sc, nc, tri, tv, out);
-    else if (B.ng == 1 && B.nk == 2) fast_block_flat<1, 2, MODE>(B, sm, maxn, sb, nb, sc, nc, tri, tv, out);
-    else if (B.ng == 1 && B.nk == 1) fast_block_flat<1, 1, MODE>(B, sm, maxn, sb, nb, sc, nc, tri, tv, out);
-    else if (B.ng == 2 && B.nk == 2) fast_block_flat<2, 2, MODE>(B, sm, maxn, sb, nb, sc, nc, tri, tv, out);
-    else if (B.ng == 4 && B.nk == 8) fast_block_flat<4, 8, MODE>(B, sm, maxn, sb, nb, sc, nc, tri, tv, out);
-    else if (B.ng == 2 && B.nk == 8) fast_block_flat<2, 8, MODE>(B, sm, maxn, sb, nb, sc, nc, tri, tv, out);
-    else if (B.ng == 4 && B.nk == 2) fast_block_flat<4, 2, MODE>(B, sm, maxn, sb, nb, sc, nc, tri, tv, out);
-    else if (B.ng == 3 && B.nk == 2) fast_block_flat<3, 2, MODE>(B, sm, maxn, sb, nb, sc, nc, tri, tv, out);
-    else if (B.ng == 3 && B.nk == 8) fast_block_flat<3, 8, MODE>(B, sm, maxn, sb, nb, sc, nc, tri, tv, out);
-}
+#define ANN_SHAPES(X) X(1, 8) X(4, 4) X(3, 4) X(2, 4) X(1, 4) X(1, 2) X(1, 1) X(2, 2) X(4, 2) X(3, 2)
 
-__device__ void dispatch_adj(const AngBlock& B, const FastSmem& sm, int maxn, int sb, int nb, int sc, int nc,
-                             bool tri, const float* sd, float* sP) {
-    if (B.ng == 1 && B.nk == 8) fast_block_adj<1, 8>(B, sm, maxn, sb, nb, sc, nc, tri, sd, sP);
-    else if (B.ng == 4 && B.nk == 4) fast_block_adj<4, 4>(B, sm, maxn, sb, nb, sc, nc, tri, sd, sP);
-    else if (B.ng == 3 && B.nk == 4) fast_block_adj<3, 4>(B, sm, maxn, sb, nb, sc, nc, tri, sd, sP);
-    else if (B.ng == 2 && B.nk == 4) fast_block_adj<2, 4>(B, sm, maxn, sb, nb, sc, nc, tri, sd, sP);
-    else if (B.ng == 1 && B.nk == 4) fast_block_adj<1, 4>(B, sm, maxn, sb, nb, sc, nc, tri, sd, sP);
-    else if (B.ng == 1 && B.nk == 2) fast_block_adj<1, 2>(B, sm, maxn, sb, nb, sc, nc, tri, sd, sP);
-    else if (B.ng == 1 && B.nk == 1) fast_block_adj<1, 1>(B, sm, maxn, sb, nb, sc, nc, tri, sd, sP);
-    else if (B.ng == 2 && B.nk == 2) fast_block_adj<2, 2>(B, sm, maxn, sb, nb, sc, nc, tri, sd, sP);
-    else if (B.ng == 4 && B.nk == 8) fast_block_adj<4, 8>(B, sm, maxn, sb, nb, sc, nc, tri, sd, sP);
-    else if (B.ng == 2 && B.nk == 8) fast_block_adj<2, 8>(B, sm, maxn, sb, nb, sc, nc, tri, sd, sP);
-    else if (B.ng == 4 && B.nk == 2) fast_block_adj<4, 2>(B, sm, maxn, sb, nb, sc, nc, tri, sd, sP);
-    else if (B.ng == 3 && B.nk == 2) fast_block_adj<3, 2>(B, sm, maxn, sb, nb, sc, nc, tri, sd, sP);
-    else if (B.ng == 3 && B.nk == 8) fast_block_adj<3, 8>(B, sm, maxn, sb, nb, sc, nc, tri, sd, sP);
-}
-
-// shapes handled by the dispatchers above
+// shapes with a compiled fast kernel
 __host__ __device__ inline bool fast_shape_ok(int ng, int nk) {
-    return (ng == 1 && (nk == 1 || nk == 2 || nk == 4 || nk == 8)) ||
-           ((ng == 2 || ng == 3 || ng == 4) && (nk == 2 || nk == 4 || nk == 8));
+#define X(G, K) if (ng == G && nk == K) return true;
+    ANN_SHAPES(X)
+#undef X
+    return false;
 }
 
-// radial part, owner = lane over the neighbours of one species segment
+// radial part, lane over the neighbours of one species segment, 16 functions at a time
+constexpr int kRadChunk = 16;
 template <int MODE>
 __device__ void fast_radial(const SymmArgs& A, const DevModel& M, int a, const FastSmem& sm, const float4* tv,
                             const float* sd, float* sP, float* out) {
@@ -481,10 +534,12 @@ __device__ void fast_radial(const SymmArgs& A, const DevModel& M, int a, const F
         if (MODE == 1) {
             for (int J = s0 + lane; J < s1; J += kWave) {
                 const float4 gj = sm.g[J];
+                float fc, gc;
+                cutoff_fg(M.cutoff, M.rc, M.rci, M.pirc, sm.r[J], fc, gc);
                 float amp = 0.0f;
                 for (int m = 0; m < rb.n; ++m) {
                     float v, d;
-                    radial_fd(rb.name, A.rad_eta[rb.par + m], A.rad_rs[rb.par + m], gj.w, sm.fc[J], sm.gc[J], v, d);
+                    radial_fd(rb.name, A.rad_eta[rb.par + m], A.rad_rs[rb.par + m], sm.r[J], fc, gc, v, d);
                     amp += sd[rb.goff + m] * d;
                 }
                 sP[3 * J] += amp * gj.x;
@@ -493,63 +548,59 @@ __device__ void fast_radial(const SymmArgs& A, const DevModel& M, int a, const F
             }
             continue;
         }
-        float acc[kMaxFunc];
+        for (int m0 = 0; m0 < rb.n; m0 += kRadChunk) {
+            float acc[kRadChunk];
 #pragma unroll
-        for (int m = 0; m < kMaxFunc; ++m) acc[m] = 0.0f;
-        for (int J = s0 + lane; J < s1; J += kWave) {
-            const float r = sm.g[J].w;
-            const float w = MODE == 2 ? tv[J].w : 0.0f;
+            for (int m = 0; m < kRadChunk; ++m) acc[m] = 0.0f;
+            for (int J = s0 + lane; J < s1; J += kWave) {
+                const float r = sm.r[J];
+                const float w = MODE == 2 ? tv[J].w : 0.0f;
+                float fc, gc;
+                cutoff_fg(M.cutoff, M.rc, M.rci, M.pirc, r, fc, gc);
 #pragma unroll
-            for (int m = 0; m < kMaxFunc; ++m) {
-                if (m < rb.n) {
-                    float v, d;
-                    radial_fd(rb.name, A.rad_eta[rb.par + m], A.rad_rs[rb.par + m], r, sm.fc[J], sm.gc[J], v, d);
-                    acc[m] += MODE == 0 ? v : d * w;
+                for (int m = 0; m < kRadChunk; ++m) {
+                    if (m0 + m < rb.n) {
+                        float v, d;
+                        radial_fd(rb.name, A.rad_eta[rb.par + m0 + m], A.rad_rs[rb.par + m0 + m], r, fc, gc, v, d);
+                        acc[m] += MODE == 0 ? v : d * w;
+                    }
                 }
             }
-        }
-#pragma unroll
-        for (int m = 0; m < kMaxFunc; ++m) {
-            if (m < rb.n) {
-                const float s = wave_sum(acc[m]);
-                if (lane == 0) out[rb.goff + m] = s;
-            }
+            int idx;
+            const float s = wave_sum_multi<kRadChunk>(acc, idx);
+            if ((lane & (kWave / kRadChunk - 1)) == 0 && m0 + idx < rb.n) out[rb.goff + m0 + idx] = s;
         }
     }
 }
 
-// mode 0: G, 1: adjoint P, 2: tangent v
-template <int MODE>
+// mode 0: G, 1: adjoint P, 2: tangent v; every angular block has the grid shape NG x NK
+template <int MODE, int NG, int NK>
 __global__ __launch_bounds__(64) void k_fast(SymmArgs A) {
-    extern __shared__ float smem[];
+    extern __shared__ float4 smem4[];
     const int i = blockIdx.x, lane = threadIdx.x;
     if (i >= A.N) return;
     const DevModel& M = *A.dm;
     const int a = A.species[i], o = A.off[i], n = A.off[i + 1] - o;
-    const int S = M.S, NP = M.NP, ldg = A.ldg, maxn = A.maxn, U = M.nunion[a];
+    const int S = M.S, NP = M.NP, ldg = A.ldg, maxn = A.maxn;
     FastSmem sm;
-    sm.g = reinterpret_cast<float4*>(smem);
-    sm.ef = reinterpret_cast<float*>(sm.g + maxn);
-    sm.efd = sm.ef + kMaxUnionEta * maxn;
-    sm.fc = sm.efd + (MODE == 0 ? 0 : kMaxUnionEta * maxn);
-    sm.gc = sm.fc + maxn;
-    sm.canon = reinterpret_cast<int*>(sm.gc + maxn);
+    sm.g = smem4;
+    sm.ef = sm.g + maxn;
+    sm.efd = sm.ef + maxn;
+    float4* tv = sm.efd + (MODE == 0 ? 0 : maxn);                // [maxn] (mode 2)
+    sm.r = reinterpret_cast<float*>(tv + (MODE == 2 ? maxn : 0));
+    sm.canon = reinterpret_cast<int*>(sm.r + maxn);
     sm.seg = sm.canon + maxn;
     sm.extra = reinterpret_cast<float*>(sm.seg + 16);
-    (void)U;
     float* rowbuf = sm.extra;                                   // [ldg] output row (modes 0, 2) / dE/dG (mode 1)
     float* sP = sm.extra + ldg;                                 // [maxn][3] (mode 1)
-    float4* tv = reinterpret_cast<float4*>(sm.extra + ldg + (ldg & 3 ? 4 - (ldg & 3) : 0));  // [maxn] (mode 2)
     fast_load(A, M, i, a, o, n, sm, MODE != 0);
     if (MODE == 1) {
         for (int m = lane; m < ldg; m += kWave) rowbuf[m] = A.dEdG[(size_t)i * ldg + m];
+        for (int t = lane; t < 3 * n; t += kWave) sP[t] = 0.0f;
     } else {
         for (int m = lane; m < ldg; m += kWave) rowbuf[m] = 0.0f;
     }
     __syncthreads();
-    if (MODE == 1) {
-        for (int t = lane; t < 3 * n; t += kWave) sP[t] = 0.0f;
-    }
     if (MODE == 2) {
         // per neighbour tangent t_j = u_i - u_j and r^_j . t_j, species-sorted
         for (int J = lane; J < n; J += kWave) {
@@ -559,8 +610,8 @@ __global__ __launch_bounds__(64) void k_fast(SymmArgs A) {
             const float4 gq = sm.g[J];
             tv[J] = make_float4(tx, ty, tz, gq.x * tx + gq.y * ty + gq.z * tz);
         }
+        __syncthreads();
     }
-    __syncthreads();
     fast_radial<MODE>(A, M, a, sm, tv, rowbuf, sP, rowbuf);
     __syncthreads();
     for (int b = 0; b < S; ++b) {
@@ -571,8 +622,8 @@ __global__ __launch_bounds__(64) void k_fast(SymmArgs A) {
             const int sc = sm.seg[c], nc = sm.seg[c + 1] - sc;
             const bool tri = (b == c);
             if (tri ? nb < 2 : (nb == 0 || nc == 0)) continue;
-            if (MODE == 1) dispatch_adj(B, sm, maxn, sb, nb, sc, nc, tri, rowbuf, sP);
-            else dispatch_flat<MODE == 0 ? 0 : 1>(B, sm, maxn, sb, nb, sc, nc, tri, tv, rowbuf);
+            if (MODE == 1) fast_block_adj<NG, NK>(B, sm, sb, nb, sc, nc, tri, rowbuf, sP);
+            else fast_block_flat<NG, NK, MODE == 0 ? 0 : 1>(B, sm, sb, nb, sc, nc, tri, tv, rowbuf);
             __syncthreads();
         }
     }
@@ -619,28 +670,34 @@ static size_t plain_smem(const SymmArgs& A, int mode) {
 }
 
 static size_t fast_smem(const SymmArgs& A, int mode) {
-    size_t b = 16 * (size_t)A.maxn + 4 * (size_t)kMaxUnionEta * A.maxn * (mode == 0 ? 1 : 2) +
-               4 * 3 * (size_t)A.maxn + 4 * 16;
-    b += 4 * ((size_t)A.ldg + 4);
+    size_t b = 16 * (size_t)A.maxn * (2 + (mode == 0 ? 0 : 1) + (mode == 2 ? 1 : 0));   // g, ef, efd, tv
+    b += 4 * 2 * (size_t)A.maxn + 4 * 16;                                              // r, canon, seg
+    b += 4 * (size_t)A.ldg;
     if (mode == 1) b += 12 * (size_t)A.maxn;
-    if (mode == 2) b += 16 * (size_t)A.maxn;
     return b;
 }
 
 template <int MODE>
 static hipError_t run(ann_ctx* ctx, int form, SymmArgs A) {
     if (A.N == 0) return hipSuccess;
-    const bool fast = form == ANN_FORM_FAST && ctx->hm.all_grid;
+    const DevModel& M = ctx->hm;
+    const bool fast = form == ANN_FORM_FAST && M.all_grid && M.fast_ng > 0;
     const char* names[2][3] = {{"symm_fwd_plain", "symm_bwd_plain", "symm_tan_plain"},
                                {"symm_fwd_fast", "symm_bwd_fast", "symm_tan_fast"}};
     Launch L(ctx, names[fast ? 1 : 0][MODE]);
     if (fast) {
         const size_t sm = fast_smem(A, MODE);
-        k_fast<MODE><<<A.N, 64, sm, ctx->stream>>>(A);
-    } else {
-        const size_t sm = plain_smem(A, MODE);
-        k_plain<MODE><<<A.N, 64, sm, ctx->stream>>>(A);
+#define X(G, K)                                                                \
+        if (M.fast_ng == G && M.fast_nk == K) {                                \
+            k_fast<MODE, G, K><<<A.N, 64, sm, ctx->stream>>>(A);               \
+            return hipGetLastError();                                          \
+        }
+        ANN_SHAPES(X)
+#undef X
+        return hipErrorInvalidValue;
     }
+    const size_t sm = plain_smem(A, MODE);
+    k_plain<MODE><<<A.N, 64, sm, ctx->stream>>>(A);
     return hipGetLastError();
 }
 
