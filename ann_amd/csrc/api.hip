@@ -291,7 +291,7 @@ extern "C" int ann_destroy(ann_ctx* ctx) {
                     ctx->d_bin_start, ctx->d_bin_atoms, ctx->d_keys, ctx->d_cnt, ctx->d_flag, ctx->nl.off,
                     ctx->nl.idx, ctx->nl.img, ctx->nl.rev, ctx->nl.geo, ctx->nl.nsp, ctx->d_G, ctx->d_dEdG,
                     ctx->d_Eatom, ctx->d_P, ctx->d_u, ctx->d_v, ctx->d_F, ctx->d_E, ctx->d_gE, ctx->d_atomvir,
-                    ctx->d_scr, ctx->d_gpart, ctx->d_rank, ctx->d_spcount, ctx->d_perm, ctx->d_tiles, ctx->d_spend};
+                    ctx->d_scr, ctx->d_gpart, ctx->d_rank, ctx->d_spcount, ctx->d_perm, ctx->d_tiles, ctx->d_spend, ctx->d_grad_tiles, ctx->d_gsplit};
     for (void* p : ptrs)
         if (p) hipFree(p);
     for (hipEvent_t e : ctx->evpool) hipEventDestroy(e);

@@ -138,6 +138,10 @@ struct ann_ctx {
     int* d_spend_end = nullptr;       // = d_spend + S
     int ntiles = 0;
     bool perm_valid = false;
+    std::vector<int> grad_tiles;      // (species, layer, n0, m0) tiles of the parameter gradient
+    int* d_grad_tiles = nullptr;
+    double* d_gsplit = nullptr;       // per atom-split partial gradients
+    int64_t gsplit_cap = 0;
     bool have_batch = false, have_nlist = false;
     double nl_rc = 0;
     ann::NList nl;

@@ -48,120 +48,149 @@ struct Bwd2Args {
     int64_t base[kMaxSpecies];   // scratch start of each species
 };
 
-template <int MW>
-__global__ __launch_bounds__(256) void k_net_bwd2(Bwd2Args A) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= A.N) return;
-    const int a = A.species[i];
+// 16 lanes per atom, 4 atoms per wavefront; lane q owns neurons q, q+16, ...
+constexpr int kBLanes = 16;
+constexpr int kBOut = kMaxWidth / kBLanes;
+
+__global__ __launch_bounds__(64) void k_net_bwd2(Bwd2Args A) {
+    __shared__ float sx[4][kMaxWidth + 1], sxd[4][kMaxWidth + 1];
+    const int sub = threadIdx.x / kBLanes, q = threadIdx.x % kBLanes;
+    const int i = blockIdx.x * 4 + sub;
+    const bool valid = i < A.N;
+    const int ii = valid ? i : 0;
+    const int a = A.species[ii];
     const NetDesc& D = A.dm->net[a];
     const int L = D.nlayer;
     const float c = D.c;
-    const float* G = A.G + (size_t)i * A.ldg;
-    const float* V = A.v ? A.v + (size_t)i * A.ldg : nullptr;
+    const float* G = A.G + (size_t)ii * A.ldg;
+    const float* V = A.v ? A.v + (size_t)ii * A.ldg : nullptr;
     const float* iw = A.inpw + D.inoff;
     const float* ib = A.inpb + D.inoff;
-    float* row = A.scr + A.base[a] + (int64_t)A.rank[i] * bwd2_stride(D);
-    const float gE = (float)A.gE[A.sof[i]];
-    // scratch offsets per layer
+    float* row = A.scr + A.base[a] + (int64_t)A.rank[ii] * bwd2_stride(D);
+    const float gE = (float)A.gE[A.sof[ii]];
     int loff[kMaxLayers];
     {
-        int s = 0, prev = D.nin;
-        for (int l = 0; l < L; ++l) { loff[l] = s; s += 2 * (prev + D.nodes[l]); prev = D.nodes[l]; }
+        int s0 = 0, prev = D.nin;
+        for (int l = 0; l < L; ++l) { loff[l] = s0; s0 += 2 * (prev + D.nodes[l]); prev = D.nodes[l]; }
     }
-    float x[MW], xd[MW];
-    float dz[kMaxLayers - 1][MW], q2[kMaxLayers - 1][MW];
-    float z[MW], zd[MW];
-    // layer 0 from the scaled input s = inpw (G + inpb), tangent sd = inpw v
-    {
-        const int no = D.nodes[0];
-        const float* W0 = A.W + D.woff[0];
-        const float* b0 = A.W + D.boff[0];
-        float* xr = row + loff[0];
-#pragma unroll
-        for (int n = 0; n < MW; ++n) { z[n] = n < no ? b0[n] : 0.0f; zd[n] = 0.0f; }
-        for (int m = 0; m < D.nin; ++m) {
-            const float s = iw[m] * (G[m] + ib[m]);
-            const float sd = V ? iw[m] * V[m] : 0.0f;
-            xr[m] = s;
-            xr[D.nin + m] = sd;
-            const float* col = W0 + m * no;
-#pragma unroll
-            for (int n = 0; n < MW; ++n)
-                if (n < no) { z[n] += col[n] * s; zd[n] += col[n] * sd; }
+    // scaled input and its tangent: scratch of layer 0
+    for (int m = q; m < D.nin; m += kBLanes)
+        if (valid) {
+            row[m] = iw[m] * (G[m] + ib[m]);
+            row[D.nin + m] = V ? iw[m] * V[m] : 0.0f;
         }
-    }
+    float dz[kMaxLayers - 1][kBOut], qz[kMaxLayers - 1][kBOut];
+    float zo = 0.0f, zdo = 0.0f;   // output neuron (lane 0 after the reduction)
+    int ni = D.nin;
     for (int l = 0; l < L; ++l) {
         const int no = D.nodes[l];
-        if (l > 0) {
-            const int ni = D.nodes[l - 1];
-            const float* Wl = A.W + D.woff[l];
-            const float* bl = A.W + D.boff[l];
-            float* xr = row + loff[l];
-#pragma unroll
-            for (int m = 0; m < MW; ++m)
-                if (m < ni) { xr[m] = x[m]; xr[ni + m] = xd[m]; }
-#pragma unroll
-            for (int n = 0; n < MW; ++n) { z[n] = n < no ? bl[n] : 0.0f; zd[n] = 0.0f; }
-#pragma unroll
-            for (int m = 0; m < MW; ++m) {
-                if (m < ni) {
-                    const float* col = Wl + m * no;
-#pragma unroll
-                    for (int n = 0; n < MW; ++n)
-                        if (n < no) { z[n] += col[n] * x[m]; zd[n] += col[n] * xd[m]; }
-                }
-            }
-        }
-        if (l == L - 1) break;
-#pragma unroll
-        for (int n = 0; n < MW; ++n) {
-            float av = 0.0f, dv = 0.0f, d2 = 0.0f;
-            if (n < no) act_f<true>(D.neuron, c, z[n], av, dv, d2);
-            x[n] = av;
-            xd[n] = dv * zd[n];
-#pragma unroll
-            for (int q = 0; q < kMaxLayers - 1; ++q)
-                if (q == l) { dz[q][n] = dv; q2[q][n] = d2 * zd[n]; }
-        }
-    }
-    // backward: output layer is linear; abar = gE outw, adbar = outw (with v)
-    float ab[MW], adb[MW];
-#pragma unroll
-    for (int n = 0; n < MW; ++n) {
-        ab[n] = n == 0 ? gE * D.outw : 0.0f;
-        adb[n] = (n == 0 && V) ? D.outw : 0.0f;
-    }
-    for (int l = L - 1; l >= 0; --l) {
-        const int no = D.nodes[l];
-        const int ni = l == 0 ? D.nin : D.nodes[l - 1];
-        float zb[MW], zdb[MW];
-#pragma unroll
-        for (int n = 0; n < MW; ++n) {
-            float dd = 1.0f, qq = 0.0f;
-#pragma unroll
-            for (int q = 0; q < kMaxLayers - 1; ++q)
-                if (q == l && l < L - 1) { dd = dz[q][n]; qq = q2[q][n]; }
-            zdb[n] = adb[n] * dd;
-            zb[n] = ab[n] * dd + adb[n] * qq;
-        }
-        float* br = row + loff[l] + 2 * ni;
-#pragma unroll
-        for (int n = 0; n < MW; ++n)
-            if (n < no) { br[n] = zb[n]; br[no + n] = zdb[n]; }
-        if (l == 0) break;
         const float* Wl = A.W + D.woff[l];
+        const float* bl = A.W + D.boff[l];
+        const bool last = l == L - 1;
+        if (!last) {
+            float z[kBOut], zd[kBOut];
 #pragma unroll
-        for (int m = 0; m < MW; ++m) {
-            float s1 = 0.0f, s2 = 0.0f;
-            if (m < ni) {
+            for (int j = 0; j < kBOut; ++j) {
+                const int n = q + kBLanes * j;
+                z[j] = n < no ? bl[n] : 0.0f;
+                zd[j] = 0.0f;
+            }
+            for (int m = 0; m < ni; ++m) {
+                float x, xd;
+                if (l == 0) { x = iw[m] * (G[m] + ib[m]); xd = V ? iw[m] * V[m] : 0.0f; }
+                else { x = sx[sub][m]; xd = sxd[sub][m]; }
                 const float* col = Wl + m * no;
 #pragma unroll
-                for (int n = 0; n < MW; ++n)
-                    if (n < no) { s1 += col[n] * zb[n]; s2 += col[n] * zdb[n]; }
+                for (int j = 0; j < kBOut; ++j) {
+                    const int n = q + kBLanes * j;
+                    if (n < no) { z[j] += col[n] * x; zd[j] += col[n] * xd; }
+                }
             }
-            ab[m] = s1;
-            adb[m] = s2;
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < kBOut; ++j) {
+                const int n = q + kBLanes * j;
+                float av = 0.0f, dv = 0.0f, d2 = 0.0f;
+                if (n < no) act_f<true>(D.neuron, c, z[j], av, dv, d2);
+                if (n < no) {
+                    sx[sub][n] = av;
+                    sxd[sub][n] = dv * zd[j];
+                    if (valid) {
+                        row[loff[l + 1] + n] = av;
+                        row[loff[l + 1] + no + n] = dv * zd[j];
+                    }
+                }
+#pragma unroll
+                for (int p = 0; p < kMaxLayers - 1; ++p)
+                    if (p == l) { dz[p][j] = dv; qz[p][j] = d2 * zd[j]; }
+            }
+            __syncthreads();
+        } else {
+            // linear output neuron: only its pre-activation tangent matters below
+            (void)zo; (void)zdo;
         }
+        ni = no;
+    }
+    // backward.  Output layer (linear, one neuron): zbar = gE outw, zdbar = outw (with v)
+    const float zbL = gE * D.outw, zdbL = V ? D.outw : 0.0f;
+    if (valid && q == 0) {
+        const int nprev = L > 1 ? D.nodes[L - 2] : D.nin;
+        row[loff[L - 1] + 2 * nprev] = zbL;
+        row[loff[L - 1] + 2 * nprev + 1] = zdbL;
+    }
+    if (L == 1) return;
+    const float* WL = A.W + D.woff[L - 1];
+    // abar, adbar of the last hidden layer, then zbar/zdbar
+    float zb[kBOut], zdb[kBOut];
+    {
+        const int nl = D.nodes[L - 2];
+#pragma unroll
+        for (int j = 0; j < kBOut; ++j) {
+            const int m = q + kBLanes * j;
+            float d = 0.0f, qq = 0.0f;
+#pragma unroll
+            for (int p = 0; p < kMaxLayers - 1; ++p)
+                if (p == L - 2) { d = dz[p][j]; qq = qz[p][j]; }
+            const float ab = m < nl ? WL[m] * zbL : 0.0f, adb = m < nl ? WL[m] * zdbL : 0.0f;
+            zdb[j] = adb * d;
+            zb[j] = ab * d + adb * qq;
+        }
+    }
+    for (int l = L - 2; l >= 0; --l) {
+        const int no = D.nodes[l];
+        const int nprev = l == 0 ? D.nin : D.nodes[l - 1];
+        const int base = loff[l] + 2 * nprev;
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < kBOut; ++j) {
+            const int n = q + kBLanes * j;
+            if (n < no) {
+                sx[sub][n] = zb[j];
+                sxd[sub][n] = zdb[j];
+                if (valid) { row[base + n] = zb[j]; row[base + no + n] = zdb[j]; }
+            }
+        }
+        __syncthreads();
+        if (l == 0) break;
+        const float* Wl = A.W + D.woff[l];
+        float nb[kBOut], ndb[kBOut];
+#pragma unroll
+        for (int j = 0; j < kBOut; ++j) {
+            const int m = q + kBLanes * j;
+            float s1 = 0.0f, s2 = 0.0f;
+            if (m < nprev) {
+                const float* col = Wl + m * no;
+                for (int n = 0; n < no; ++n) { s1 += col[n] * sx[sub][n]; s2 += col[n] * sxd[sub][n]; }
+            }
+            float d = 0.0f, qq = 0.0f;
+#pragma unroll
+            for (int p = 0; p < kMaxLayers - 1; ++p)
+                if (p == l - 1) { d = dz[p][j]; qq = qz[p][j]; }
+            ndb[j] = s2 * d;
+            nb[j] = s1 * d + s2 * qq;
+        }
+#pragma unroll
+        for (int j = 0; j < kBOut; ++j) { zb[j] = nb[j]; zdb[j] = ndb[j]; }
     }
 }
 
@@ -169,52 +198,80 @@ struct GradArgs {
     const DevModel* dm;
     const float* scr;
     const int* spcount;
-    int S;
     int64_t base[kMaxSpecies];
-    double* grad;
+    const int4* tiles;        // (species, layer, n0, m0); m0 < 0: bias tile
+    int nsplit;
+    double* part;             // [nsplit][nparams]
 };
 
-// one thread per parameter; sums over the atoms of its species in rank order (FP64)
-__global__ __launch_bounds__(256) void k_grad_reduce(GradArgs A, int ptot) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= ptot) return;
-    int a = 0;
-    while (a + 1 < A.S && p >= A.dm->net[a + 1].poff) ++a;
+// One block per (16 x 16 output tile, atom split): sums over the split's atoms
+// of zbar (x) x + zdbar (x) xd, staged through LDS 64 atoms at a time; FP32
+// within a 64-atom chunk, FP64 across chunks.
+__global__ __launch_bounds__(256) void k_grad_tiles(GradArgs A, int nparams) {
+    __shared__ float sz[64][17], szd[64][17], sxx[64][17], sxd[64][17];
+    const int4 tl = A.tiles[blockIdx.x];
+    const int split = blockIdx.y;
+    const int a = tl.x, l = tl.y, n0 = tl.z, m0 = tl.w;
     const NetDesc& D = A.dm->net[a];
-    const int loc = p - D.poff;
     const int stride = bwd2_stride(D);
-    // locate layer and (n, m)
-    int l = 0, cnt = 0, prev = D.nin;
-    bool bias = false;
-    int n = 0, m = 0;
-    for (int q = 0; q < D.nlayer; ++q) {
-        if (loc < cnt + D.nodes[q]) { bias = true; l = q; n = loc - cnt; break; }
-        cnt += D.nodes[q];
-    }
-    int lo = 0;
-    if (!bias) {
-        for (int q = 0; q < D.nlayer; ++q) {
-            const int sz = D.nodes[q] * prev;
-            if (loc < cnt + sz) { l = q; const int r = loc - cnt; m = r / D.nodes[q]; n = r % D.nodes[q]; break; }
-            cnt += sz;
-            prev = D.nodes[q];
-        }
-    }
-    int ni = D.nin;
-    for (int q = 0; q < l; ++q) { lo += 2 * (ni + D.nodes[q]); ni = D.nodes[q]; }
+    int lo = 0, ni = D.nin;
+    for (int p = 0; p < l; ++p) { lo += 2 * (ni + D.nodes[p]); ni = D.nodes[p]; }
     const int no = D.nodes[l];
-    const float* base = A.scr + A.base[a] + lo;
     const int na = A.spcount[a];
+    const int per = (na + A.nsplit - 1) / A.nsplit;
+    const int r0 = split * per, r1 = min(na, r0 + per);
+    const float* base = A.scr + A.base[a] + lo;
+    const int t = threadIdx.x, tn = t / 16, tm = t % 16;
+    const bool bias = m0 < 0;
     double acc = 0.0;
-    if (bias) {
-        for (int r = 0; r < na; ++r) acc += (double)base[(int64_t)r * stride + 2 * ni + n];
-    } else {
-        for (int r = 0; r < na; ++r) {
-            const float* rr = base + (int64_t)r * stride;
-            acc += (double)rr[2 * ni + n] * (double)rr[m] + (double)rr[2 * ni + no + n] * (double)rr[ni + m];
+    for (int c0 = r0; c0 < r1; c0 += 64) {
+        for (int e = t; e < 64 * 16; e += 256) {
+            const int r = e / 16, k = e % 16;
+            const bool ok = c0 + r < r1;
+            const float* rr = base + (int64_t)(c0 + r) * stride;
+            const int n = n0 + k, m = m0 + k;
+            sz[r][k] = ok && n < no ? rr[2 * ni + n] : 0.0f;
+            szd[r][k] = ok && n < no ? rr[2 * ni + no + n] : 0.0f;
+            sxx[r][k] = ok && !bias && m < ni ? rr[m] : 0.0f;
+            sxd[r][k] = ok && !bias && m < ni ? rr[ni + m] : 0.0f;
         }
+        __syncthreads();
+        float s = 0.0f;
+        if (bias) {
+            if (tm == 0)
+                for (int r = 0; r < 64; ++r) s += sz[r][tn];
+        } else {
+            for (int r = 0; r < 64; ++r) s += sz[r][tn] * sxx[r][tm] + szd[r][tn] * sxd[r][tm];
+        }
+        acc += (double)s;
+        __syncthreads();
     }
-    A.grad[p] = acc;
+    const int n = n0 + tn;
+    if (n >= no) return;
+    int p;
+    if (bias) {
+        if (tm != 0) return;
+        int bo = 0;
+        for (int q = 0; q < l; ++q) bo += D.nodes[q];
+        p = D.poff + bo + n;
+    } else {
+        const int m = m0 + tm;
+        if (m >= ni) return;
+        int wo = 0, prev = D.nin;
+        for (int q = 0; q < D.nlayer; ++q) wo += D.nodes[q];
+        for (int q = 0; q < l; ++q) { wo += D.nodes[q] * prev; prev = D.nodes[q]; }
+        p = D.poff + wo + m * no + n;
+    }
+    A.part[(size_t)split * nparams + p] = acc;
+}
+
+// fixed-order sum over the atom splits
+__global__ void k_grad_combine(int nparams, int nsplit, const double* __restrict__ part, double* __restrict__ grad) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nparams) return;
+    double s = 0.0;
+    for (int k = 0; k < nsplit; ++k) s += part[(size_t)k * nparams + p];
+    grad[p] = s;
 }
 
 __global__ void k_flags(int N, const int* __restrict__ species, int s, int* __restrict__ flag) {
@@ -329,26 +386,51 @@ hipError_t net_bwd2(ann_ctx* ctx, const float* G, const float* v, const double* 
         A.base[s] = b;
         b += (int64_t)ctx->hm_counts[s] * bwd2_stride(ctx->hm.net[s]);
     }
-    const int nb = (A.N + 255) / 256;
-    const int w = max_width(ctx->hm);
     {
         Launch L(ctx, "net_bwd2");
-        if (w <= 16) k_net_bwd2<16><<<nb, 256, 0, ctx->stream>>>(A);
-        else if (w <= 32) k_net_bwd2<32><<<nb, 256, 0, ctx->stream>>>(A);
-        else if (w <= 48) k_net_bwd2<48><<<nb, 256, 0, ctx->stream>>>(A);
-        else k_net_bwd2<64><<<nb, 256, 0, ctx->stream>>>(A);
+        k_net_bwd2<<<(A.N + 3) / 4, 64, 0, ctx->stream>>>(A);
     }
     if ((e = hipGetLastError()) != hipSuccess) return e;
+    // tiles of every species and layer (biases as their own tiles)
+    if (ctx->grad_tiles.empty()) {
+        for (int s = 0; s < ctx->S; ++s) {
+            const NetDesc& D = ctx->hm.net[s];
+            int ni = D.nin;
+            for (int l = 0; l < D.nlayer; ++l) {
+                const int no = D.nodes[l];
+                for (int n0 = 0; n0 < no; n0 += 16) {
+                    ctx->grad_tiles.insert(ctx->grad_tiles.end(), {s, l, n0, -1});
+                    for (int m0 = 0; m0 < ni; m0 += 16) ctx->grad_tiles.insert(ctx->grad_tiles.end(), {s, l, n0, m0});
+                }
+                ni = no;
+            }
+        }
+    }
+    const int ntiles = (int)ctx->grad_tiles.size() / 4;
+    const int nsplit = 8;
+    if (!ctx->d_grad_tiles) {
+        if ((e = hipMalloc(&ctx->d_grad_tiles, sizeof(int) * ctx->grad_tiles.size())) != hipSuccess) return e;
+        if ((e = hipMemcpy(ctx->d_grad_tiles, ctx->grad_tiles.data(), sizeof(int) * ctx->grad_tiles.size(),
+                           hipMemcpyHostToDevice)) != hipSuccess)
+            return e;
+    }
+    if (ctx->gsplit_cap < (int64_t)nsplit * ctx->nparams) {
+        if (ctx->d_gsplit) hipFree(ctx->d_gsplit);
+        if ((e = hipMalloc(&ctx->d_gsplit, sizeof(double) * nsplit * ctx->nparams)) != hipSuccess) return e;
+        ctx->gsplit_cap = (int64_t)nsplit * ctx->nparams;
+    }
     GradArgs R{};
     R.dm = ctx->dm;
     R.scr = ctx->d_scr;
     R.spcount = ctx->d_spcount;
-    R.S = ctx->S;
     for (int s = 0; s < kMaxSpecies; ++s) R.base[s] = A.base[s];
-    R.grad = grad;
+    R.tiles = reinterpret_cast<const int4*>(ctx->d_grad_tiles);
+    R.nsplit = nsplit;
+    R.part = ctx->d_gsplit;
     {
         Launch L(ctx, "grad_reduce");
-        k_grad_reduce<<<(ctx->nparams + 255) / 256, 256, 0, ctx->stream>>>(R, ctx->nparams);
+        k_grad_tiles<<<dim3(ntiles, nsplit), 256, 0, ctx->stream>>>(R, ctx->nparams);
+        k_grad_combine<<<(ctx->nparams + 255) / 256, 256, 0, ctx->stream>>>(ctx->nparams, nsplit, ctx->d_gsplit, grad);
     }
     return hipGetLastError();
 }
