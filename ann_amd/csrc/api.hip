@@ -291,7 +291,8 @@ extern "C" int ann_destroy(ann_ctx* ctx) {
                     ctx->d_bin_start, ctx->d_bin_atoms, ctx->d_keys, ctx->d_cnt, ctx->d_flag, ctx->nl.off,
                     ctx->nl.idx, ctx->nl.img, ctx->nl.rev, ctx->nl.geo, ctx->nl.nsp, ctx->d_G, ctx->d_dEdG,
                     ctx->d_Eatom, ctx->d_P, ctx->d_u, ctx->d_v, ctx->d_F, ctx->d_E, ctx->d_gE, ctx->d_atomvir,
-                    ctx->d_scr, ctx->d_gpart, ctx->d_rank, ctx->d_spcount, ctx->d_perm, ctx->d_tiles, ctx->d_spend, ctx->d_grad_tiles, ctx->d_gsplit};
+                    ctx->d_scr, ctx->d_gpart, ctx->d_rank, ctx->d_spcount, ctx->d_perm, ctx->d_tiles, ctx->d_spend, ctx->d_grad_tiles, ctx->d_gsplit,
+                    ctx->d_chunks, ctx->d_chunk_off, ctx->d_part, ctx->d_pos_ref, ctx->d_keatom, ctx->d_md_flag};
     for (void* p : ptrs)
         if (p) hipFree(p);
     for (hipEvent_t e : ctx->evpool) hipEventDestroy(e);
@@ -376,8 +377,33 @@ extern "C" int ann_set_batch(ann_ctx* ctx, const ann_batch* b) {
         ANN_HIP(dalloc(ctx->nl.off, N + 1));
         ctx->atom_cap = N;
     }
+    // chunks of the per-structure sums
+    std::vector<int> chunks, chunk_off(S + 1, 0);
+    for (int s = 0; s < S; ++s) {
+        chunk_off[s] = (int)chunks.size() / 3;
+        for (int a = b->atom_off[s]; a < b->atom_off[s + 1]; a += kSumChunk) {
+            chunks.push_back(s);
+            chunks.push_back(a);
+            chunks.push_back(std::min(a + kSumChunk, b->atom_off[s + 1]));
+        }
+    }
+    const int nch = (int)chunks.size() / 3;
+    chunk_off[S] = nch;
+    if (nch > ctx->chunk_cap || S + 1 > ctx->chunk_cap) {
+        const int cap = std::max(nch, S + 1);
+        ANN_HIP(dalloc(ctx->d_chunks, 3 * (size_t)cap));
+        ANN_HIP(dalloc(ctx->d_chunk_off, (size_t)cap + 1));
+        ANN_HIP(dalloc(ctx->d_part, (size_t)kSumQ * cap));
+        ctx->chunk_cap = cap;
+    }
+    ctx->nchunks = nch;
+    ANN_HIP(hipMemcpyAsync(ctx->d_chunks, chunks.data(), sizeof(int) * chunks.size(), hipMemcpyHostToDevice,
+                           ctx->stream));
+    ANN_HIP(hipMemcpyAsync(ctx->d_chunk_off, chunk_off.data(), sizeof(int) * (S + 1), hipMemcpyHostToDevice,
+                           ctx->stream));
     ANN_HIP(hipMemsetAsync(ctx->d_cnt, 0, sizeof(int) * (N + 1), ctx->stream));
     ANN_HIP(struc_index(ctx));
+    ANN_HIP(hipStreamSynchronize(ctx->stream));
     ctx->have_batch = true;
     ctx->have_nlist = false;
     ctx->G_valid = false;
@@ -398,6 +424,35 @@ extern "C" int ann_neighbors(ann_ctx* ctx, double rc) {
     ctx->have_nlist = true;
     ctx->G_valid = false;
     ctx->nl_rc = r;
+    // reference positions of the skin criterion
+    if (ctx->natoms > ctx->md_cap) {
+        ANN_HIP(dalloc(ctx->d_pos_ref, 3 * (size_t)ctx->natoms));
+        ANN_HIP(dalloc(ctx->d_keatom, (size_t)ctx->natoms));
+        if (!ctx->d_md_flag) ANN_HIP(hipMalloc(&ctx->d_md_flag, sizeof(int)));
+        ctx->md_cap = ctx->natoms;
+    }
+    ANN_HIP(hipMemcpyAsync(ctx->d_pos_ref, ctx->d_pos, sizeof(double) * 3 * ctx->natoms, hipMemcpyDeviceToDevice,
+                           ctx->stream));
+    ANN_HIP(hipMemsetAsync(ctx->d_md_flag, 0, sizeof(int), ctx->stream));
+    return ANN_OK;
+}
+
+extern "C" int ann_refresh_geometry(ann_ctx* ctx) {
+    CHECK_CTX(ctx);
+    if (!ctx->have_nlist) return fail(ANN_ERR_STATE, "build the neighbour list first (ann_neighbors)");
+    hipError_t e = refresh_geometry(ctx);
+    if (e != hipSuccess) return fail(ANN_ERR_HIP, std::string("refresh_geometry: ") + hipGetErrorString(e));
+    ctx->G_valid = false;
+    return ANN_OK;
+}
+
+extern "C" int ann_wrap_positions(ann_ctx* ctx) {
+    CHECK_CTX(ctx);
+    if (!ctx->have_batch) return fail(ANN_ERR_STATE, "no batch set");
+    hipError_t e = wrap_positions(ctx, const_cast<double*>(ctx->d_pos));
+    if (e != hipSuccess) return fail(ANN_ERR_HIP, std::string("wrap_positions: ") + hipGetErrorString(e));
+    ctx->have_nlist = false;      // image vectors of the old list refer to the unwrapped positions
+    ctx->G_valid = false;
     return ANN_OK;
 }
 
@@ -528,7 +583,34 @@ extern "C" int ann_energy_forces(ann_ctx* ctx, int form, int with_evac, double* 
     RET_HIP(net_fwd(ctx, form, ctx->d_G, ctx->d_Eatom, F || V ? ctx->d_dEdG : nullptr), "net_fwd");
     if (F || V) RET_HIP(symm_bwd(ctx, form, ctx->d_dEdG, ctx->d_P), "symm_bwd");
     if (F || V) RET_HIP(force_reduce(ctx, ctx->d_P, ctx->d_Eatom, with_evac, F, E, V), "force_reduce");
-    else if (E) RET_HIP(force_reduce(ctx, ctx->d_P, ctx->d_Eatom, with_evac, nullptr, E, nullptr), "energy");
+    else if (E) RET_HIP(struc_sums(ctx, ctx->d_Eatom, with_evac, nullptr, nullptr, E, nullptr, nullptr), "energy");
+    return ANN_OK;
+}
+
+extern "C" int ann_md_step(ann_ctx* ctx, int form, double dt, const double* mass, double* vel, double* F,
+                           double* E, double* KE, double* V, int* moved) {
+    int rc = need_nlist(ctx); if (rc) return rc;
+    if ((rc = check_form(form))) return rc;
+    if (!mass || !vel || !F) return fail(ANN_ERR_ARG, "null mass, velocity or force array");
+    if (!(dt > 0.0)) return fail(ANN_ERR_ARG, "time step must be positive");
+    RET_HIP(ensure_work(ctx), "allocation");
+    const double skin = std::max(ctx->nl_rc - (double)ctx->hm.rc, 0.0);
+    double* pos = const_cast<double*>(ctx->d_pos);
+    RET_HIP(md_kick_drift(ctx, dt, mass, F, vel, pos, 0.25 * skin * skin), "md_kick_drift");
+    RET_HIP(refresh_geometry(ctx), "refresh_geometry");
+    ctx->G_valid = false;
+    RET_HIP(symm_fwd(ctx, form, ctx->d_G), "symm_fwd");
+    RET_HIP(net_fwd(ctx, form, ctx->d_G, ctx->d_Eatom, ctx->d_dEdG), "net_fwd");
+    RET_HIP(symm_bwd(ctx, form, ctx->d_dEdG, ctx->d_P), "symm_bwd");
+    RET_HIP(force_reduce(ctx, ctx->d_P, ctx->d_Eatom, 1, F, E, V), "force_reduce");
+    RET_HIP(md_kick(ctx, dt, mass, F, vel), "md_kick");
+    if (KE) RET_HIP(struc_sums(ctx, nullptr, 0, nullptr, ctx->d_keatom, nullptr, nullptr, KE), "kinetic energy");
+    if (moved) {
+        int flag = 0;
+        RET_HIP(hipMemcpyAsync(&flag, ctx->d_md_flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream), "flag");
+        RET_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
+        *moved = flag;
+    }
     return ANN_OK;
 }
 

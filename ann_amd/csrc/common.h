@@ -20,6 +20,8 @@ constexpr int kMaxFunc = 32;       // functions per basis block
 constexpr int kMaxUnionEta = 4;    // distinct angular widths per centre species (fast form)
 constexpr int kGridG = 4;          // fast angular tile: widths per block
 constexpr int kGridK = 8;          // fast angular tile: (lambda, zeta) combinations per block
+constexpr int kSumChunk = 4096;    // atoms per block of the per-structure sums
+constexpr int kSumQ = 11;          // summed quantities: energy, virial[9], kinetic energy
 
 // per (centre, neighbour) radial block
 struct RadBlock {
@@ -175,6 +177,16 @@ struct ann_ctx {
     int64_t gpart_cap = 0;
     int64_t work_atoms = 0, work_pairs = 0;
     int adam_t = 0;
+    // per-structure sums: structures split into chunks of at most kSumChunk atoms
+    int nchunks = 0, chunk_cap = 0;
+    int* d_chunks = nullptr;          // [nchunks][3]: structure, first atom, end atom
+    int* d_chunk_off = nullptr;       // [S + 1]: first chunk of each structure
+    double* d_part = nullptr;         // [nchunks][kSumQ] partial sums
+    // molecular dynamics
+    double* d_pos_ref = nullptr;      // positions at the last neighbour build
+    double* d_keatom = nullptr;       // per-atom kinetic energy
+    int* d_md_flag = nullptr;         // an atom moved more than skin / 2 since the build
+    int md_cap = 0;
     // profiling
     bool prof = false;
     std::map<std::string, ann::Timer> timers;
@@ -211,4 +223,11 @@ hipError_t net_fwd(ann_ctx* ctx, int form, const float* G, float* Eatom, float* 
 hipError_t net_bwd2(ann_ctx* ctx, const float* G, const float* v, const double* gE, double* grad);
 hipError_t force_reduce(ann_ctx* ctx, const float* P, const float* Eatom, int with_evac, double* F,
                         double* E, double* V);
+hipError_t struc_sums(ann_ctx* ctx, const float* Eatom, int with_evac, const double* avir, const double* keatom,
+                      double* E, double* V, double* KE);
+hipError_t refresh_geometry(ann_ctx* ctx);
+hipError_t wrap_positions(ann_ctx* ctx, double* pos);
+hipError_t md_kick_drift(ann_ctx* ctx, double dt, const double* mass, const double* F, double* vel, double* pos,
+                         double lim2);
+hipError_t md_kick(ann_ctx* ctx, double dt, const double* mass, const double* F, double* vel);
 }  // namespace ann

@@ -366,6 +366,66 @@ hipError_t neighbors_build(ann_ctx* ctx, double rc) {
     return hipSuccess;
 }
 
+// Pair geometry of every slot from the current positions and the list's image
+// vectors (verlet-list reuse between rebuilds, the geometry of
+// NNP::symm(..., verlet::List) src/nnp/nnp.cpp:389-395): same FP64 operation
+// order as the build, so an unchanged position set reproduces the build's
+// geometry bit for bit.  One wavefront per atom, lanes over its slots.
+__global__ __launch_bounds__(256) void k_geom(int N, const double* __restrict__ pos, const int* __restrict__ sof,
+                                              const double* __restrict__ cells, const int* __restrict__ off,
+                                              const int* __restrict__ idx, const uint32_t* __restrict__ img,
+                                              float4* __restrict__ geo) {
+    const int i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
+    if (i >= N) return;
+    const double* R = cells + 9 * sof[i];
+    for (int s = off[i] + lane; s < off[i + 1]; s += kWave) {
+        int n0, n1, n2;
+        unpack_img(img[s], n0, n1, n2);
+        double x, y, z, d2;
+        pair_vec(pos, i, idx[s], R, n0, n1, n2, x, y, z, d2);
+        geo[s] = make_float4((float)x, (float)y, (float)z, (float)sqrt(d2));
+    }
+}
+
+hipError_t refresh_geometry(ann_ctx* ctx) {
+    const int N = ctx->natoms;
+    if (N == 0) return hipSuccess;
+    Launch L(ctx, "nb_geometry");
+    k_geom<<<(N + 3) / 4, 256, 0, ctx->stream>>>(N, ctx->d_pos, ctx->d_struc_of, ctx->d_cells, ctx->nl.off,
+                                                 ctx->nl.idx, ctx->nl.img, ctx->nl.geo);
+    return hipGetLastError();
+}
+
+// Positions back into the cell, x <- R (f - floor f) with f = R^-1 x
+// (Cell::returnToCell, src/format/xyz_struc.cpp:176-181).
+__global__ void k_wrap(int N, double* __restrict__ pos, const int* __restrict__ sof,
+                       const double* __restrict__ cells) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const double* R = cells + 9 * sof[i];
+    const double a = R[0], b = R[1], c = R[2], d = R[3], e = R[4], f = R[5], g = R[6], h = R[7], k = R[8];
+    const double A = e * k - f * h, B = -(d * k - f * g), C = d * h - e * g;
+    const double id = 1.0 / (a * A + b * B + c * C);
+    const double Ri[9] = {A * id, -(b * k - c * h) * id, (b * f - c * e) * id,
+                          B * id, (a * k - c * g) * id,  -(a * f - c * d) * id,
+                          C * id, -(a * h - b * g) * id, (a * e - b * d) * id};
+    const double x[3] = {pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]};
+    double fr[3];
+    for (int q = 0; q < 3; ++q) {
+        const double t = Ri[3 * q] * x[0] + Ri[3 * q + 1] * x[1] + Ri[3 * q + 2] * x[2];
+        fr[q] = t - floor(t);
+    }
+    for (int q = 0; q < 3; ++q) pos[3 * i + q] = R[3 * q] * fr[0] + R[3 * q + 1] * fr[1] + R[3 * q + 2] * fr[2];
+}
+
+hipError_t wrap_positions(ann_ctx* ctx, double* pos) {
+    const int N = ctx->natoms;
+    if (N == 0) return hipSuccess;
+    k_wrap<<<(N + 255) / 256, 256, 0, ctx->stream>>>(N, pos, ctx->d_struc_of, ctx->d_cells);
+    return hipGetLastError();
+}
+
 hipError_t struc_index(ann_ctx* ctx) {
     k_struc_of<<<ctx->nstruc, 256, 0, ctx->stream>>>(ctx->d_atom_off, ctx->nstruc, ctx->d_struc_of);
     return hipGetLastError();

@@ -264,16 +264,19 @@ __device__ __forceinline__ float pow_m1(float w, int zi, float zeta) {
 }
 
 // Regroup the neighbours of atom i by species (stable) and precompute the
-// per-neighbour factors; seg[] gives the species segments.
-__device__ void fast_load(const SymmArgs& A, const DevModel& M, int i, int a, int o, int n, FastSmem& sm,
-                          bool need_deriv) {
+// per-neighbour factors; seg[] gives the species segments.  Entries beyond the
+// cutoff (a list built with rc + skin) are dropped here; their adjoint slots
+// are written as zero when zero_P is set.  Returns the number kept.
+__device__ int fast_load(const SymmArgs& A, const DevModel& M, int i, int a, int o, int n, FastSmem& sm,
+                         bool need_deriv, bool zero_P) {
     const int lane = threadIdx.x, S = M.S, U = M.nunion[a];
+    const float rc = M.rc;
     int cnt[kMaxSpecies];
 #pragma unroll
     for (int b = 0; b < kMaxSpecies; ++b) cnt[b] = 0;
     for (int q = 0; q < n; q += kWave) {
         const int t = q + lane;
-        const int sp = t < n ? (int)A.nsp[o + t] : -1;
+        const int sp = (t < n && A.geo[o + t].w <= rc) ? (int)A.nsp[o + t] : -1;
 #pragma unroll
         for (int b = 0; b < kMaxSpecies; ++b)
             if (b < S) cnt[b] += __popcll(__ballot(sp == b));
@@ -295,7 +298,14 @@ __device__ void fast_load(const SymmArgs& A, const DevModel& M, int i, int a, in
     const uint64_t lt = (1ull << lane) - 1ull;
     for (int q = 0; q < n; q += kWave) {
         const int t = q + lane;
-        const int sp = t < n ? (int)A.nsp[o + t] : -1;
+        const float4 gq = t < n ? A.geo[o + t] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const bool keep = t < n && gq.w <= rc;
+        const int sp = keep ? (int)A.nsp[o + t] : -1;
+        if (zero_P && t < n && !keep) {
+            A.P[3 * ((size_t)o + t)] = 0.0f;
+            A.P[3 * ((size_t)o + t) + 1] = 0.0f;
+            A.P[3 * ((size_t)o + t) + 2] = 0.0f;
+        }
         int pos = 0;
 #pragma unroll
         for (int b = 0; b < kMaxSpecies; ++b) {
@@ -305,8 +315,7 @@ __device__ void fast_load(const SymmArgs& A, const DevModel& M, int i, int a, in
                 base[b] += __popcll(mk);
             }
         }
-        if (t < n) {
-            const float4 gq = A.geo[o + t];
+        if (keep) {
             const float r = gq.w, ir = 1.0f / r;
             sm.g[pos] = make_float4(gq.x * ir, gq.y * ir, gq.z * ir, ir);
             sm.r[pos] = r;
@@ -324,6 +333,7 @@ __device__ void fast_load(const SymmArgs& A, const DevModel& M, int i, int a, in
             if (need_deriv) sm.efd[pos] = make_float4(d[0], d[1], d[2], d[3]);
         }
     }
+    return acc;
 }
 
 // ---- forward / tangent over one grid block (unordered pairs, flattened) ----
@@ -580,7 +590,8 @@ __global__ __launch_bounds__(64) void k_fast(SymmArgs A) {
     const int i = blockIdx.x, lane = threadIdx.x;
     if (i >= A.N) return;
     const DevModel& M = *A.dm;
-    const int a = A.species[i], o = A.off[i], n = A.off[i + 1] - o;
+    const int a = A.species[i], o = A.off[i];
+    int n = A.off[i + 1] - o;
     const int S = M.S, NP = M.NP, ldg = A.ldg, maxn = A.maxn;
     FastSmem sm;
     sm.g = smem4;
@@ -593,7 +604,7 @@ __global__ __launch_bounds__(64) void k_fast(SymmArgs A) {
     sm.extra = reinterpret_cast<float*>(sm.seg + 16);
     float* rowbuf = sm.extra;                                   // [ldg] output row (modes 0, 2) / dE/dG (mode 1)
     float* sP = sm.extra + ldg;                                 // [maxn][3] (mode 1)
-    fast_load(A, M, i, a, o, n, sm, MODE != 0);
+    n = fast_load(A, M, i, a, o, n, sm, MODE != 0, MODE == 1);   // n: neighbours within rc
     if (MODE == 1) {
         for (int m = lane; m < ldg; m += kWave) rowbuf[m] = A.dEdG[(size_t)i * ldg + m];
         for (int t = lane; t < 3 * n; t += kWave) sP[t] = 0.0f;

@@ -54,7 +54,7 @@ EXPORTS = ["ann_last_error", "ann_create", "ann_destroy", "ann_set_stream", "ann
            "ann_set_scaling", "ann_set_batch", "ann_neighbors", "ann_nlist_size", "ann_nlist_copy",
            "ann_symm_fwd", "ann_net_fwd", "ann_symm_bwd", "ann_force_reduce", "ann_energy_forces",
            "ann_symm_tangent", "ann_net_bwd2", "ann_train_step", "ann_get_params", "ann_profile",
-           "ann_profile_get"]
+           "ann_profile_get", "ann_refresh_geometry", "ann_wrap_positions", "ann_md_step"]
 
 
 def lib():
@@ -218,6 +218,26 @@ class Engine:
         _check(lib().ann_train_step(self.h, form, _ptr(Eref.contiguous()), _ptr(Fref.contiguous()),
                                     C.c_double(wE), C.c_double(wF), C.c_double(gamma), _ptr(loss), _ptr(g)))
         return loss, g
+
+    # ------------------------------------------------------------ molecular dynamics
+    def refresh_geometry(self) -> None:
+        """Pair geometry from the current positions (self.d_pos) and the list's images."""
+        _check(lib().ann_refresh_geometry(self.h))
+
+    def wrap_positions(self) -> None:
+        """Return self.d_pos to the cell in place; the list must be rebuilt afterwards."""
+        _check(lib().ann_wrap_positions(self.h))
+
+    def md_step(self, dt: float, mass: torch.Tensor, vel: torch.Tensor, F: torch.Tensor, form: int = FAST,
+                E: Optional[torch.Tensor] = None, KE: Optional[torch.Tensor] = None,
+                V: Optional[torch.Tensor] = None, check: bool = False) -> bool:
+        """One velocity-Verlet step; updates self.d_pos, vel and F in place.
+        Returns True when an atom moved more than skin/2 since the last build
+        (only if check is set; that synchronises)."""
+        moved = C.c_int(0)
+        _check(lib().ann_md_step(self.h, form, C.c_double(dt), _ptr(mass), _ptr(vel), _ptr(F), _ptr(E), _ptr(KE),
+                                 _ptr(V), C.byref(moved) if check else None))
+        return bool(moved.value)
 
     # ------------------------------------------------------------ timing
     def profile(self, enable: bool = True) -> None:

@@ -138,6 +138,31 @@ int ann_force_reduce(ann_ctx* ctx, const float* d_P, const float* d_Eatom, int w
 int ann_energy_forces(ann_ctx* ctx, int form, int with_evac, double* d_E, double* d_F,
                       double* d_virial);
 
+/* ---- molecular dynamics either side of the E+F call ----
+ * A list built with ann_neighbors(ctx, rc + skin) stays valid while no atom
+ * moves more than skin/2 (verlet::List reuse, src/nnp/nnp.cpp:377-427 and
+ * src/torch/torch.cpp:272-276 rebuilding every `stride` steps); entries
+ * beyond the model cutoff contribute nothing. */
+/* pair geometry of every slot from the batch's current positions and the
+ * list's image vectors */
+int ann_refresh_geometry(ann_ctx* ctx);
+/* batch positions returned to the cell, x <- R frac(R^-1 x) (WRITES the batch
+ * positions; Cell::returnToCell, src/format/xyz_struc.cpp:176-181).  The
+ * list's image vectors refer to the old positions: rebuild afterwards. */
+int ann_wrap_positions(ann_ctx* ctx);
+/* one velocity-Verlet step (Verlet::compute, src/torch/integrator.cpp:177-200)
+ * with PotNNPE::compute energies (src/torch/pot_nnpe.cpp:166-287, vacuum
+ * energies included): v += dt/2 F/m, x += v dt (the batch positions are
+ * WRITTEN, not wrapped), geometry refresh, E / F / virial at the new positions,
+ * v += dt/2 F/m, kinetic energy per structure.  d_mass [N], d_vel / d_F
+ * [N][3]; d_F holds the forces at the current positions on entry and the new
+ * ones on exit.  d_E, d_KE [nstruc], d_virial [nstruc][9] may be NULL.  If
+ * `moved` is not NULL the call synchronises and sets *moved = 1 when an atom
+ * has moved more than skin/2 since the last build (rebuild before the next
+ * step). */
+int ann_md_step(ann_ctx* ctx, int form, double dt, const double* d_mass, double* d_vel, double* d_F,
+                double* d_E, double* d_KE, double* d_virial, int* moved);
+
 /* ---- second backward (weight gradient of a force loss) ---- */
 /* v_i = sum_s dG_i/dr_ij . (u_i - u_j): forward-mode tangent of the basis
  * along the per-atom field u [N][3]; v [N][ldg] */

@@ -1,0 +1,147 @@
+"""GPU molecular dynamics either side of the E+F path: neighbour lists built
+with rc + skin (verlet::List reuse, src/nnp/nnp.cpp:377-427), the geometry
+refresh from image vectors, velocity Verlet (src/torch/integrator.cpp:177-200)
+with a rebuild every `stride` steps (src/torch/torch.cpp:272-276), and the
+chunked FP64 per-structure sums of large structures.
+
+Tolerances: energies <= 1e-6 eV/atom and forces <= 1e-5 relative RMS against
+the FP64 oracle (as the E+F parity tests); geometry refresh bit-exact against a
+fresh build; MD trajectory against an FP64 oracle Verlet: positions <= 1e-6 A,
+velocities <= 1e-5 relative RMS after 20 steps.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SKIN = 0.5
+
+
+def _water(reps=1, jitter=0.05, seed=15):
+    from ann_amd import configs
+    return configs.model_water(), configs.struc_water_md(seed=seed, reps=reps, jitter=jitter)
+
+
+def _engine(model, batch, rc):
+    from ann_amd.gpu import Engine
+    eng = Engine(model, "cuda:0")
+    eng.set_batch(batch)
+    eng.neighbors(rc)
+    return eng
+
+
+@pytest.mark.parametrize("form", [0, 1])
+def test_skin_list_energy_forces_match_oracle(oracle_mod, form):
+    model, b = _water()
+    print(f"skin list E+F, form {form}, {b.ntotal} atoms", flush=True)
+    eng = _engine(model, b, model.rc + SKIN)
+    P, maxn = eng.nlist_size()
+    E, F, V = eng.energy_forces(form)
+    orc = oracle_mod.Oracle(model)
+    Eo, _, Fo, Vo = orc.compute(b.pos, b.cells[0], b.species)
+    de = abs(E.item() - Eo) / b.ntotal
+    rf = np.sqrt(((F.cpu().numpy() - Fo) ** 2).sum() / (Fo ** 2).sum())
+    rv = np.abs(V.cpu().numpy()[0] - Vo).max() / np.abs(Vo).max()
+    print(f"  slots {P} maxn {maxn} dE/atom={de:.3e} force relRMS={rf:.3e} virial rel={rv:.3e}", flush=True)
+    assert de <= 1e-6 and rf <= 1e-5 and rv <= 1e-4
+
+
+def test_refresh_geometry_is_bit_exact():
+    import torch
+    model, b = _water()
+    eng = _engine(model, b, model.rc + SKIN)
+    ref = eng.nlist()
+    rng = np.random.default_rng(5)
+    shift = torch.from_numpy(rng.uniform(-0.2, 0.2, size=(b.ntotal, 3))).to(eng.d_pos.device)
+    eng.d_pos += shift
+    eng.refresh_geometry()
+    got = eng.nlist()
+    # a fresh build at the moved positions, with a cutoff wide enough to hold every old slot
+    moved = b.__class__(b.natoms, eng.d_pos.cpu().numpy(), b.cells, b.species)
+    fresh = _engine(model, moved, model.rc + SKIN + 1.0).nlist()   # pairs move apart by < 0.7 A
+
+    def keyed(nl):
+        i = np.repeat(np.arange(b.ntotal), np.diff(nl["off"]))
+        return {(int(a), int(j), *map(int, n)): g for a, j, n, g in zip(i, nl["idx"], nl["img"], nl["geo"])}
+
+    kf = keyed(fresh)
+    kg = keyed(got)
+    assert set(keyed(ref)) == set(kg)
+    for k, g in kg.items():
+        assert np.array_equal(g, kf[k]), k
+
+
+def _verlet_oracle(orc, pos, cell, sp, vel, mass, dt, nsteps):
+    """Velocity Verlet of src/torch/integrator.cpp:177-200 on the FP64 oracle."""
+    E, _, F, _ = orc.compute(pos, cell, sp)
+    pos, vel = pos.copy(), vel.copy()
+    for _ in range(nsteps):
+        vel += 0.5 * dt * F / mass[:, None]
+        pos += vel * dt
+        E, _, F, _ = orc.compute(pos, cell, sp)
+        vel += 0.5 * dt * F / mass[:, None]
+    return pos, vel, E, F
+
+
+DT, NSTEPS, STRIDE = 0.02, 20, 10
+
+
+@pytest.fixture(scope="module")
+def md_case(oracle_mod):
+    model, b = _water()
+    rng = np.random.default_rng(8)
+    mass = model.masses[b.species]
+    vel0 = rng.normal(0.0, 0.02, size=(b.ntotal, 3)) / np.sqrt(mass)[:, None]
+    vel0 -= (mass[:, None] * vel0).sum(0) / mass.sum()
+    orc = oracle_mod.Oracle(model)
+    return model, b, mass, vel0, _verlet_oracle(orc, b.pos, b.cells[0], b.species, vel0, mass, DT, NSTEPS)
+
+
+@pytest.mark.parametrize("form", [0, 1])
+def test_md_trajectory_matches_oracle(md_case, form):
+    import torch
+    model, b, mass, vel0, (po, vo, Eo, Fo) = md_case
+    dt, nsteps, stride = DT, NSTEPS, STRIDE
+    print(f"MD form {form}: {b.ntotal} atoms, {nsteps} steps, rebuild every {stride}", flush=True)
+    eng = _engine(model, b, model.rc + SKIN)
+    dev = eng.d_pos.device
+    m = torch.from_numpy(mass).to(dev)
+    v = torch.from_numpy(vel0.copy()).to(dev)
+    E, F, _ = eng.energy_forces(form)
+    KE = torch.empty(1, dtype=torch.float64, device=dev)
+    for t in range(nsteps):
+        if t > 0 and t % stride == 0:
+            eng.wrap_positions()
+            eng.neighbors(model.rc + SKIN)
+        moved = eng.md_step(dt, m, v, F, form=form, E=E, KE=KE, check=True)
+        assert not moved
+    L = b.cells[0][0, 0]
+    d = eng.d_pos.cpu().numpy() - po
+    d -= L * np.round(d / L)                       # the GPU positions were wrapped at the rebuild
+    dv = np.sqrt(((v.cpu().numpy() - vo) ** 2).sum() / (vo ** 2).sum())
+    de = abs(E.item() - Eo) / b.ntotal
+    ke = 0.5 * (mass[:, None] * vo ** 2).sum()
+    print(f"  max |dx|={np.abs(d).max():.3e} vel relRMS={dv:.3e} dE/atom={de:.3e} "
+          f"KE {KE.item():.6e} vs {ke:.6e}", flush=True)
+    assert np.abs(d).max() <= 1e-6
+    assert dv <= 1e-5
+    assert de <= 1e-6
+    assert abs(KE.item() - ke) <= 1e-5 * ke
+
+
+def test_chunked_sums_of_a_replica():
+    """A 4x4x4 replica (16,384 atoms: four sum chunks) of a cell has 64 times
+    the cell's energy and virial, and every copy of an atom the cell's force."""
+    model, small = _water(reps=1, jitter=0.0)
+    _, big = _water(reps=4, jitter=0.0)
+    es = _engine(model, small, model.rc)
+    eb = _engine(model, big, model.rc)
+    Es, Fs, Vs = es.energy_forces(1)
+    Eb, Fb, Vb = eb.energy_forces(1)
+    Es, Eb = Es.item(), Eb.item()
+    Fs, Fb = Fs.cpu().numpy(), Fb.cpu().numpy().reshape(64, small.ntotal, 3)
+    rf = np.sqrt(((Fb - Fs[None]) ** 2).sum() / (64 * (Fs ** 2).sum()))
+    print(f"replica: E {Eb:.10e} vs 64 x {Es:.10e}; force relRMS {rf:.3e}", flush=True)
+    assert abs(Eb - 64 * Es) <= 1e-6 * big.ntotal
+    assert rf <= 1e-5
+    assert np.abs(Vb.cpu().numpy() - 64 * Vs.cpu().numpy()).max() <= 1e-4 * np.abs(64 * Vs.cpu().numpy()).max()
