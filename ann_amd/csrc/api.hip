@@ -197,6 +197,21 @@ static int upload_model(ann_ctx* ctx, const ann_model_desc* d) {
         else if (ab.ng != M.fast_ng || ab.nk != M.fast_nk) M.all_grid = 0;
     }
     if (M.fast_ng == 0) { M.fast_ng = 1; M.fast_nk = 1; }
+    // moment form: grid blocks with integer zeta in [1, kMomZ] and |lambda| <= 1
+    M.moment_z = 0;
+    if (M.all_grid) {
+        bool ok = true;
+        int z = 0;
+        for (int t = 0; t < S * NP; ++t) {
+            const AngBlock& ab = M.ang[t];
+            if (ab.n == 0) continue;
+            for (int k = 0; k < ab.nk; ++k) {
+                if (ab.zint[k] < 1 || ab.zint[k] > kMomZ || std::fabs(ab.lamh[k]) > 0.5f) ok = false;
+                z = std::max(z, ab.zint[k]);
+            }
+        }
+        if (ok && z > 0) M.moment_z = z;
+    }
     int maxin = 1;
     for (int a = 0; a < S; ++a) maxin = std::max(maxin, nin[a]);
     ctx->max_nin = maxin;
@@ -501,7 +516,8 @@ static int need_nlist(ann_ctx* ctx) {
 }
 
 static int check_form(int form) {
-    if (form != ANN_FORM_PLAIN && form != ANN_FORM_FAST) return fail(ANN_ERR_ARG, "invalid form");
+    if (form != ANN_FORM_PLAIN && form != ANN_FORM_FAST && form != ANN_FORM_TRIPLET)
+        return fail(ANN_ERR_ARG, "invalid form");
     return ANN_OK;
 }
 

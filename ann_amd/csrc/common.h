@@ -20,6 +20,7 @@ constexpr int kMaxFunc = 32;       // functions per basis block
 constexpr int kMaxUnionEta = 4;    // distinct angular widths per centre species (fast form)
 constexpr int kGridG = 4;          // fast angular tile: widths per block
 constexpr int kGridK = 8;          // fast angular tile: (lambda, zeta) combinations per block
+constexpr int kMomZ = 4;           // largest integer zeta of the moment form
 constexpr int kSumChunk = 4096;    // atoms per block of the per-structure sums
 constexpr int kSumQ = 11;          // summed quantities: energy, virial[9], kinetic energy
 
@@ -63,6 +64,7 @@ struct DevModel {
     NetDesc net[kMaxSpecies];
     int all_grid;                               // every angular block is grid-shaped
     int fast_ng, fast_nk;                       // the common grid shape (0 if blocks differ)
+    int moment_z;                               // largest zeta if the moment form applies, else 0
 };
 
 // device pointers of the variable-size tables

@@ -172,7 +172,7 @@ hipError_t net_fwd_mfma(ann_ctx* ctx, const float* G, float* Eatom, float* dEdG)
 bool mfma_net_supported(const DevModel& M);
 
 hipError_t net_fwd(ann_ctx* ctx, int form, const float* G, float* Eatom, float* dEdG) {
-    if (form == ANN_FORM_FAST && mfma_net_supported(ctx->hm)) return net_fwd_mfma(ctx, G, Eatom, dEdG);
+    if (form != ANN_FORM_PLAIN && mfma_net_supported(ctx->hm)) return net_fwd_mfma(ctx, G, Eatom, dEdG);
     return net_fwd_plain(ctx, G, Eatom, dEdG);
 }
 

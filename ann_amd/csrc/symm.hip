@@ -267,7 +267,7 @@ __device__ __forceinline__ float pow_m1(float w, int zi, float zeta) {
 // per-neighbour factors; seg[] gives the species segments.  Entries beyond the
 // cutoff (a list built with rc + skin) are dropped here; their adjoint slots
 // are written as zero when zero_P is set.  Returns the number kept.
-__device__ int fast_load(const SymmArgs& A, const DevModel& M, int i, int a, int o, int n, FastSmem& sm,
+__device__ __forceinline__ int fast_load(const SymmArgs& A, const DevModel& M, int i, int a, int o, int n, FastSmem& sm,
                          bool need_deriv, bool zero_P) {
     const int lane = threadIdx.x, S = M.S, U = M.nunion[a];
     const float rc = M.rc;
@@ -338,7 +338,7 @@ __device__ int fast_load(const SymmArgs& A, const DevModel& M, int i, int a, int
 
 // ---- forward / tangent over one grid block (unordered pairs, flattened) ----
 template <int NG, int NK, int MODE>
-__device__ void fast_block_flat(const AngBlock& B, const FastSmem& sm, int sb, int nb, int sc, int nc, bool tri,
+__device__ __forceinline__ void fast_block_flat(const AngBlock& B, const FastSmem& sm, int sb, int nb, int sc, int nc, bool tri,
                                 const float4* tv, float* out) {
     const int lane = threadIdx.x;
     float lamh[NK], zeta[NK];
@@ -434,7 +434,7 @@ __device__ void fast_block_flat(const AngBlock& B, const FastSmem& sm, int sb, i
 
 // ---- adjoint over one grid block: skewed owner/partner schedule ----
 template <int NG, int NK>
-__device__ void fast_block_adj(const AngBlock& B, const FastSmem& sm, int sb, int nb, int sc, int nc, bool tri,
+__device__ __forceinline__ void fast_block_adj(const AngBlock& B, const FastSmem& sm, int sb, int nb, int sc, int nc, bool tri,
                                const float* sd, float* sP) {
     const int lane = threadIdx.x;
     float lamh[NK], zeta[NK], dl[NG][NK];
@@ -534,7 +534,7 @@ __host__ __device__ inline bool fast_shape_ok(int ng, int nk) {
 // radial part, lane over the neighbours of one species segment, 16 functions at a time
 constexpr int kRadChunk = 16;
 template <int MODE>
-__device__ void fast_radial(const SymmArgs& A, const DevModel& M, int a, const FastSmem& sm, const float4* tv,
+__device__ __forceinline__ void fast_radial(const SymmArgs& A, const DevModel& M, int a, const FastSmem& sm, const float4* tv,
                             const float* sd, float* sP, float* out) {
     const int lane = threadIdx.x, S = M.S;
     for (int b = 0; b < S; ++b) {
@@ -558,10 +558,32 @@ __device__ void fast_radial(const SymmArgs& A, const DevModel& M, int a, const F
             }
             continue;
         }
+        if (rb.name != RAD_GAUSSIAN) {
+            // other families: one function at a time, plain wave sums
+            for (int m = 0; m < rb.n; ++m) {
+                float acc = 0.0f;
+                for (int J = s0 + lane; J < s1; J += kWave) {
+                    const float r = sm.r[J];
+                    float fc, gc, v, d;
+                    cutoff_fg(M.cutoff, M.rc, M.rci, M.pirc, r, fc, gc);
+                    radial_fd(rb.name, A.rad_eta[rb.par + m], A.rad_rs[rb.par + m], r, fc, gc, v, d);
+                    acc += MODE == 0 ? v : d * tv[J].w;
+                }
+                acc = wave_sum(acc);
+                if (lane == 0) out[rb.goff + m] = acc;
+            }
+            continue;
+        }
+        // GAUSSIAN: 16 functions per pass in registers, one transposing reduction
         for (int m0 = 0; m0 < rb.n; m0 += kRadChunk) {
-            float acc[kRadChunk];
+            float acc[kRadChunk], eta[kRadChunk], rs[kRadChunk];
 #pragma unroll
-            for (int m = 0; m < kRadChunk; ++m) acc[m] = 0.0f;
+            for (int m = 0; m < kRadChunk; ++m) {
+                acc[m] = 0.0f;
+                const bool in = m0 + m < rb.n;
+                eta[m] = in ? A.rad_eta[rb.par + m0 + m] : 0.0f;
+                rs[m] = in ? A.rad_rs[rb.par + m0 + m] : 0.0f;
+            }
             for (int J = s0 + lane; J < s1; J += kWave) {
                 const float r = sm.r[J];
                 const float w = MODE == 2 ? tv[J].w : 0.0f;
@@ -569,11 +591,8 @@ __device__ void fast_radial(const SymmArgs& A, const DevModel& M, int a, const F
                 cutoff_fg(M.cutoff, M.rc, M.rci, M.pirc, r, fc, gc);
 #pragma unroll
                 for (int m = 0; m < kRadChunk; ++m) {
-                    if (m0 + m < rb.n) {
-                        float v, d;
-                        radial_fd(rb.name, A.rad_eta[rb.par + m0 + m], A.rad_rs[rb.par + m0 + m], r, fc, gc, v, d);
-                        acc[m] += MODE == 0 ? v : d * w;
-                    }
+                    const float x = eta[m] * (r - rs[m]), e = __expf(-x * x);
+                    acc[m] += MODE == 0 ? e * fc : e * (gc - 2.0f * eta[m] * x * fc) * w;
                 }
             }
             int idx;
@@ -651,6 +670,322 @@ __global__ __launch_bounds__(64) void k_fast(SymmArgs A) {
     }
 }
 
+// =========================================================== MOMENT form ====
+//
+// For GAUSS angular blocks with integer zeta <= 4 the triplet sum factorises
+// exactly.  With e_u(r) = exp(-r^2/eta_u^2) fc(r) and c = r^_j . r^_k,
+//   (1/2 + lambda c / 2)^zeta = sum_m C(zeta,m) 2^(m-zeta) (lambda/2)^m c^m,
+//   c^m = sum_{|alpha|=m} (m!/alpha!) r^_j^alpha r^_k^alpha,
+// so a block (species b, species c, width u) needs only the per-species
+// moments T_b[u][alpha] = sum_{j in b} e_u(r_j) r^_j^alpha and
+// D_b[u] = sum_{j in b} e_u(r_j)^2:
+//   A_m = sum_{|alpha|=m} w_alpha T_b T_c                (b != c)
+//   A_m = (sum_{|alpha|=m} w_alpha T_b T_b - D_b) / 2    (b == c: pairs j < k)
+//   G(u, lambda, zeta) = sum_m C(zeta,m) 2^(m-zeta) (lambda/2)^m A_m.
+// Work per atom is O(n M) with M = (Z+1)(Z+2)(Z+3)/6 monomials instead of
+// O(n^2) triplets.  The adjoint differentiates the same chain: dE/dT_b from
+// the block contractions, then per neighbour the polynomial
+// Q(r^) = sum_alpha dE/dT_b[u][alpha] r^^alpha and its gradient.  The
+// tangent (force-loss second backward) is the forward-mode derivative.
+// FP32 throughout: the alternating binomial sum amplifies rounding by at most
+// C(4,2) = 6 for zeta <= 4; larger zeta keeps the triplet kernels.
+
+__host__ __device__ constexpr int nmono(int Z) { return (Z + 1) * (Z + 2) * (Z + 3) / 6; }
+__host__ __device__ constexpr int mono_start(int m) { return m * (m + 1) * (m + 2) / 6; }
+__host__ __device__ constexpr int ifact(int n) { return n <= 1 ? 1 : n * ifact(n - 1); }
+
+// monomials of total degree <= kMomZ ordered by degree, then exponent of x and
+// y descending; every smaller degree bound is a prefix
+struct MonoTable {
+    int a[nmono(kMomZ)], b[nmono(kMomZ)], c[nmono(kMomZ)], m[nmono(kMomZ)];
+    float w[nmono(kMomZ)];
+    constexpr MonoTable() : a(), b(), c(), m(), w() {
+        int q = 0;
+        for (int mm = 0; mm <= kMomZ; ++mm)
+            for (int aa = mm; aa >= 0; --aa)
+                for (int bb = mm - aa; bb >= 0; --bb) {
+                    const int cc = mm - aa - bb;
+                    a[q] = aa; b[q] = bb; c[q] = cc; m[q] = mm;
+                    w[q] = (float)(ifact(mm) / (ifact(aa) * ifact(bb) * ifact(cc)));
+                    ++q;
+                }
+    }
+};
+__constant__ MonoTable c_mono = MonoTable();
+
+// C(z, m) 2^(m - z) lamh^m
+__device__ __forceinline__ float mom_coef(int z, float lamh, int m) {
+    if (m > z) return 0.0f;
+    float c = 1.0f;
+    for (int q = 0; q < m; ++q) c = c * (float)(z - q) / (float)(q + 1) * lamh;
+    for (int q = m; q < z; ++q) c *= 0.5f;
+    return c;
+}
+
+__device__ __forceinline__ int pair_of(int b, int c) { return b <= c ? c * (c + 1) / 2 + b : b * (b + 1) / 2 + c; }
+
+template <int MODE, int Z>
+__global__ __launch_bounds__(64) void k_moment(SymmArgs A) {
+    constexpr int NM = nmono(Z);
+    constexpr int ZP = Z + 1;
+    constexpr int PWS = (3 * ZP * (MODE == 2 ? 2 : 1)) | 1;   // powers per neighbour, odd stride
+    constexpr int U4 = kMaxUnionEta;
+    constexpr int NG = kWave / NM;                           // lane groups over the neighbours
+    extern __shared__ float4 smem4[];
+    const int i = blockIdx.x, lane = threadIdx.x;
+    if (i >= A.N) return;
+    const DevModel& Mo = *A.dm;
+    const int a = A.species[i], o = A.off[i];
+    int n = A.off[i + 1] - o;
+    const int S = Mo.S, NP = Mo.NP, ldg = A.ldg, maxn = A.maxn;
+    FastSmem sm;
+    sm.g = smem4;
+    sm.ef = sm.g + maxn;
+    sm.efd = sm.ef + maxn;
+    float4* tv = sm.efd + (MODE == 0 ? 0 : maxn);
+    sm.r = reinterpret_cast<float*>(tv + (MODE == 2 ? maxn : 0));
+    sm.canon = reinterpret_cast<int*>(sm.r + maxn);
+    sm.seg = sm.canon + maxn;
+    sm.extra = reinterpret_cast<float*>(sm.seg + 16);
+    float* rowbuf = sm.extra;                                   // [ldg]
+    float* sP = rowbuf + ldg;                                   // [maxn][3] (mode 1)
+    float* pw = sP + (MODE == 1 ? 3 * maxn : 0);                // [maxn][PWS]
+    float* T = pw + (size_t)maxn * PWS;                         // [S][U4][NM]
+    float* dT = T + S * U4 * NM;                                // [S][U4][NM] (mode 2)
+    float* D = dT + (MODE == 2 ? S * U4 * NM : 0);              // [S][U4]
+    float* dD = D + S * U4;                                     // [S][U4] (mode 2: dD, mode 1: dE/dD)
+    float* Am = dD + S * U4;                                    // [NP][U4][ZP]
+    float* Ub = Am + NP * U4 * ZP;                              // [S][NM][U4] (mode 1)
+    float* part = Ub + (MODE == 1 ? S * NM * U4 : 0);           // [NG][NM][U4] x (1 or 2)
+
+    n = fast_load(A, Mo, i, a, o, n, sm, MODE != 0, MODE == 1);
+    if (MODE == 1) {
+        for (int m = lane; m < ldg; m += kWave) rowbuf[m] = A.dEdG[(size_t)i * ldg + m];
+        for (int t = lane; t < 3 * n; t += kWave) sP[t] = 0.0f;
+    } else {
+        for (int m = lane; m < ldg; m += kWave) rowbuf[m] = 0.0f;
+    }
+    __syncthreads();
+    if (MODE == 2) {
+        for (int J = lane; J < n; J += kWave) {
+            const int j = A.nidx[o + sm.canon[J]];
+            const float tx = A.u[3 * i] - A.u[3 * j], ty = A.u[3 * i + 1] - A.u[3 * j + 1],
+                        tz = A.u[3 * i + 2] - A.u[3 * j + 2];
+            const float4 gq = sm.g[J];
+            tv[J] = make_float4(tx, ty, tz, gq.x * tx + gq.y * ty + gq.z * tz);
+        }
+        __syncthreads();
+    }
+    fast_radial<MODE>(A, Mo, a, sm, tv, rowbuf, sP, rowbuf);
+    // powers of the unit vector (and, for the tangent, their derivatives along
+    // the projected direction (t - r^ (r^.t)) / r)
+    for (int J = lane; J < n; J += kWave) {
+        const float4 gq = sm.g[J];
+        float* row = pw + (size_t)J * PWS;
+        const float v3[3] = {gq.x, gq.y, gq.z};
+        float d3[3] = {0.f, 0.f, 0.f};
+        if (MODE == 2) {
+            const float4 t4 = tv[J];
+            d3[0] = (t4.x - gq.x * t4.w) * gq.w;
+            d3[1] = (t4.y - gq.y * t4.w) * gq.w;
+            d3[2] = (t4.z - gq.z * t4.w) * gq.w;
+        }
+#pragma unroll
+        for (int x = 0; x < 3; ++x) {
+            float p = 1.0f;
+#pragma unroll
+            for (int k = 0; k < ZP; ++k) {
+                if (MODE == 2) row[3 * ZP + x * ZP + k] = k == 0 ? 0.0f : (float)k * p * d3[x];
+                row[x * ZP + k] = k == 0 ? 1.0f : (p *= v3[x]);
+            }
+        }
+    }
+    __syncthreads();
+    // moments T_b[u][alpha]: lane = (alpha, group); groups stride the neighbours
+    {
+        const int al = lane % NM, grp = lane / NM;
+        const bool act = grp < NG;
+        const int ea = c_mono.a[al], eb = c_mono.b[al], ec = c_mono.c[al];
+        for (int b = 0; b < S; ++b) {
+            const int s0 = sm.seg[b], s1 = sm.seg[b + 1];
+            float acc[U4], dacc[U4];
+#pragma unroll
+            for (int u = 0; u < U4; ++u) { acc[u] = 0.0f; dacc[u] = 0.0f; }
+            if (act) {
+                for (int J = s0 + grp; J < s1; J += NG) {
+                    const float* row = pw + (size_t)J * PWS;
+                    const float px = row[ea], py = row[ZP + eb], pz = row[2 * ZP + ec];
+                    const float mono = px * py * pz;
+                    const float4 e4 = sm.ef[J];
+                    acc[0] += e4.x * mono; acc[1] += e4.y * mono; acc[2] += e4.z * mono; acc[3] += e4.w * mono;
+                    if (MODE == 2) {
+                        const float dmono = row[3 * ZP + ea] * py * pz + px * row[4 * ZP + eb] * pz +
+                                            px * py * row[5 * ZP + ec];
+                        const float4 f4 = sm.efd[J];
+                        const float wm = tv[J].w * mono;
+                        dacc[0] += f4.x * wm + e4.x * dmono;
+                        dacc[1] += f4.y * wm + e4.y * dmono;
+                        dacc[2] += f4.z * wm + e4.z * dmono;
+                        dacc[3] += f4.w * wm + e4.w * dmono;
+                    }
+                }
+            }
+            if (NG == 1) {
+                if (act) {
+#pragma unroll
+                    for (int u = 0; u < U4; ++u) {
+                        T[(b * U4 + u) * NM + al] = acc[u];
+                        if (MODE == 2) dT[(b * U4 + u) * NM + al] = dacc[u];
+                    }
+                }
+            } else {
+                if (act) {
+#pragma unroll
+                    for (int u = 0; u < U4; ++u) {
+                        part[(grp * NM + al) * U4 + u] = acc[u];
+                        if (MODE == 2) part[NG * NM * U4 + (grp * NM + al) * U4 + u] = dacc[u];
+                    }
+                }
+                __syncthreads();
+                for (int q = lane; q < NM * U4; q += kWave) {
+                    float s = 0.0f, ds = 0.0f;
+                    for (int gq = 0; gq < NG; ++gq) {
+                        s += part[gq * NM * U4 + q];
+                        if (MODE == 2) ds += part[NG * NM * U4 + gq * NM * U4 + q];
+                    }
+                    const int al2 = q / U4, u = q % U4;
+                    T[(b * U4 + u) * NM + al2] = s;
+                    if (MODE == 2) dT[(b * U4 + u) * NM + al2] = ds;
+                }
+                __syncthreads();
+            }
+        }
+        // diagonal sums D_b[u] (mode 2 also their tangent)
+        if (lane < S * U4) {
+            const int b = lane / U4, u = lane % U4;
+            float s = 0.0f, ds = 0.0f;
+            for (int J = sm.seg[b]; J < sm.seg[b + 1]; ++J) {
+                const float e = f4get(sm.ef[J], u);
+                s += e * e;
+                if (MODE == 2) ds += 2.0f * e * f4get(sm.efd[J], u) * tv[J].w;
+            }
+            D[lane] = s;
+            if (MODE == 2) dD[lane] = ds;
+        }
+    }
+    __syncthreads();
+    // block contractions (modes 0, 2) or their adjoint seeds (mode 1)
+    for (int it = lane; it < NP * U4 * ZP; it += kWave) {
+        const int p = it / (U4 * ZP), u = (it / ZP) % U4, m = it % ZP;
+        int c = 0;
+        while ((c + 1) * (c + 2) / 2 <= p) ++c;
+        const int b = p - c * (c + 1) / 2;
+        float s = 0.0f;
+        if (MODE == 1) {
+            const AngBlock& B = Mo.ang[a * NP + p];
+            if (B.n > 0 && u < B.ng)
+                for (int k = 0; k < B.nk; ++k)
+                    s += rowbuf[B.goff + B.outm[u * B.nk + k]] * mom_coef(B.zint[k], B.lamh[k], m);
+        } else {
+            const float* Tb = T + (b * U4 + u) * NM;
+            const float* Tc = T + (c * U4 + u) * NM;
+            const float* dTb = dT + (b * U4 + u) * NM;
+            const float* dTc = dT + (c * U4 + u) * NM;
+            const int q0 = mono_start(m), q1 = mono_start(m + 1);
+            for (int q = q0; q < q1; ++q) {
+                const float w = c_mono.w[q];
+                s += MODE == 0 ? w * Tb[q] * Tc[q] : w * (dTb[q] * Tc[q] + Tb[q] * dTc[q]);
+            }
+            if (b == c) s = 0.5f * (s - (MODE == 0 ? D[b * U4 + u] : dD[b * U4 + u]));
+        }
+        Am[it] = s;
+    }
+    __syncthreads();
+    if (MODE != 1) {
+        for (int p = 0; p < NP; ++p) {
+            const AngBlock& B = Mo.ang[a * NP + p];
+            for (int q = lane; q < B.ng * B.nk; q += kWave) {
+                const int g = q / B.nk, k = q % B.nk;
+                float v = 0.0f;
+                for (int m = 0; m <= B.zint[k]; ++m) v += mom_coef(B.zint[k], B.lamh[k], m) * Am[(p * U4 + g) * ZP + m];
+                rowbuf[B.goff + B.outm[q]] = v;
+            }
+        }
+        __syncthreads();
+        float* out = (MODE == 0 ? A.G : A.v) + (size_t)i * ldg;
+        for (int m = lane; m < ldg; m += kWave) out[m] = rowbuf[m];
+        return;
+    }
+    // adjoint: dE/dT_b[alpha][u] and dE/dD_b[u]
+    for (int it = lane; it < S * NM * U4; it += kWave) {
+        const int b = it / (NM * U4), al = (it / U4) % NM, u = it % U4;
+        const int m = c_mono.m[al];
+        float s = 0.0f;
+        for (int c = 0; c < S; ++c) s += Am[(pair_of(b, c) * U4 + u) * ZP + m] * T[(c * U4 + u) * NM + al];
+        Ub[it] = c_mono.w[al] * s;
+    }
+    if (lane < S * U4) {
+        const int b = lane / U4, u = lane % U4;
+        float s = 0.0f;
+        for (int m = 0; m < ZP; ++m) s += Am[(pair_of(b, b) * U4 + u) * ZP + m];
+        dD[lane] = -0.5f * s;
+    }
+    __syncthreads();
+    // per neighbour: Q(r^) = sum_alpha W_alpha r^^alpha and grad of sum V_alpha r^^alpha
+    constexpr MonoTable mt{};
+    for (int J = lane; J < n; J += kWave) {
+        int b = 0;
+        while (b + 1 < S && J >= sm.seg[b + 1]) ++b;
+        const float4 gq = sm.g[J];
+        const float4 e4 = sm.ef[J], f4 = sm.efd[J];
+        float px[ZP], py[ZP], pz[ZP];
+        px[0] = py[0] = pz[0] = 1.0f;
+#pragma unroll
+        for (int k = 1; k < ZP; ++k) { px[k] = px[k - 1] * gq.x; py[k] = py[k - 1] * gq.y; pz[k] = pz[k - 1] * gq.z; }
+        float Qd = 0.0f, g0 = 0.0f, g1 = 0.0f, g2 = 0.0f;
+        const float4* Ubb = reinterpret_cast<const float4*>(Ub + b * NM * U4);
+#pragma unroll
+        for (int q = 0; q < NM; ++q) {
+            const int ea = mt.a[q], eb = mt.b[q], ec = mt.c[q];
+            const float4 w4 = Ubb[q];
+            const float V = e4.x * w4.x + e4.y * w4.y + e4.z * w4.z + e4.w * w4.w;
+            const float W = f4.x * w4.x + f4.y * w4.y + f4.z * w4.z + f4.w * w4.w;
+            const float yz = py[eb] * pz[ec];
+            Qd += W * (px[ea] * yz);
+            if (ea > 0) g0 += V * ((float)ea * px[ea > 0 ? ea - 1 : 0] * yz);
+            if (eb > 0) g1 += V * ((float)eb * px[ea] * py[eb > 0 ? eb - 1 : 0] * pz[ec]);
+            if (ec > 0) g2 += V * ((float)ec * px[ea] * py[eb] * pz[ec > 0 ? ec - 1 : 0]);
+        }
+        const float dd = 2.0f * (dD[b * U4] * e4.x * f4.x + dD[b * U4 + 1] * e4.y * f4.y +
+                                 dD[b * U4 + 2] * e4.z * f4.z + dD[b * U4 + 3] * e4.w * f4.w);
+        const float rg = gq.x * g0 + gq.y * g1 + gq.z * g2;
+        const float radial = Qd + dd;
+        sP[3 * J] += radial * gq.x + (g0 - gq.x * rg) * gq.w;
+        sP[3 * J + 1] += radial * gq.y + (g1 - gq.y * rg) * gq.w;
+        sP[3 * J + 2] += radial * gq.z + (g2 - gq.z * rg) * gq.w;
+    }
+    __syncthreads();
+    for (int J = lane; J < n; J += kWave) {
+        const int t = sm.canon[J];
+        A.P[3 * ((size_t)o + t)] = sP[3 * J];
+        A.P[3 * ((size_t)o + t) + 1] = sP[3 * J + 1];
+        A.P[3 * ((size_t)o + t) + 2] = sP[3 * J + 2];
+    }
+}
+
+static size_t moment_smem(const SymmArgs& A, int mode, int Z, int S, int NP) {
+    const int NM = nmono(Z), ZP = Z + 1, U4 = kMaxUnionEta;
+    const int PWS = (3 * ZP * (mode == 2 ? 2 : 1)) | 1;
+    const int NG = kWave / NM;
+    size_t b = 16 * (size_t)A.maxn * (2 + (mode == 0 ? 0 : 1) + (mode == 2 ? 1 : 0));
+    b += 4 * 2 * (size_t)A.maxn + 4 * 16;
+    size_t f = (size_t)A.ldg + (mode == 1 ? 3 * (size_t)A.maxn : 0) + (size_t)A.maxn * PWS;
+    f += (size_t)S * U4 * NM * (mode == 2 ? 2 : 1) + 2 * (size_t)S * U4 + (size_t)NP * U4 * ZP;
+    f += (mode == 1 ? (size_t)S * NM * U4 : 0);
+    f += (NG > 1 ? (size_t)NG * NM * U4 * (mode == 2 ? 2 : 1) : 0);
+    return b + 4 * f;
+}
+
 // ================================================================ host ====
 
 static SymmArgs make_args(ann_ctx* ctx) {
@@ -692,10 +1027,22 @@ template <int MODE>
 static hipError_t run(ann_ctx* ctx, int form, SymmArgs A) {
     if (A.N == 0) return hipSuccess;
     const DevModel& M = ctx->hm;
-    const bool fast = form == ANN_FORM_FAST && M.all_grid && M.fast_ng > 0;
-    const char* names[2][3] = {{"symm_fwd_plain", "symm_bwd_plain", "symm_tan_plain"},
-                               {"symm_fwd_fast", "symm_bwd_fast", "symm_tan_fast"}};
-    Launch L(ctx, names[fast ? 1 : 0][MODE]);
+    const bool fast = form != ANN_FORM_PLAIN && M.all_grid && M.fast_ng > 0;
+    const bool moment = fast && form == ANN_FORM_FAST && M.moment_z > 0;
+    const char* names[3][3] = {{"symm_fwd_plain", "symm_bwd_plain", "symm_tan_plain"},
+                               {"symm_fwd_fast", "symm_bwd_fast", "symm_tan_fast"},
+                               {"symm_fwd_mom", "symm_bwd_mom", "symm_tan_mom"}};
+    Launch L(ctx, names[moment ? 2 : (fast ? 1 : 0)][MODE]);
+    if (moment) {
+        const size_t sm = moment_smem(A, MODE, M.moment_z, M.S, M.NP);
+        switch (M.moment_z) {
+            case 1: k_moment<MODE, 1><<<A.N, 64, sm, ctx->stream>>>(A); break;
+            case 2: k_moment<MODE, 2><<<A.N, 64, sm, ctx->stream>>>(A); break;
+            case 3: k_moment<MODE, 3><<<A.N, 64, sm, ctx->stream>>>(A); break;
+            default: k_moment<MODE, 4><<<A.N, 64, sm, ctx->stream>>>(A); break;
+        }
+        return hipGetLastError();
+    }
     if (fast) {
         const size_t sm = fast_smem(A, MODE);
 #define X(G, K)                                                                \

@@ -20,7 +20,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libann_amd.so")
 _LIB = None
 
-PLAIN, FAST = 0, 1
+PLAIN, FAST, TRIPLET = 0, 1, 2   # computation forms (include/ann_gpu.h)
 P_D = C.POINTER(C.c_double)
 P_I = C.POINTER(C.c_int)
 

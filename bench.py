@@ -36,23 +36,37 @@ FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 
 
-def flop_counts(model, batch, nl):
-    """Algorithmic FLOPs of the basis kernels from the actual list:
-    symm_fwd = P(10 + 6 nR) + T(8 + 4 nA), symm_bwd = P(20 + 6 nR) + T(52 + 8 nA)
-    (together the E+F convention P(30 + 12 nR) + T(60 + 12 nA)), nR / nA the
-    functions of the pair's / triplet's basis block; network 4 MAC per atom."""
+def work_model(model, batch, nl):
+    """Algorithmic work of one launch of each kernel family, from the actual
+    neighbour list (P slots, T = sum_i n_i(n_i-1)/2 triplets, nR / nA the
+    functions of a pair's / triplet's block, FMA = 2 FLOP):
+
+    triplet tiles and PLAIN form (the E+F convention P(30 + 12 nR) + T(60 + 12 nA)):
+      symm_fwd = P(10 + 6 nR) + T(8 + 4 nA),  symm_bwd = P(20 + 6 nR) + T(52 + 8 nA)
+    moment form (M = (Z+1)(Z+2)(Z+3)/6 monomials, U widths of the centre species,
+    NP species pairs, nk (lambda, zeta) combinations, S species):
+      symm_fwd_mom = P(10 + 6 nR) + sum_i [n_i (M(2 + 2U) + 2U) + NP U 3M + sum_f 2(zeta_f + 1)]
+      symm_bwd_mom = P(20 + 6 nR) + sum_i [n_i (M(2 + 2U) + 2U + M(4U + 16) + 20)
+                                            + NP U (Z+1) 2 nk + S M U 2S]
+    networks: 4 MAC per atom and parameter-matrix entry (forward + input backward);
+    HBM kernels in bytes: nb_search 24 B/atom + 8 B/slot, nb_compact 24 B/atom +
+    45 B/slot, force_gather 96 B/atom + 44 B/slot, struc_sums 76 B/atom."""
     off = nl["off"]
     nsp = batch.species[nl["idx"]]
     S = model.nspecies
-    fwd = bwd = 0.0
+    NP = S * (S + 1) // 2
+    fwd = bwd = fwd_m = bwd_m = 0.0
     P = T = 0
     for i in range(batch.ntotal):
         a = batch.species[i]
         cnt = np.bincount(nsp[off[i]:off[i + 1]], minlength=S)
+        n_i = int(cnt.sum())
+        pair = 0.0
         for b in range(S):
             nR = len(model.radial[a][b][1])
             fwd += cnt[b] * (10 + 6 * nR)
             bwd += cnt[b] * (20 + 6 * nR)
+            pair += cnt[b] * nR
             P += cnt[b]
             for c in range(b, S):
                 t = cnt[b] * (cnt[b] - 1) // 2 if b == c else cnt[b] * cnt[c]
@@ -60,6 +74,15 @@ def flop_counts(model, batch, nl):
                 fwd += t * (8 + 4 * nA)
                 bwd += t * (52 + 8 * nA)
                 T += t
+        blocks = [model.angular[a][p] for p in range(NP)]
+        Z = int(max(max(bl[2]) for bl in blocks if len(bl[2])))
+        M = (Z + 1) * (Z + 2) * (Z + 3) // 6
+        U = len(set(np.concatenate([bl[1] for bl in blocks]).tolist()))
+        nk = max(len(set(zip(bl[2].tolist(), bl[3].tolist()))) for bl in blocks)
+        zsum = sum(float((bl[2] + 1).sum()) for bl in blocks)
+        fwd_m += n_i * 10 + 6 * pair + n_i * (M * (2 + 2 * U) + 2 * U) + NP * U * 3 * M + 2 * zsum
+        bwd_m += n_i * 20 + 6 * pair + n_i * (M * (2 + 2 * U) + 2 * U + M * (4 * U + 16) + 20) \
+            + NP * U * (Z + 1) * 2 * nk + S * M * U * 2 * S
     mac = 0
     for i in range(batch.ntotal):
         n = model.nets[batch.species[i]]
@@ -67,7 +90,37 @@ def flop_counts(model, batch, nl):
         for h in n.nodes:
             mac += prev * h
             prev = h
-    return {"symm_fwd": fwd, "symm_bwd": bwd, "net": 4.0 * mac, "pairs": P, "triplets": T}
+    N = batch.ntotal
+    return {"symm_fwd": fwd, "symm_bwd": bwd, "symm_fwd_mom": fwd_m, "symm_bwd_mom": bwd_m, "net": 4.0 * mac,
+            "pairs": P, "triplets": T,
+            "bytes": {"nb_search": 24.0 * N + 8.0 * P, "nb_compact": 24.0 * N + 45.0 * P,
+                      "force_gather": 96.0 * N + 44.0 * P, "struc_sums": 76.0 * N}}
+
+
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def kernel_rooflines(kern, work):
+    """Achieved rate and fraction of the MI355X peak for each timed kernel."""
+    out = {}
+    for k, v in kern.items():
+        sec = v["avg_us"] * 1e-6
+        if k.startswith("symm_"):
+            base = k.rsplit("_", 1)[0]            # symm_fwd / symm_bwd
+            fl = work[base + "_mom"] if k.endswith("_mom") else work[base]
+            ach = fl / sec / 1e12
+            out[k] = {"bound": "valu", "work": fl, "unit": "TFLOP/s", "achieved": ach, "peak": FP32_PEAK_TFLOPS,
+                      "frac": ach / FP32_PEAK_TFLOPS}
+        elif k.startswith("net_"):
+            ach = work["net"] / sec / 1e12
+            peak = MFMA_BF16_PEAK_TFLOPS if k == "net_mfma" else FP32_PEAK_TFLOPS
+            out[k] = {"bound": "mfma" if k == "net_mfma" else "valu", "work": work["net"], "unit": "TFLOP/s",
+                      "achieved": ach, "peak": peak, "frac": ach / peak}
+        elif k in work["bytes"]:
+            ach = work["bytes"][k] / sec / 1e9
+            out[k] = {"bound": "hbm", "work": work["bytes"][k], "unit": "GB/s", "achieved": ach,
+                      "peak": HBM_PEAK_GBS, "frac": ach / HBM_PEAK_GBS}
+    return out
 
 
 def time_ef(eng, form, steps, warmup, out):
@@ -116,16 +169,95 @@ def run_train(steps, warmup, form):
             "loss_last": loss.cpu().numpy().tolist()}
 
 
+def run_uneven(steps, warmup):
+    """512 uneven triclinic water structures (atom counts U[32, 1024], density
+    0.9 g/cm^3, cells strained by a random symmetric matrix): neighbour build +
+    E + F of the whole batch per step, FAST and PLAIN."""
+    import torch
+    from ann_amd import configs
+    from ann_amd.gpu import Engine, FAST, PLAIN
+    model = configs.model_water()
+    batch = configs.batch_uneven(512, seed=13)
+    eng = Engine(model)
+    eng.set_batch(batch)
+    eng.neighbors()
+    P, maxn = eng.nlist_size()
+    dev = eng.device
+    out = (torch.empty(batch.nstruc, dtype=torch.float64, device=dev),
+           torch.empty((batch.ntotal, 3), dtype=torch.float64, device=dev),
+           torch.empty((batch.nstruc, 3, 3), dtype=torch.float64, device=dev))
+    res = {"structures": batch.nstruc, "atoms": batch.ntotal, "pairs": P}
+    for form, name, k in ((FAST, "fast", steps), (PLAIN, "plain", max(2, steps // 5))):
+        dt = time_ef(eng, form, k, max(1, warmup // 2), out)
+        res[name] = {"atoms_per_s": batch.ntotal / dt, "ms_per_step": dt * 1e3, "steps": k}
+    res["fast_over_plain"] = res["fast"]["atoms_per_s"] / res["plain"]["atoms_per_s"]
+    return res
+
+
+def run_md(nsteps, stride, plain_steps, skin=0.5, dt=0.02, reps=16):
+    """Velocity-Verlet MD of a reps^3 replica of a 256-atom H/O cell (1,048,576
+    atoms for reps = 16, L = 198.8 A): list at rc + skin rebuilt every
+    `stride` steps (positions returned to the cell first), E + F + virial every
+    step, FP64 energy / kinetic-energy sums.  Time unit A sqrt(amu/eV);
+    initial velocities Maxwell-Boltzmann at 300 K."""
+    import torch
+    from ann_amd import configs
+    from ann_amd.gpu import Engine, FAST, PLAIN
+    model = configs.model_water()
+    b = configs.struc_water_md(seed=15, reps=reps)
+    rng = np.random.default_rng(16)
+    mass = model.masses[b.species]
+    kT = 8.617333262e-5 * 300.0
+    vel0 = rng.normal(size=(b.ntotal, 3)) * np.sqrt(kT / mass)[:, None]
+    vel0 -= (mass[:, None] * vel0).sum(0) / mass.sum()
+    out = {"atoms": b.ntotal, "steps": nsteps, "rebuild_every": stride, "skin": skin, "dt": dt}
+    for form, name, k in ((FAST, "fast", nsteps), (PLAIN, "plain", plain_steps)):
+        eng = Engine(model)
+        eng.set_batch(b)
+        dev = eng.device
+        m = torch.from_numpy(mass).to(dev)
+        v = torch.from_numpy(vel0.copy()).to(dev)
+        E = torch.empty(1, dtype=torch.float64, device=dev)
+        KE = torch.empty(1, dtype=torch.float64, device=dev)
+        V = torch.empty((1, 3, 3), dtype=torch.float64, device=dev)
+        eng.neighbors(model.rc + skin)
+        _, F, _ = eng.energy_forces(form, out=(E, torch.empty((b.ntotal, 3), dtype=torch.float64, device=dev), V))
+        moved_any = False
+        e0 = E.item() + 0.5 * float((mass[:, None] * vel0 ** 2).sum())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for t in range(k):
+            if t > 0 and t % stride == 0:
+                eng.wrap_positions()
+                eng.neighbors(model.rc + skin)
+            moved_any |= eng.md_step(dt, m, v, F, form=form, E=E, KE=KE, V=V,
+                                     check=(t % stride == stride - 1 or t == k - 1))
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        P, maxn = eng.nlist_size()
+        etot = (E + KE).item()
+        out[name] = {"atom_steps_per_s": b.ntotal * k / wall, "ms_per_step": wall / k * 1e3, "steps": k,
+                     "skin_exceeded": bool(moved_any)}
+        if name == "fast":
+            out.update({"pairs_in_list": P, "max_neighbours": maxn,
+                        "energy_per_atom_first": e0 / b.ntotal, "energy_per_atom_last": etot / b.ntotal,
+                        "drift_per_atom": (etot - e0) / b.ntotal})
+        del eng
+        torch.cuda.empty_cache()
+    out["fast_over_plain"] = out["fast"]["atom_steps_per_s"] / out["plain"]["atom_steps_per_s"]
+    return out
+
+
 def cpu_baseline(model, batch, gpu_forces):
     """FP64 oracle (the port of the reference algorithm) on a bounded sample:
-    the first structures of the batch, one per host thread (about 1 s of CPU
-    work per structure), every host core.  Also returns the force error of each
+    the 64 structures of the batch (about 0.4 s of CPU work each, 25-30 s in
+    total), spread over every host core.  Also returns the force error of each
     GPU form on that sample (the oracle as checker)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
     from ann_amd.configs import Batch
     cores = os.cpu_count() or 1
-    nsample = int(min(batch.nstruc, max(8, cores)))
+    nsample = batch.nstruc
     sub = batch.subset(range(nsample))
     orc = pyoracle.Oracle(model)
     t0 = time.perf_counter()
@@ -149,6 +281,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-train", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--no-md", action="store_true")
+    ap.add_argument("--md-steps", type=int, default=100)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy three times (calibrates HBM counters)")
     args = ap.parse_args()
@@ -167,7 +302,7 @@ def main():
         sys.exit(2)
     import torch
     from ann_amd import configs
-    from ann_amd.gpu import Engine, FAST, PLAIN
+    from ann_amd.gpu import Engine, FAST, PLAIN, TRIPLET
 
     model = configs.model_water()
     batch = configs.batch_water(64, seed=12)
@@ -175,19 +310,20 @@ def main():
     eng.set_batch(batch)
     eng.neighbors()
     nl = eng.nlist()
-    flops = flop_counts(model, batch, nl)
+    flops = work_model(model, batch, nl)
     dev = eng.device
     out = (torch.empty(batch.nstruc, dtype=torch.float64, device=dev),
            torch.empty((batch.ntotal, 3), dtype=torch.float64, device=dev),
            torch.empty((batch.nstruc, 3, 3), dtype=torch.float64, device=dev))
     res = {}
     forces = {}
-    for form, name in ((PLAIN, "plain"), (FAST, "fast")):
+    for form, name in ((PLAIN, "plain"), (TRIPLET, "triplet"), (FAST, "fast")):
         eng.profile(True)
         dt = time_ef(eng, form, args.steps, args.warmup, out)
         kern = {}
-        for k in ("nb_bin", "nb_search", "nb_compact", "symm_fwd_plain", "symm_fwd_fast", "symm_bwd_plain",
-                  "symm_bwd_fast", "net_plain", "net_mfma", "force_gather", "struc_sums"):
+        for k in ("nb_bin", "nb_search", "nb_compact", "symm_fwd_plain", "symm_fwd_fast", "symm_fwd_mom",
+                  "symm_bwd_plain", "symm_bwd_fast", "symm_bwd_mom", "net_plain", "net_mfma", "force_gather",
+                  "struc_sums"):
             ms, n = eng.profile_get(k)
             if n:
                 kern[k] = {"avg_us": ms / n * 1e3, "launches": n}
@@ -197,14 +333,15 @@ def main():
         forces[name] = (E.cpu().numpy(), F.cpu().numpy())
     # dominant kernel of the FAST form
     fk = res["fast"]["kernels"]
-    dom = max((k for k in fk if k.startswith("symm")), key=lambda k: fk[k]["avg_us"])
-    fl = flops["symm_bwd"] if "bwd" in dom else flops["symm_fwd"]
-    achieved = fl / (fk[dom]["avg_us"] * 1e-6) / 1e12
-    roof = {"bound": "valu", "kernel": dom, "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
-            "algorithmic_flop_per_launch": fl, "avg_launch_us": fk[dom]["avg_us"],
-            "note": "FP32 vector ALU bound; peak is the MI355X FP32 figure (packed FP32 disabled in this build, "
-                    "so the attainable non-packed VALU rate is half of it)"}
+    roofs = {name: kernel_rooflines(res[name]["kernels"], flops) for name in res}
+    dom = max((k for k in fk if k in roofs["fast"]), key=lambda k: fk[k]["avg_us"])
+    rd = roofs["fast"][dom]
+    roof = {"bound": rd["bound"], "kernel": dom, "achieved": rd["achieved"], "peak": rd["peak"], "unit": rd["unit"],
+            "frac": rd["frac"], "traffic": None, "algorithmic_work_per_launch": rd["work"],
+            "avg_launch_us": fk[dom]["avg_us"]}
+    if rd["bound"] == "valu":
+        roof["note"] = ("FP32 vector ALU; peak is the MI355X FP32 figure (packed FP32 disabled in this build, "
+                        "so the attainable non-packed VALU rate is half of it)")
     traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(traffic_file):
         tj = json.load(open(traffic_file))
@@ -215,6 +352,20 @@ def main():
         train = run_train(max(3, args.steps // 2), max(1, args.warmup // 2), FAST)
         train_plain = run_train(max(3, args.steps // 4), 1, PLAIN)
         train["plain_structs_per_s"] = train_plain["structs_per_s"]
+        train["triplet_structs_per_s"] = run_train(max(3, args.steps // 2), 1, TRIPLET)["structs_per_s"]
+        train["fast_over_plain"] = train["structs_per_s"] / train["plain_structs_per_s"]
+    def note(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    note(f"E+F fast {res['fast']['atoms_per_s']:.4g} atoms/s, plain {res['plain']['atoms_per_s']:.4g}")
+    if train:
+        note(f"train {train['structs_per_s']:.4g} structs/s")
+    uneven = None if args.no_uneven else run_uneven(max(3, args.steps // 2), args.warmup)
+    if uneven:
+        note(f"uneven batch fast {uneven['fast']['atoms_per_s']:.4g} atoms/s")
+    md = None if args.no_md else run_md(args.md_steps, 10, 2)
+    if md:
+        note(f"md fast {md['fast']['atom_steps_per_s']:.4g} atom-steps/s")
     cpu, errs = (None, {}) if args.no_cpu else cpu_baseline(model, batch, forces)
     line = {
         "metric": METRIC,
@@ -233,10 +384,14 @@ def main():
                    "global_batch": batch.nstruc, "atoms": batch.ntotal, "pairs": flops["pairs"],
                    "triplets": flops["triplets"], "parallelism": "single GPU"},
         "plain": {"atoms_per_s": res["plain"]["atoms_per_s"], "ms_per_step": res["plain"]["ms_per_step"]},
+        "triplet": {"atoms_per_s": res["triplet"]["atoms_per_s"], "ms_per_step": res["triplet"]["ms_per_step"]},
         "fast_over_plain": res["fast"]["atoms_per_s"] / res["plain"]["atoms_per_s"],
-        "kernels": {"fast": res["fast"]["kernels"], "plain": res["plain"]["kernels"]},
+        "kernels": {name: res[name]["kernels"] for name in res},
         "roofline": roof,
+        "kernel_rooflines": roofs,
         "train": train,
+        "uneven_batch": uneven,
+        "md": md,
         "cpu_baseline": cpu,
         "errors_vs_fp64_oracle": errs,
     }

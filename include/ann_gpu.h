@@ -42,9 +42,13 @@ enum {
 enum {
     ANN_FORM_PLAIN = 0,  /* FP32, reference-shaped loops: per-triplet function
                             evaluation, LDS-atomic accumulation, FP32 GEMV network */
-    ANN_FORM_FAST = 1    /* species-blocked balanced triplet tiles, pair-separable
-                            exponentials, integer powers, atomic-free adjoint,
-                            network layers on BF16 MFMA with split hi+lo products */
+    ANN_FORM_FAST = 1,   /* angular sums as per-neighbour tensor moments where every
+                            angular block is GAUSS with integer zeta <= 4 (exact
+                            factorisation, O(n) per atom), otherwise species-blocked
+                            triplet tiles; pair-separable exponentials, atomic-free
+                            adjoint, network layers on BF16 MFMA with split hi+lo
+                            products */
+    ANN_FORM_TRIPLET = 2 /* FAST with the triplet tiles everywhere (no moment form) */
 };
 
 /* Potential description, flat arrays (host memory, copied by ann_create).

@@ -30,7 +30,7 @@ def _engine(model, batch, rc):
     return eng
 
 
-@pytest.mark.parametrize("form", [0, 1])
+@pytest.mark.parametrize("form", [0, 1, 2])
 def test_skin_list_energy_forces_match_oracle(oracle_mod, form):
     model, b = _water()
     print(f"skin list E+F, form {form}, {b.ntotal} atoms", flush=True)
@@ -97,7 +97,7 @@ def md_case(oracle_mod):
     return model, b, mass, vel0, _verlet_oracle(orc, b.pos, b.cells[0], b.species, vel0, mass, DT, NSTEPS)
 
 
-@pytest.mark.parametrize("form", [0, 1])
+@pytest.mark.parametrize("form", [0, 1, 2])
 def test_md_trajectory_matches_oracle(md_case, form):
     import torch
     model, b, mass, vel0, (po, vo, Eo, Fo) = md_case

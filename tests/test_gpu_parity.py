@@ -14,7 +14,7 @@ import pytest
 
 pytestmark = pytest.mark.gpu
 
-FORMS = [0, 1]
+FORMS = [0, 1, 2]   # plain, fast (moment form where it applies), fast with triplet tiles
 
 
 def _engine(model, batch):

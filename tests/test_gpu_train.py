@@ -23,7 +23,7 @@ def _setup(name):
     return model, batch, eng
 
 
-@pytest.mark.parametrize("form", [0, 1])
+@pytest.mark.parametrize("form", [0, 1, 2])
 @pytest.mark.parametrize("name", ["lips3", "water2"])
 def test_tangent_parity(oracle_mod, name, form):
     import torch
@@ -64,7 +64,7 @@ def test_second_backward_parity(oracle_mod, name):
     assert err <= 1e-5, err
 
 
-@pytest.mark.parametrize("form", [0, 1])
+@pytest.mark.parametrize("form", [0, 1, 2])
 @pytest.mark.parametrize("name", ["lips3", "water2"])
 def test_train_step_parity(oracle_mod, name, form):
     import torch
