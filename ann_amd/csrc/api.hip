@@ -212,6 +212,18 @@ static int upload_model(ann_ctx* ctx, const ann_model_desc* d) {
         }
         if (ok && z > 0) M.moment_z = z;
     }
+    for (int t = 0; t < S * NP; ++t) {
+        AngBlock& ab = M.ang[t];
+        for (int k = 0; k < kGridK; ++k)
+            for (int m = 0; m <= kMomZ; ++m) {
+                double c = 0.0;
+                if (M.moment_z > 0 && k < ab.nk && m <= ab.zint[k]) {
+                    c = std::pow(2.0, m - ab.zint[k]) * std::pow((double)ab.lamh[k], m);
+                    for (int q = 0; q < m; ++q) c = c * (ab.zint[k] - q) / (q + 1);
+                }
+                ab.mcoef[k][m] = (float)c;
+            }
+    }
     int maxin = 1;
     for (int a = 0; a < S; ++a) maxin = std::max(maxin, nin[a]);
     ctx->max_nin = maxin;
@@ -278,6 +290,7 @@ static int upload_model(ann_ctx* ctx, const ann_model_desc* d) {
     ANN_HIP(hipMemset(ctx->da.adam_m, 0, sizeof(double) * ptot));
     ANN_HIP(hipMemset(ctx->da.adam_v, 0, sizeof(double) * ptot));
     ANN_HIP(hipMalloc(&ctx->d_flag, sizeof(int) * 4));
+    ANN_HIP(hipMemset(ctx->d_flag, 0, sizeof(int) * 4));
     return ANN_OK;
 }
 
@@ -307,7 +320,8 @@ extern "C" int ann_destroy(ann_ctx* ctx) {
                     ctx->nl.idx, ctx->nl.img, ctx->nl.rev, ctx->nl.geo, ctx->nl.nsp, ctx->d_G, ctx->d_dEdG,
                     ctx->d_Eatom, ctx->d_P, ctx->d_u, ctx->d_v, ctx->d_F, ctx->d_E, ctx->d_gE, ctx->d_atomvir,
                     ctx->d_scr, ctx->d_gpart, ctx->d_rank, ctx->d_spcount, ctx->d_perm, ctx->d_tiles, ctx->d_spend, ctx->d_grad_tiles, ctx->d_gsplit,
-                    ctx->d_chunks, ctx->d_chunk_off, ctx->d_part, ctx->d_pos_ref, ctx->d_keatom, ctx->d_md_flag};
+                    ctx->d_chunks, ctx->d_chunk_off, ctx->d_part, ctx->d_pos_ref, ctx->d_keatom, ctx->d_md_flag,
+                    ctx->d_bin_struc};
     for (void* p : ptrs)
         if (p) hipFree(p);
     for (hipEvent_t e : ctx->evpool) hipEventDestroy(e);
@@ -421,6 +435,7 @@ extern "C" int ann_set_batch(ann_ctx* ctx, const ann_batch* b) {
     ANN_HIP(hipStreamSynchronize(ctx->stream));
     ctx->have_batch = true;
     ctx->have_nlist = false;
+    ctx->grid_valid = false;
     ctx->G_valid = false;
     ctx->rank_valid = false;
     ctx->perm_valid = false;
@@ -439,16 +454,7 @@ extern "C" int ann_neighbors(ann_ctx* ctx, double rc) {
     ctx->have_nlist = true;
     ctx->G_valid = false;
     ctx->nl_rc = r;
-    // reference positions of the skin criterion
-    if (ctx->natoms > ctx->md_cap) {
-        ANN_HIP(dalloc(ctx->d_pos_ref, 3 * (size_t)ctx->natoms));
-        ANN_HIP(dalloc(ctx->d_keatom, (size_t)ctx->natoms));
-        if (!ctx->d_md_flag) ANN_HIP(hipMalloc(&ctx->d_md_flag, sizeof(int)));
-        ctx->md_cap = ctx->natoms;
-    }
-    ANN_HIP(hipMemcpyAsync(ctx->d_pos_ref, ctx->d_pos, sizeof(double) * 3 * ctx->natoms, hipMemcpyDeviceToDevice,
-                           ctx->stream));
-    ANN_HIP(hipMemsetAsync(ctx->d_md_flag, 0, sizeof(int), ctx->stream));
+    ctx->pos_ref_valid = false;   // the skin reference is taken by the next MD step
     return ANN_OK;
 }
 
@@ -474,6 +480,10 @@ extern "C" int ann_wrap_positions(ann_ctx* ctx) {
 extern "C" int ann_nlist_size(ann_ctx* ctx, int64_t* np, int* maxn) {
     CHECK_CTX(ctx);
     if (!ctx->have_nlist) return fail(ANN_ERR_STATE, "no neighbour list");
+    int rev_err = 0;
+    ANN_HIP(hipMemcpyAsync(&rev_err, ctx->d_flag + 2, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    ANN_HIP(hipStreamSynchronize(ctx->stream));
+    if (rev_err) return fail(ANN_ERR_HIP, "neighbour list is not symmetric (reverse slot missing)");
     if (np) *np = ctx->nl.npairs;
     if (maxn) *maxn = ctx->nl.maxn;
     return ANN_OK;
@@ -612,6 +622,19 @@ extern "C" int ann_md_step(ann_ctx* ctx, int form, double dt, const double* mass
     RET_HIP(ensure_work(ctx), "allocation");
     const double skin = std::max(ctx->nl_rc - (double)ctx->hm.rc, 0.0);
     double* pos = const_cast<double*>(ctx->d_pos);
+    if (ctx->natoms > ctx->md_cap) {
+        ANN_HIP(dalloc(ctx->d_pos_ref, 3 * (size_t)ctx->natoms));
+        ANN_HIP(dalloc(ctx->d_keatom, (size_t)ctx->natoms));
+        if (!ctx->d_md_flag) ANN_HIP(hipMalloc(&ctx->d_md_flag, sizeof(int)));
+        ctx->md_cap = ctx->natoms;
+        ctx->pos_ref_valid = false;
+    }
+    if (!ctx->pos_ref_valid) {   // first step after a build: positions are still those of the build
+        ANN_HIP(hipMemcpyAsync(ctx->d_pos_ref, pos, sizeof(double) * 3 * ctx->natoms, hipMemcpyDeviceToDevice,
+                               ctx->stream));
+        ANN_HIP(hipMemsetAsync(ctx->d_md_flag, 0, sizeof(int), ctx->stream));
+        ctx->pos_ref_valid = true;
+    }
     RET_HIP(md_kick_drift(ctx, dt, mass, F, vel, pos, 0.25 * skin * skin), "md_kick_drift");
     RET_HIP(refresh_geometry(ctx), "refresh_geometry");
     ctx->G_valid = false;

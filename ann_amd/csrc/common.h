@@ -39,6 +39,7 @@ struct AngBlock {
     float zeta[kGridK];
     int zint[kGridK];              // integer zeta (>=0) or -1
     int outm[kGridG * kGridK];     // function index of (g, k)
+    float mcoef[kGridK][kMomZ + 1];  // moment form: C(zeta,m) 2^(m-zeta) (lambda/2)^m of combination k
 };
 
 struct NetDesc {
@@ -156,6 +157,12 @@ struct ann_ctx {
     int* d_shift = nullptr;           // per atom integer shift [N][3]
     int* d_bin_cnt = nullptr;
     int* d_bin_start = nullptr;
+    int* d_bin_struc = nullptr;       // structure of each bin
+    int* d_bin_ovf = nullptr;         // bin whose candidate set exceeds the staged search (in d_bin_cnt)
+    bool grid_valid = false;          // bin grid of the current batch for grid_rc
+    double grid_rc = 0;
+    int grid_nbins = 0;
+    bool pos_ref_valid = false;       // d_pos_ref holds the positions of the last build
     int* d_bin_atoms = nullptr;
     int bin_cap = 0;
     uint64_t* d_keys = nullptr;       // [N][kMaxNeigh] sorted candidate keys

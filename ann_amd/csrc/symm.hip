@@ -713,15 +713,6 @@ struct MonoTable {
 };
 __constant__ MonoTable c_mono = MonoTable();
 
-// C(z, m) 2^(m - z) lamh^m
-__device__ __forceinline__ float mom_coef(int z, float lamh, int m) {
-    if (m > z) return 0.0f;
-    float c = 1.0f;
-    for (int q = 0; q < m; ++q) c = c * (float)(z - q) / (float)(q + 1) * lamh;
-    for (int q = m; q < z; ++q) c *= 0.5f;
-    return c;
-}
-
 __device__ __forceinline__ int pair_of(int b, int c) { return b <= c ? c * (c + 1) / 2 + b : b * (b + 1) / 2 + c; }
 
 template <int MODE, int Z>
@@ -860,33 +851,54 @@ __global__ __launch_bounds__(64) void k_moment(SymmArgs A) {
                 __syncthreads();
             }
         }
-        // diagonal sums D_b[u] (mode 2 also their tangent)
-        if (lane < S * U4) {
-            const int b = lane / U4, u = lane % U4;
-            float s = 0.0f, ds = 0.0f;
-            for (int J = sm.seg[b]; J < sm.seg[b + 1]; ++J) {
-                const float e = f4get(sm.ef[J], u);
-                s += e * e;
-                if (MODE == 2) ds += 2.0f * e * f4get(sm.efd[J], u) * tv[J].w;
+        // diagonal sums D_b[u] (mode 2 also their tangent): lanes over the
+        // segment, one transposing reduction of the four widths
+        if (MODE != 1) {
+            for (int b = 0; b < S; ++b) {
+                float d4[U4], t4[U4];
+#pragma unroll
+                for (int u = 0; u < U4; ++u) { d4[u] = 0.0f; t4[u] = 0.0f; }
+                for (int J = sm.seg[b] + lane; J < sm.seg[b + 1]; J += kWave) {
+                    const float4 e4 = sm.ef[J];
+                    d4[0] += e4.x * e4.x; d4[1] += e4.y * e4.y; d4[2] += e4.z * e4.z; d4[3] += e4.w * e4.w;
+                    if (MODE == 2) {
+                        const float4 f4 = sm.efd[J];
+                        const float w2 = 2.0f * tv[J].w;
+                        t4[0] += w2 * e4.x * f4.x; t4[1] += w2 * e4.y * f4.y;
+                        t4[2] += w2 * e4.z * f4.z; t4[3] += w2 * e4.w * f4.w;
+                    }
+                }
+                int idx;
+                const float sd = wave_sum_multi<U4>(d4, idx);
+                if ((lane & (kWave / U4 - 1)) == 0) D[b * U4 + idx] = sd;
+                if (MODE == 2) {
+                    const float st = wave_sum_multi<U4>(t4, idx);
+                    if ((lane & (kWave / U4 - 1)) == 0) dD[b * U4 + idx] = st;
+                }
             }
-            D[lane] = s;
-            if (MODE == 2) dD[lane] = ds;
         }
     }
     __syncthreads();
     // block contractions (modes 0, 2) or their adjoint seeds (mode 1)
-    for (int it = lane; it < NP * U4 * ZP; it += kWave) {
+    if (MODE == 1) {
+        for (int p = 0; p < NP; ++p) {
+            const AngBlock& B = Mo.ang[a * NP + p];
+            for (int it = lane; it < U4 * ZP; it += kWave) {
+                const int u = it / ZP, m = it % ZP;
+                float s = 0.0f;
+                if (B.n > 0 && u < B.ng)
+                    for (int k = 0; k < B.nk; ++k) s += rowbuf[B.goff + B.outm[u * B.nk + k]] * B.mcoef[k][m];
+                Am[p * U4 * ZP + it] = s;
+            }
+        }
+    }
+    for (int it = lane; MODE != 1 && it < NP * U4 * ZP; it += kWave) {
         const int p = it / (U4 * ZP), u = (it / ZP) % U4, m = it % ZP;
         int c = 0;
         while ((c + 1) * (c + 2) / 2 <= p) ++c;
         const int b = p - c * (c + 1) / 2;
         float s = 0.0f;
-        if (MODE == 1) {
-            const AngBlock& B = Mo.ang[a * NP + p];
-            if (B.n > 0 && u < B.ng)
-                for (int k = 0; k < B.nk; ++k)
-                    s += rowbuf[B.goff + B.outm[u * B.nk + k]] * mom_coef(B.zint[k], B.lamh[k], m);
-        } else {
+        {
             const float* Tb = T + (b * U4 + u) * NM;
             const float* Tc = T + (c * U4 + u) * NM;
             const float* dTb = dT + (b * U4 + u) * NM;
@@ -907,7 +919,8 @@ __global__ __launch_bounds__(64) void k_moment(SymmArgs A) {
             for (int q = lane; q < B.ng * B.nk; q += kWave) {
                 const int g = q / B.nk, k = q % B.nk;
                 float v = 0.0f;
-                for (int m = 0; m <= B.zint[k]; ++m) v += mom_coef(B.zint[k], B.lamh[k], m) * Am[(p * U4 + g) * ZP + m];
+#pragma unroll
+                for (int m = 0; m < ZP; ++m) v += B.mcoef[k][m] * Am[(p * U4 + g) * ZP + m];
                 rowbuf[B.goff + B.outm[q]] = v;
             }
         }
