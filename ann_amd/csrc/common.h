@@ -161,11 +161,13 @@ struct ann_ctx {
     int* d_bin_ovf = nullptr;         // bin whose candidate set exceeds the staged search (in d_bin_cnt)
     bool grid_valid = false;          // bin grid of the current batch for grid_rc
     double grid_rc = 0;
-    int grid_nbins = 0;
+    int grid_nbins = 0, grid_nsb = 0;  // bins and staged-search super-bins
     bool pos_ref_valid = false;       // d_pos_ref holds the positions of the last build
     int* d_bin_atoms = nullptr;
     int bin_cap = 0;
     uint64_t* d_keys = nullptr;       // [N][kMaxNeigh] sorted candidate keys
+    double4* d_binrec = nullptr;      // [N] atoms in bin order: x, y, z, (atom, packed shift)
+    int64_t binrec_cap = 0;
     int* d_cnt = nullptr;             // [N] neighbour counts
     int* d_flag = nullptr;            // error flags
     int64_t keys_cap = 0;

@@ -55,9 +55,12 @@ __device__ __forceinline__ void pair_vec_v(double pix, double piy, double piz, d
     d2 = __dadd_rn(__dadd_rn(__dmul_rn(x, x), __dmul_rn(y, y)), __dmul_rn(z, z));
 }
 
-// per structure grid: nb[3], k[3], nbins, bin offset
+// per structure grid: nb[3] bins, search half-width k[3] in bins, bin count
+// and offset; the staged search groups sb^3 bins into one block (super-bins:
+// nsb[3], offset sboff)
 struct CellInfo {
     int nb[3], k[3], nbins, boff;
+    int sb, nsb[3], sboff;
 };
 
 __global__ void k_struc_of(const int* __restrict__ off, int nstruc, int* __restrict__ sof) {
@@ -87,13 +90,20 @@ __global__ void k_bin(int N, const double* __restrict__ pos, const int* __restri
     atomicAdd(&bin_cnt[g], 1);
 }
 
+// Atoms in bin order: index, and a 32-byte record (x, y, z, atom | shift)
+// that the staged search copies with one load per candidate.
 __global__ void k_fill_bins(int N, const int* __restrict__ bin_of, const int* __restrict__ bin_start,
-                            int* __restrict__ bin_fill, int* __restrict__ bin_atoms) {
+                            int* __restrict__ bin_fill, int* __restrict__ bin_atoms,
+                            const double* __restrict__ pos, const int* __restrict__ shift,
+                            double4* __restrict__ rec) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     const int g = bin_of[i];
     const int p = atomicAdd(&bin_fill[g], 1);
-    bin_atoms[bin_start[g] + p] = i;
+    const int slot = bin_start[g] + p;
+    bin_atoms[slot] = i;
+    const uint64_t js = ((uint64_t)(uint32_t)i << 32) | pack_img(shift[3 * i], shift[3 * i + 1], shift[3 * i + 2]);
+    rec[slot] = make_double4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], __longlong_as_double((long long)js));
 }
 
 __device__ __forceinline__ int floordiv(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
@@ -198,53 +208,126 @@ __global__ __launch_bounds__(64) void k_search(
 }
 
 // Staged search, one block per bin: the candidates of the bin's search box
-// (every atom of the (2k+1)^3 bin images, with its image offset) are copied
-// to LDS once and sorted by (j, image - shift_j); each wavefront then tests
-// one centre atom of the bin against them in that order, so its accepted
-// keys (j, si + image - shift_j) come out already in canonical order and go
-// straight to the key scratch without a per-atom sort.  Bins whose search box
-// holds more than kStageCap candidates or kStageImages images are flagged and
-// left to k_search.
-constexpr int kStageCap = 1024;
-constexpr int kStageImages = 125;
+// (every atom of the (2k+1)^3 bin images with its image offset) are copied
+// to LDS with one 32-byte load each; each wavefront then tests one centre
+// atom of the bin against them, compacts the accepted keys (j << 30 | packed
+// image) into its own LDS buffer and sorts them in registers with a bitonic
+// network over lane shuffles (no block barriers after the staging).  Bins
+// whose search box holds more than kStageCap candidates or kStageImages
+// images are flagged and left to k_search.
+constexpr int kStageCap = 1280;
+constexpr int kStageImages = 216;
 constexpr int kStageThreads = 256;
+constexpr int kStageWaves = kStageThreads / kWave;
+
+__device__ __forceinline__ void cswap_u64(uint64_t& a, uint64_t& b, bool asc) {
+    const uint64_t lo = a < b ? a : b, hi = a < b ? b : a;
+    a = asc ? lo : hi;
+    b = asc ? hi : lo;
+}
+
+// ascending bitonic sort of the 64 R keys held as v[r] = element lane + 64 r
+template <int R>
+__device__ __forceinline__ void wave_sort_u64(uint64_t (&v)[R]) {
+    const int lane = threadIdx.x % kWave;
+    constexpr int n2 = kWave * R;
+#pragma unroll
+    for (int k = 2; k <= n2; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j < kWave) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int e = lane + kWave * r;
+                    const uint64_t o = (uint64_t)__shfl_xor((long long)v[r], j);
+                    const bool asc = (e & k) == 0, lower = (lane & j) == 0;
+                    const uint64_t mn = v[r] < o ? v[r] : o, mx = v[r] < o ? o : v[r];
+                    v[r] = (asc == lower) ? mn : mx;
+                }
+            } else {
+                const int rj = j / kWave;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    if ((r & rj) == 0) {
+                        const int e = lane + kWave * r;
+                        cswap_u64(v[r], v[r | rj], (e & k) == 0);
+                    }
+                }
+            }
+        }
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void sort_and_store(const uint64_t* buf, int n, uint64_t* dst) {
+    const int lane = threadIdx.x % kWave;
+    uint64_t v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int e = lane + kWave * r;
+        v[r] = e < n ? buf[e] : ~0ull;
+    }
+    wave_sort_u64<R>(v);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int e = lane + kWave * r;
+        if (e < n) dst[e] = v[r];
+    }
+}
 
 __global__ __launch_bounds__(kStageThreads) void k_search_bins(
-    const double* __restrict__ pos, const CellInfo* __restrict__ ci, const double* __restrict__ cells,
-    const int* __restrict__ bin_struc, const int* __restrict__ shift, const int* __restrict__ bin_start,
-    const int* __restrict__ bin_atoms, double rc2, uint64_t* __restrict__ keys_out, int* __restrict__ cnt_out,
-    int* __restrict__ flag, int* __restrict__ bin_ovf) {
-    __shared__ int im_bin[kStageImages];
+    const CellInfo* __restrict__ ci, const double* __restrict__ cells, const int* __restrict__ bin_struc,
+    const int* __restrict__ bin_start, const double4* __restrict__ rec, double rc2,
+    uint64_t* __restrict__ keys_out, int* __restrict__ cnt_out, int* __restrict__ flag, int* __restrict__ bin_ovf) {
+    __shared__ int im_first[kStageImages];
     __shared__ int im_m[kStageImages];
     __shared__ int im_pre[kStageImages + 1];
-    __shared__ uint64_t ckey[kStageCap];
-    __shared__ int cslot[kStageCap];
-    __shared__ double cpos[3 * kStageCap];
-    __shared__ int s_total;
+    __shared__ double cx[kStageCap], cy[kStageCap], cz[kStageCap];
+    __shared__ uint64_t cjd[kStageCap];                     // j << 32 | packed (image - shift_j)
+    __shared__ uint64_t wkeys[kStageWaves][kMaxNeigh];
+    __shared__ int cen_bin[8], cen_pre[9];
+    __shared__ int s_total, s_max;
     const int g = blockIdx.x, t = threadIdx.x;
     const int lane = t % kWave, wave = t / kWave;
-    const int a0 = bin_start[g], a1 = bin_start[g + 1];
-    if (a0 == a1) return;                                   // empty bin: no centres
-    const int s = bin_struc[g];
+    const int s = bin_struc[g];                             // structure of this super-bin
     const CellInfo c = ci[s];
     const double* R = cells + 9 * s;
-    const int gi = g - c.boff;
-    const int b0 = gi / (c.nb[1] * c.nb[2]), b1 = (gi / c.nb[2]) % c.nb[1], b2 = gi % c.nb[2];
-    const int e0 = 2 * c.k[0] + 1, e1 = 2 * c.k[1] + 1, e2 = 2 * c.k[2] + 1;
+    const int gi = g - c.sboff;
+    const int B0 = gi / (c.nsb[1] * c.nsb[2]), B1 = (gi / c.nsb[2]) % c.nsb[1], B2 = gi % c.nsb[2];
+    const int lo0 = c.sb * B0, lo1 = c.sb * B1, lo2 = c.sb * B2;
+    const int n0b = min(c.sb, c.nb[0] - lo0), n1b = min(c.sb, c.nb[1] - lo1), n2b = min(c.sb, c.nb[2] - lo2);
+    // centre bins of the super-bin and their atom ranges
+    if (t == 0) {
+        int acc = 0, q = 0;
+        for (int d0 = 0; d0 < n0b; ++d0)
+            for (int d1 = 0; d1 < n1b; ++d1)
+                for (int d2 = 0; d2 < n2b; ++d2, ++q) {
+                    const int bg = c.boff + ((lo0 + d0) * c.nb[1] + (lo1 + d1)) * c.nb[2] + (lo2 + d2);
+                    cen_bin[q] = bg;
+                    cen_pre[q] = acc;
+                    acc += bin_start[bg + 1] - bin_start[bg];
+                }
+        cen_pre[q] = acc;
+    }
+    const int ncb = n0b * n1b * n2b;
+    const int e0 = n0b + 2 * c.k[0], e1 = n1b + 2 * c.k[1], e2 = n2b + 2 * c.k[2];
     const int nimg = e0 * e1 * e2;
     if (nimg > kStageImages) {
-        if (t == 0) { bin_ovf[g] = 1; atomicOr(flag, 8); }
+        __syncthreads();
+        if (t < ncb) bin_ovf[cen_bin[t]] = 1;
+        if (t == 0) atomicOr(flag, 8);
         return;
     }
     for (int q = t; q < nimg; q += kStageThreads) {
-        const int u0 = b0 - c.k[0] + q / (e1 * e2);
-        const int u1 = b1 - c.k[1] + (q / e2) % e1;
-        const int u2 = b2 - c.k[2] + q % e2;
+        const int u0 = lo0 - c.k[0] + q / (e1 * e2);
+        const int u1 = lo1 - c.k[1] + (q / e2) % e1;
+        const int u2 = lo2 - c.k[2] + q % e2;
         const int m0 = floordiv(u0, c.nb[0]), m1 = floordiv(u1, c.nb[1]), m2 = floordiv(u2, c.nb[2]);
         const int gg = c.boff + ((u0 - m0 * c.nb[0]) * c.nb[1] + (u1 - m1 * c.nb[1])) * c.nb[2] + (u2 - m2 * c.nb[2]);
-        im_bin[q] = gg;
+        const int f = bin_start[gg];
+        im_first[q] = f;
         im_m[q] = (int)pack_img(m0, m1, m2);
-        im_pre[q + 1] = bin_start[gg + 1] - bin_start[gg];
+        im_pre[q + 1] = bin_start[gg + 1] - f;
     }
     __syncthreads();
     if (t < kWave) {   // inclusive scan of the image counts by the first wavefront
@@ -264,81 +347,84 @@ __global__ __launch_bounds__(kStageThreads) void k_search_bins(
     }
     __syncthreads();
     const int total = s_total;
+    const int ncen = cen_pre[ncb];
+    if (ncen == 0) return;                                  // no centres in this super-bin
     if (total > kStageCap) {
-        if (t == 0) { bin_ovf[g] = 1; atomicOr(flag, 8); }
+        if (t < ncb) bin_ovf[cen_bin[t]] = 1;
+        if (t == 0) atomicOr(flag, 8);
         return;
     }
-    // stage candidates: key (j, image offset - shift_j), slot, position
     for (int q = t; q < total; q += kStageThreads) {
         int lo = 0, hi = nimg;
         while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;
             if (im_pre[mid] <= q) lo = mid; else hi = mid;
         }
-        const int j = bin_atoms[bin_start[im_bin[lo]] + (q - im_pre[lo])];
-        int m0, m1, m2;
+        const double4 r4 = rec[im_first[lo] + (q - im_pre[lo])];
+        const uint64_t js = (uint64_t)__double_as_longlong(r4.w);
+        int m0, m1, m2, s0, s1, s2;
         unpack_img((uint32_t)im_m[lo], m0, m1, m2);
-        const uint32_t d = pack_img(m0 - shift[3 * j], m1 - shift[3 * j + 1], m2 - shift[3 * j + 2]);
-        ckey[q] = ((uint64_t)j << 30) | (uint64_t)d;
-        cslot[q] = q;
-        cpos[3 * q] = pos[3 * j];
-        cpos[3 * q + 1] = pos[3 * j + 1];
-        cpos[3 * q + 2] = pos[3 * j + 2];
+        unpack_img((uint32_t)(js & 0xffffffffu), s0, s1, s2);
+        cx[q] = r4.x;
+        cy[q] = r4.y;
+        cz[q] = r4.z;
+        cjd[q] = (js & 0xffffffff00000000ull) | (uint64_t)pack_img(m0 - s0, m1 - s1, m2 - s2);
     }
-    int np2 = 1;
-    while (np2 < total) np2 <<= 1;
-    for (int q = total + t; q < np2; q += kStageThreads) { ckey[q] = ~0ull; cslot[q] = 0; }
     __syncthreads();
-    for (int k = 2; k <= np2; k <<= 1) {
-        for (int jj = k >> 1; jj > 0; jj >>= 1) {
-            for (int q = t; q < np2; q += kStageThreads) {
-                const int p = q ^ jj;
-                if (p > q) {
-                    const uint64_t ka = ckey[q], kb = ckey[p];
-                    const bool up = (q & k) == 0;
-                    if ((ka > kb) == up) {
-                        ckey[q] = kb; ckey[p] = ka;
-                        const int sa = cslot[q]; cslot[q] = cslot[p]; cslot[p] = sa;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
-    // centres of the bin, one wavefront each
-    for (int ca = a0 + wave; ca < a1; ca += kStageThreads / kWave) {
-        const int i = bin_atoms[ca];
-        const int si0 = shift[3 * i], si1 = shift[3 * i + 1], si2 = shift[3 * i + 2];
-        const double pix = pos[3 * i], piy = pos[3 * i + 1], piz = pos[3 * i + 2];
-        uint64_t* dst = keys_out + (size_t)i * kMaxNeigh;
+    uint64_t* wk = wkeys[wave];
+    int wmax = 0;          // largest neighbour count of this wavefront's centres
+    if (t == 0) s_max = 0;
+    for (int cq = wave; cq < ncen; cq += kStageWaves) {
+        int cb = 0;
+        while (cb + 1 < ncb && cq >= cen_pre[cb + 1]) ++cb;
+        const double4 ri = rec[bin_start[cen_bin[cb]] + (cq - cen_pre[cb])];
+        const uint64_t jsi = (uint64_t)__double_as_longlong(ri.w);
+        const int i = (int)(jsi >> 32);
+        int si0, si1, si2;
+        unpack_img((uint32_t)(jsi & 0xffffffffu), si0, si1, si2);
         int count = 0;
         for (int base = 0; base < total; base += kWave) {
             const int q = base + lane;
             bool ok = false;
             uint64_t key = 0;
             if (q < total) {
-                const uint64_t ck = ckey[q];
-                const int sl = cslot[q];
-                const int j = (int)(ck >> 30);
+                const uint64_t jd = cjd[q];
+                const int j = (int)(jd >> 32);
                 int d0, d1, d2i;
-                unpack_img((uint32_t)(ck & 0x3fffffffu), d0, d1, d2i);
+                unpack_img((uint32_t)(jd & 0xffffffffu), d0, d1, d2i);
                 const int n0 = si0 + d0, n1 = si1 + d1, n2 = si2 + d2i;
                 double x, y, z, d2;
-                pair_vec_v(pix, piy, piz, cpos[3 * sl], cpos[3 * sl + 1], cpos[3 * sl + 2], R, n0, n1, n2, x, y, z, d2);
+                pair_vec_v(ri.x, ri.y, ri.z, cx[q], cy[q], cz[q], R, n0, n1, n2, x, y, z, d2);
                 ok = d2 > 0.0 && d2 < rc2 && !(j == i && n0 == 0 && n1 == 0 && n2 == 0);
                 key = ((uint64_t)j << 30) | (uint64_t)pack_img(n0, n1, n2);
             }
             const uint64_t mask = __ballot(ok);
             const int before = __popcll(mask & ((1ull << lane) - 1ull));
-            if (ok && count + before < kMaxNeigh) dst[count + before] = key;
+            if (ok && count + before < kMaxNeigh) wk[count + before] = key;
             count += __popcll(mask);
         }
+        const int n = count < kMaxNeigh ? count : kMaxNeigh;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        uint64_t* dst = keys_out + (size_t)i * kMaxNeigh;
+        if (n <= kWave) sort_and_store<1>(wk, n, dst);
+        else if (n <= 2 * kWave) sort_and_store<2>(wk, n, dst);
+        else if (n <= 4 * kWave) sort_and_store<4>(wk, n, dst);
+        else sort_and_store<8>(wk, n, dst);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         if (lane == 0) {
-            cnt_out[i] = count < kMaxNeigh ? count : kMaxNeigh;
+            cnt_out[i] = n;
             if (count > kMaxNeigh) atomicOr(flag, 1);
-            atomicMax(flag + 1, count);
         }
+        wmax = max(wmax, count);
     }
+    // one global atomic per block (a contended atomic per atom serialises the kernel)
+    if (lane == 0) atomicMax(&s_max, wmax);
+    __syncthreads();
+    if (t == 0) atomicMax(flag + 1, s_max);
 }
 
 // CSR fill: index, packed image, neighbour species and FP32 pair geometry
@@ -454,7 +540,15 @@ static hipError_t grid_setup(ann_ctx* ctx, double rc) {
         c.nbins = c.nb[0] * c.nb[1] * c.nb[2];
         c.boff = nbins;
         nbins += c.nbins;
-        bin_struc.insert(bin_struc.end(), c.nbins, s);
+        // 2x2x2 bins per staged block when the enlarged search box still fits
+        const double per_bin = (double)(ctx->atom_off[s + 1] - ctx->atom_off[s]) / c.nbins;
+        const double box2 = (double)(std::min(2, c.nb[0]) + 2 * c.k[0]) * (std::min(2, c.nb[1]) + 2 * c.k[1]) *
+                            (std::min(2, c.nb[2]) + 2 * c.k[2]);
+        c.sb = (box2 * per_bin <= 0.7 * kStageCap && box2 <= kStageImages) ? 2 : 1;
+        for (int a = 0; a < 3; ++a) c.nsb[a] = (c.nb[a] + c.sb - 1) / c.sb;
+        const int nsbins = c.nsb[0] * c.nsb[1] * c.nsb[2];
+        c.sboff = (int)bin_struc.size();
+        bin_struc.insert(bin_struc.end(), nsbins, s);       // structure of each super-bin
     }
     if (S > ctx->struc_cap || !ctx->d_cellinfo) {
         if (ctx->d_cellinfo) { hipFree(ctx->d_cellinfo); hipFree(ctx->d_cellmat); }
@@ -473,11 +567,12 @@ static hipError_t grid_setup(ann_ctx* ctx, double rc) {
     ctx->d_bin_ovf = ctx->d_bin_cnt + 2 * (size_t)ctx->bin_cap;
     HIPCHK(hipMemcpyAsync(ctx->d_cellinfo, info.data(), sizeof(CellInfo) * S, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(ctx->d_cellmat, rinv.data(), sizeof(double) * 9 * S, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(ctx->d_bin_struc, bin_struc.data(), sizeof(int) * nbins, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(ctx->d_bin_struc, bin_struc.data(), sizeof(int) * bin_struc.size(), hipMemcpyHostToDevice, st));
     HIPCHK(hipStreamSynchronize(st));      // the host vectors go out of scope
     ctx->grid_valid = true;
     ctx->grid_rc = rc;
     ctx->grid_nbins = nbins;
+    ctx->grid_nsb = (int)bin_struc.size();
     return hipSuccess;
 }
 
@@ -489,6 +584,9 @@ hipError_t neighbors_build(ann_ctx* ctx, double rc) {
     int64_t kc = ctx->keys_cap;
     HIPCHK(grow(ctx->d_keys, kc, (int64_t)N * kMaxNeigh));
     ctx->keys_cap = kc;
+    int64_t rcap = ctx->binrec_cap;
+    HIPCHK(grow(ctx->d_binrec, rcap, (int64_t)N));
+    ctx->binrec_cap = rcap;
     const CellInfo* dci = reinterpret_cast<const CellInfo*>(ctx->d_cellinfo);
     int* bin_fill = ctx->d_bin_cnt + (nbins + 1);
     HIPCHK(hipMemsetAsync(ctx->d_bin_cnt, 0, sizeof(int) * 3 * (size_t)ctx->bin_cap, st));
@@ -500,12 +598,12 @@ hipError_t neighbors_build(ann_ctx* ctx, double rc) {
     }
     HIPCHK(exclusive_scan(ctx->d_bin_cnt, ctx->d_bin_start, nbins + 1, st));
     k_fill_bins<<<(N + 255) / 256, 256, 0, st>>>(N, ctx->d_bin_of, ctx->d_bin_start, bin_fill,
-                                                ctx->d_bin_atoms);
+                                                ctx->d_bin_atoms, ctx->d_pos, ctx->d_shift, ctx->d_binrec);
     {
         Launch L(ctx, "nb_search");
-        k_search_bins<<<nbins, kStageThreads, 0, st>>>(ctx->d_pos, dci, ctx->d_cells, ctx->d_bin_struc, ctx->d_shift,
-                                                      ctx->d_bin_start, ctx->d_bin_atoms, rc * rc, ctx->d_keys,
-                                                      ctx->d_cnt, ctx->d_flag, ctx->d_bin_ovf);
+        k_search_bins<<<ctx->grid_nsb, kStageThreads, 0, st>>>(dci, ctx->d_cells, ctx->d_bin_struc, ctx->d_bin_start,
+                                                      ctx->d_binrec, rc * rc, ctx->d_keys, ctx->d_cnt, ctx->d_flag,
+                                                      ctx->d_bin_ovf);
     }
     HIPCHK(hipGetLastError());
     HIPCHK(exclusive_scan(ctx->d_cnt, ctx->nl.off, N + 1, st));
