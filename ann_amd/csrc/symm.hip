@@ -316,7 +316,7 @@ __device__ __forceinline__ int fast_load(const SymmArgs& A, const DevModel& M, i
             }
         }
         if (keep) {
-            const float r = gq.w, ir = 1.0f / r;
+            const float r = gq.w, ir = __builtin_amdgcn_rcpf(r);   // 1 ulp
             sm.g[pos] = make_float4(gq.x * ir, gq.y * ir, gq.z * ir, ir);
             sm.r[pos] = r;
             sm.canon[pos] = t;
@@ -803,8 +803,8 @@ __global__ __launch_bounds__(64) void k_moment(SymmArgs A) {
 #pragma unroll
             for (int u = 0; u < U4; ++u) { acc[u] = 0.0f; dacc[u] = 0.0f; }
             if (act) {
-                for (int J = s0 + grp; J < s1; J += NG) {
-                    const float* row = pw + (size_t)J * PWS;
+                // one neighbour of the segment: T[u][alpha] += e_u(r) r^^alpha
+                auto step = [&](const float* row, int J) {
                     const float px = row[ea], py = row[ZP + eb], pz = row[2 * ZP + ec];
                     const float mono = px * py * pz;
                     const float4 e4 = sm.ef[J];
@@ -819,7 +819,18 @@ __global__ __launch_bounds__(64) void k_moment(SymmArgs A) {
                         dacc[2] += f4.z * wm + e4.z * dmono;
                         dacc[3] += f4.w * wm + e4.w * dmono;
                     }
+                };
+                int J = s0 + grp;
+                // four neighbours per trip: their power rows sit at fixed offsets
+                // from one address, so the loads use immediate offsets and overlap
+                for (; J + 3 * NG < s1; J += 4 * NG) {
+                    const float* row = pw + (size_t)J * PWS;
+                    step(row, J);
+                    step(row + NG * PWS, J + NG);
+                    step(row + 2 * NG * PWS, J + 2 * NG);
+                    step(row + 3 * NG * PWS, J + 3 * NG);
                 }
+                for (; J < s1; J += NG) step(pw + (size_t)J * PWS, J);
             }
             if (NG == 1) {
                 if (act) {
