@@ -199,16 +199,20 @@ struct GradArgs {
     const float* scr;
     const int* spcount;
     int64_t base[kMaxSpecies];
-    const int4* tiles;        // (species, layer, n0, m0); m0 < 0: bias tile
+    const int4* tiles;        // (species, layer, n0, m0) of 32 x 32 output tiles
     int nsplit;
     double* part;             // [nsplit][nparams]
 };
 
-// One block per (16 x 16 output tile, atom split): sums over the split's atoms
-// of zbar (x) x + zdbar (x) xd, staged through LDS 64 atoms at a time; FP32
-// within a 64-atom chunk, FP64 across chunks.
-__global__ __launch_bounds__(256) void k_grad_tiles(GradArgs A, int nparams) {
-    __shared__ float sz[64][17], szd[64][17], sxx[64][17], sxd[64][17];
+// Parameter gradient of one layer of one species as GEMMs over the atoms on
+// the matrix cores (v_mfma_f32_16x16x4_f32: FP32 operands, FP32 accumulate):
+//   dL/dW[n][m] = sum_atoms zbar[n] x[m] + zdbar[n] xd[m],   dL/db[n] = sum zbar[n]
+// with the bias as the extra column m = ni (x = 1, xd = 0).  One wavefront per
+// (32 x 32 output tile, atom split), four atoms per instruction; FP32 over
+// 64 atoms, then FP64 (the same rounding scheme as a 64-atom FP32 chunk).
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(64) void k_grad_mfma(GradArgs A, int nparams) {
     const int4 tl = A.tiles[blockIdx.x];
     const int split = blockIdx.y;
     const int a = tl.x, l = tl.y, n0 = tl.z, m0 = tl.w;
@@ -219,50 +223,70 @@ __global__ __launch_bounds__(256) void k_grad_tiles(GradArgs A, int nparams) {
     const int no = D.nodes[l];
     const int na = A.spcount[a];
     const int per = (na + A.nsplit - 1) / A.nsplit;
-    const int r0 = split * per, r1 = min(na, r0 + per);
+    const int r0 = min(na, split * per), r1 = min(na, r0 + per);
     const float* base = A.scr + A.base[a] + lo;
-    const int t = threadIdx.x, tn = t / 16, tm = t % 16;
-    const bool bias = m0 < 0;
-    double acc = 0.0;
-    for (int c0 = r0; c0 < r1; c0 += 64) {
-        for (int e = t; e < 64 * 16; e += 256) {
-            const int r = e / 16, k = e % 16;
-            const bool ok = c0 + r < r1;
-            const float* rr = base + (int64_t)(c0 + r) * stride;
-            const int n = n0 + k, m = m0 + k;
-            sz[r][k] = ok && n < no ? rr[2 * ni + n] : 0.0f;
-            szd[r][k] = ok && n < no ? rr[2 * ni + no + n] : 0.0f;
-            sxx[r][k] = ok && !bias && m < ni ? rr[m] : 0.0f;
-            sxd[r][k] = ok && !bias && m < ni ? rr[ni + m] : 0.0f;
+    const int lane = threadIdx.x, c16 = lane % 16, k4 = lane / 16;
+    const int nA[2] = {n0 + c16, n0 + 16 + c16};          // A operand rows (outputs n)
+    const int mB[2] = {m0 + c16, m0 + 16 + c16};          // B operand columns (inputs m, bias at ni)
+    floatx4 acc[2][2];
+    double dacc[2][2][4];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int v = 0; v < 2; ++v) {
+            acc[u][v] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) dacc[u][v][q] = 0.0;
         }
-        __syncthreads();
-        float s = 0.0f;
-        if (bias) {
-            if (tm == 0)
-                for (int r = 0; r < 64; ++r) s += sz[r][tn];
-        } else {
-            for (int r = 0; r < 64; ++r) s += sz[r][tn] * sxx[r][tm] + szd[r][tn] * sxd[r][tm];
+    int steps = 0;
+    for (int r = r0; r < r1; r += 4) {
+        const int row = r + k4;
+        const bool ok = row < r1;
+        const float* rr = base + (int64_t)(ok ? row : r0) * stride;
+        float z[2], zd[2], x[2], xd[2];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const bool okn = ok && nA[u] < no;
+            z[u] = okn ? rr[2 * ni + nA[u]] : 0.0f;
+            zd[u] = okn ? rr[2 * ni + no + nA[u]] : 0.0f;
+            const int m = mB[u];
+            x[u] = ok ? (m < ni ? rr[m] : (m == ni ? 1.0f : 0.0f)) : 0.0f;
+            xd[u] = ok && m < ni ? rr[ni + m] : 0.0f;
         }
-        acc += (double)s;
-        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int v = 0; v < 2; ++v) {
+                acc[u][v] = __builtin_amdgcn_mfma_f32_16x16x4f32(z[u], x[v], acc[u][v], 0, 0, 0);
+                acc[u][v] = __builtin_amdgcn_mfma_f32_16x16x4f32(zd[u], xd[v], acc[u][v], 0, 0, 0);
+            }
+        if (++steps == 16) {
+            steps = 0;
+#pragma unroll
+            for (int u = 0; u < 2; ++u)
+#pragma unroll
+                for (int v = 0; v < 2; ++v) {
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) dacc[u][v][q] += (double)acc[u][v][q];
+                    acc[u][v] = floatx4{0.f, 0.f, 0.f, 0.f};
+                }
+        }
     }
-    const int n = n0 + tn;
-    if (n >= no) return;
-    int p;
-    if (bias) {
-        if (tm != 0) return;
-        int bo = 0;
-        for (int q = 0; q < l; ++q) bo += D.nodes[q];
-        p = D.poff + bo + n;
-    } else {
-        const int m = m0 + tm;
-        if (m >= ni) return;
-        int wo = 0, prev = D.nin;
-        for (int q = 0; q < D.nlayer; ++q) wo += D.nodes[q];
-        for (int q = 0; q < l; ++q) { wo += D.nodes[q] * prev; prev = D.nodes[q]; }
-        p = D.poff + wo + m * no + n;
-    }
-    A.part[(size_t)split * nparams + p] = acc;
+    int bo = 0, wo = 0, prev = D.nin;
+    for (int q = 0; q < l; ++q) bo += D.nodes[q];
+    for (int q = 0; q < D.nlayer; ++q) wo += D.nodes[q];
+    for (int q = 0; q < l; ++q) { wo += D.nodes[q] * prev; prev = D.nodes[q]; }
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int v = 0; v < 2; ++v)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int n = n0 + 16 * u + 4 * k4 + q, m = m0 + 16 * v + c16;
+                if (n >= no || m > ni) continue;
+                const int p = m < ni ? D.poff + wo + m * no + n : D.poff + bo + n;
+                A.part[(size_t)split * nparams + p] = dacc[u][v][q] + (double)acc[u][v][q];
+            }
 }
 
 // fixed-order sum over the atom splits
@@ -391,23 +415,22 @@ hipError_t net_bwd2(ann_ctx* ctx, const float* G, const float* v, const double* 
         k_net_bwd2<<<(A.N + 3) / 4, 64, 0, ctx->stream>>>(A);
     }
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    // tiles of every species and layer (biases as their own tiles)
+    // 32 x 32 output tiles of every species and layer (bias = column ni)
     if (ctx->grad_tiles.empty()) {
         for (int s = 0; s < ctx->S; ++s) {
             const NetDesc& D = ctx->hm.net[s];
             int ni = D.nin;
             for (int l = 0; l < D.nlayer; ++l) {
                 const int no = D.nodes[l];
-                for (int n0 = 0; n0 < no; n0 += 16) {
-                    ctx->grad_tiles.insert(ctx->grad_tiles.end(), {s, l, n0, -1});
-                    for (int m0 = 0; m0 < ni; m0 += 16) ctx->grad_tiles.insert(ctx->grad_tiles.end(), {s, l, n0, m0});
-                }
+                for (int n0 = 0; n0 < no; n0 += 32)
+                    for (int m0 = 0; m0 <= ni; m0 += 32) ctx->grad_tiles.insert(ctx->grad_tiles.end(), {s, l, n0, m0});
                 ni = no;
             }
         }
     }
     const int ntiles = (int)ctx->grad_tiles.size() / 4;
-    const int nsplit = 8;
+    // atom splits: about 2048 wavefronts in flight, fixed for a given model
+    const int nsplit = std::max(8, std::min(64, 2048 / std::max(ntiles, 1)));
     if (!ctx->d_grad_tiles) {
         if ((e = hipMalloc(&ctx->d_grad_tiles, sizeof(int) * ctx->grad_tiles.size())) != hipSuccess) return e;
         if ((e = hipMemcpy(ctx->d_grad_tiles, ctx->grad_tiles.data(), sizeof(int) * ctx->grad_tiles.size(),
@@ -429,7 +452,7 @@ hipError_t net_bwd2(ann_ctx* ctx, const float* G, const float* v, const double* 
     R.part = ctx->d_gsplit;
     {
         Launch L(ctx, "grad_reduce");
-        k_grad_tiles<<<dim3(ntiles, nsplit), 256, 0, ctx->stream>>>(R, ctx->nparams);
+        k_grad_mfma<<<dim3(ntiles, nsplit), 64, 0, ctx->stream>>>(R, ctx->nparams);
         k_grad_combine<<<(ctx->nparams + 255) / 256, 256, 0, ctx->stream>>>(ctx->nparams, nsplit, ctx->d_gsplit, grad);
     }
     return hipGetLastError();
