@@ -144,7 +144,9 @@ struct DevBuf {
 
 int form_from_env() {
     const char* f = std::getenv("ANN_FORM");
-    return (f && std::string(f) == "plain") ? ANN_FORM_PLAIN : ANN_FORM_FAST;
+    if (f && std::string(f) == "plain") return ANN_FORM_PLAIN;
+    if (f && std::string(f) == "triplet") return ANN_FORM_TRIPLET;
+    return ANN_FORM_FAST;
 }
 
 }  // namespace
