@@ -28,14 +28,14 @@ static int pool_event(ann_ctx* ctx) {
 }
 
 Launch::Launch(ann_ctx* c, const char* n) : ctx(c), name(n) {
-    if (!ctx->prof) return;
+    if (!ctx->prof || ctx->capturing) return;
     const int a = pool_event(ctx), b = pool_event(ctx);
     if (a < 0 || b < 0) return;
     hipEventRecord(ctx->evpool[a], ctx->stream);
     ctx->timers[name].ev.push_back({a, b});
 }
 Launch::~Launch() {
-    if (!ctx->prof) return;
+    if (!ctx->prof || ctx->capturing) return;
     auto& v = ctx->timers[name].ev;
     if (!v.empty()) hipEventRecord(ctx->evpool[v.back().second], ctx->stream);
 }
@@ -291,6 +291,8 @@ static int upload_model(ann_ctx* ctx, const ann_model_desc* d) {
     ANN_HIP(hipMemset(ctx->da.adam_v, 0, sizeof(double) * ptot));
     ANN_HIP(hipMalloc(&ctx->d_flag, sizeof(int) * 4));
     ANN_HIP(hipMemset(ctx->d_flag, 0, sizeof(int) * 4));
+    ANN_HIP(hipMalloc(&ctx->d_adam_t, sizeof(int)));
+    ANN_HIP(hipMemset(ctx->d_adam_t, 0, sizeof(int)));
     return ANN_OK;
 }
 
@@ -321,10 +323,14 @@ extern "C" int ann_destroy(ann_ctx* ctx) {
                     ctx->d_Eatom, ctx->d_P, ctx->d_u, ctx->d_v, ctx->d_F, ctx->d_E, ctx->d_gE, ctx->d_atomvir,
                     ctx->d_scr, ctx->d_gpart, ctx->d_rank, ctx->d_spcount, ctx->d_perm, ctx->d_tiles, ctx->d_spend, ctx->d_grad_tiles, ctx->d_gsplit,
                     ctx->d_chunks, ctx->d_chunk_off, ctx->d_part, ctx->d_pos_ref, ctx->d_keatom, ctx->d_md_flag,
-                    ctx->d_bin_struc, ctx->d_binrec};
+                    ctx->d_bin_struc, ctx->d_binrec, ctx->d_adam_t};
     for (void* p : ptrs)
         if (p) hipFree(p);
     for (hipEvent_t e : ctx->evpool) hipEventDestroy(e);
+    if (ctx->gexec) hipGraphExecDestroy(ctx->gexec);
+    for (hipEvent_t e : ctx->gev)
+        if (e) hipEventDestroy(e);
+    if (ctx->gstream) hipStreamDestroy(ctx->gstream);
     delete ctx;
     return ANN_OK;
 }
@@ -352,7 +358,7 @@ extern "C" int ann_set_params(ann_ctx* ctx, const double* p, int n) {
     ANN_HIP(hipMemcpyAsync(ctx->da.params, p, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
     ANN_HIP(hipMemcpyAsync(ctx->da.W, W.data(), sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
     ANN_HIP(hipStreamSynchronize(ctx->stream));
-    ctx->adam_t = 0;
+    ANN_HIP(hipMemsetAsync(ctx->d_adam_t, 0, sizeof(int), ctx->stream));
     ANN_HIP(hipMemsetAsync(ctx->da.adam_m, 0, sizeof(double) * n, ctx->stream));
     ANN_HIP(hipMemsetAsync(ctx->da.adam_v, 0, sizeof(double) * n, ctx->stream));
     return ANN_OK;
@@ -436,6 +442,7 @@ extern "C" int ann_set_batch(ann_ctx* ctx, const ann_batch* b) {
     ctx->have_batch = true;
     ctx->have_nlist = false;
     ctx->grid_valid = false;
+    ctx->gen++;
     ctx->G_valid = false;
     ctx->rank_valid = false;
     ctx->perm_valid = false;
@@ -455,6 +462,7 @@ extern "C" int ann_neighbors(ann_ctx* ctx, double rc) {
     ctx->G_valid = false;
     ctx->nl_rc = r;
     ctx->pos_ref_valid = false;   // the skin reference is taken by the next MD step
+    ctx->gen++;
     return ANN_OK;
 }
 
@@ -464,6 +472,7 @@ extern "C" int ann_refresh_geometry(ann_ctx* ctx) {
     hipError_t e = refresh_geometry(ctx);
     if (e != hipSuccess) return fail(ANN_ERR_HIP, std::string("refresh_geometry: ") + hipGetErrorString(e));
     ctx->G_valid = false;
+    ctx->gen++;
     return ANN_OK;
 }
 
@@ -474,6 +483,7 @@ extern "C" int ann_wrap_positions(ann_ctx* ctx) {
     if (e != hipSuccess) return fail(ANN_ERR_HIP, std::string("wrap_positions: ") + hipGetErrorString(e));
     ctx->have_nlist = false;      // image vectors of the old list refer to the unwrapped positions
     ctx->G_valid = false;
+    ctx->gen++;
     return ANN_OK;
 }
 
@@ -638,6 +648,7 @@ extern "C" int ann_md_step(ann_ctx* ctx, int form, double dt, const double* mass
     RET_HIP(md_kick_drift(ctx, dt, mass, F, vel, pos, 0.25 * skin * skin), "md_kick_drift");
     RET_HIP(refresh_geometry(ctx), "refresh_geometry");
     ctx->G_valid = false;
+    ctx->gen++;
     RET_HIP(symm_fwd(ctx, form, ctx->d_G), "symm_fwd");
     RET_HIP(net_fwd(ctx, form, ctx->d_G, ctx->d_Eatom, ctx->d_dEdG), "net_fwd");
     RET_HIP(symm_bwd(ctx, form, ctx->d_dEdG, ctx->d_P), "symm_bwd");
@@ -674,7 +685,64 @@ extern "C" int ann_train_step(ann_ctx* ctx, int form, const double* Eref, const 
     if ((rc = check_form(form))) return rc;
     if (!Eref || !Fref) return fail(ANN_ERR_ARG, "null reference energies/forces");
     RET_HIP(ensure_work(ctx), "allocation");
+    // graph replay: same inputs, outputs and weights as the captured step, and
+    // no change of positions, list or batch since (every kernel reads its
+    // buffers by pointer, and the Adam step counter lives on the device)
+    const void* ptr[4] = {Eref, Fref, loss, grad};
+    const bool same = ctx->gexec && ctx->gkey_gen == ctx->gen && ctx->gkey_form == form &&
+                      std::memcmp(ptr, ctx->gkey_ptr, sizeof(ptr)) == 0 && ctx->gkey_w[0] == wE &&
+                      ctx->gkey_w[1] == wF && ctx->gkey_w[2] == gamma;
+    if (ctx->graph_on && ctx->train_warm && ctx->G_valid) {
+        if (!same) {
+            if (ctx->gexec) { hipGraphExecDestroy(ctx->gexec); ctx->gexec = nullptr; }
+            hipStream_t user = ctx->stream;
+            RET_HIP(hipEventRecord(ctx->gev[0], user), "graph event");
+            RET_HIP(hipStreamWaitEvent(ctx->gstream, ctx->gev[0], 0), "graph wait");
+            ctx->stream = ctx->gstream;
+            ctx->capturing = true;
+            hipGraph_t graph = nullptr;
+            hipError_t e = hipStreamBeginCapture(ctx->gstream, hipStreamCaptureModeRelaxed);
+            hipError_t e2 = e == hipSuccess ? train_step_impl(ctx, form, Eref, Fref, wE, wF, gamma, loss, grad) : e;
+            hipError_t e3 = hipStreamEndCapture(ctx->gstream, &graph);
+            ctx->capturing = false;
+            ctx->stream = user;
+            if (e != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
+                return fail(ANN_ERR_HIP, std::string("train step capture: ") +
+                                             hipGetErrorString(e != hipSuccess ? e : (e2 != hipSuccess ? e2 : e3)));
+            e = hipGraphInstantiate(&ctx->gexec, graph, nullptr, nullptr, 0);
+            hipGraphDestroy(graph);
+            if (e != hipSuccess) { ctx->gexec = nullptr; return fail(ANN_ERR_HIP, std::string("graph: ") + hipGetErrorString(e)); }
+            ctx->gkey_gen = ctx->gen;
+            ctx->gkey_form = form;
+            std::memcpy(ctx->gkey_ptr, ptr, sizeof(ptr));
+            ctx->gkey_w[0] = wE; ctx->gkey_w[1] = wF; ctx->gkey_w[2] = gamma;
+        } else {
+            RET_HIP(hipEventRecord(ctx->gev[0], ctx->stream), "graph event");
+            RET_HIP(hipStreamWaitEvent(ctx->gstream, ctx->gev[0], 0), "graph wait");
+        }
+        RET_HIP(hipGraphLaunch(ctx->gexec, ctx->gstream), "graph launch");
+        RET_HIP(hipEventRecord(ctx->gev[1], ctx->gstream), "graph event");
+        RET_HIP(hipStreamWaitEvent(ctx->stream, ctx->gev[1], 0), "graph wait");
+        return ANN_OK;
+    }
     RET_HIP(train_step_impl(ctx, form, Eref, Fref, wE, wF, gamma, loss, grad), "train_step");
+    ctx->train_warm = true;
+    return ANN_OK;
+}
+
+extern "C" int ann_train_graph(ann_ctx* ctx, int enable) {
+    CHECK_CTX(ctx);
+    if (enable && !ctx->gstream) {
+        ANN_HIP(hipStreamCreateWithFlags(&ctx->gstream, hipStreamNonBlocking));
+        ANN_HIP(hipEventCreateWithFlags(&ctx->gev[0], hipEventDisableTiming));
+        ANN_HIP(hipEventCreateWithFlags(&ctx->gev[1], hipEventDisableTiming));
+    }
+    if (!enable && ctx->gexec) {
+        ANN_HIP(hipStreamSynchronize(ctx->gstream));
+        ANN_HIP(hipGraphExecDestroy(ctx->gexec));
+        ctx->gexec = nullptr;
+    }
+    ctx->graph_on = enable != 0;
     return ANN_OK;
 }
 

@@ -187,7 +187,17 @@ struct ann_ctx {
     double* d_gpart = nullptr;        // partial gradients
     int64_t gpart_cap = 0;
     int64_t work_atoms = 0, work_pairs = 0;
-    int adam_t = 0;
+    int* d_adam_t = nullptr;          // Adam step counter (device, so a captured step can advance it)
+    // training step captured as a HIP graph and replayed
+    bool graph_on = false, capturing = false, train_warm = false;
+    hipStream_t gstream = nullptr;    // private non-blocking stream the graph is captured on and replayed
+    hipEvent_t gev[2] = {nullptr, nullptr};
+    hipGraphExec_t gexec = nullptr;
+    uint64_t gen = 0;                 // bumped whenever positions, the list or the batch change
+    uint64_t gkey_gen = 0;
+    int gkey_form = -1;
+    const void* gkey_ptr[4] = {nullptr, nullptr, nullptr, nullptr};
+    double gkey_w[3] = {0, 0, 0};
     // per-structure sums: structures split into chunks of at most kSumChunk atoms
     int nchunks = 0, chunk_cap = 0;
     int* d_chunks = nullptr;          // [nchunks][3]: structure, first atom, end atom

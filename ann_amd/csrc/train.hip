@@ -504,12 +504,16 @@ __global__ void k_loss_total(int B, const double* lossE, const double* lossF, do
     if (t == 0) { out[0] = red[0][0]; out[1] = red[1][0]; }
 }
 
+// step t = *tdev + 1 (the counter lives on the device so that a captured
+// training step advances it on every replay; k_adam_tick increments it)
 __global__ void k_adam(int n, double* __restrict__ p, double* __restrict__ m1, double* __restrict__ m2,
-                       const double* __restrict__ g, double gamma, double b1ii, double b2ii, double eps,
+                       const double* __restrict__ g, double gamma, const int* __restrict__ tdev, double eps,
                        float* __restrict__ W) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double b1 = 0.9, b2 = 0.999;
+    const double t = (double)(*tdev + 1);
+    const double b1ii = 1.0 / (1.0 - pow(b1, t)), b2ii = 1.0 / (1.0 - pow(b2, t));
     const double gi = g[i];
     const double a = m1[i] * b1 + (1.0 - b1) * gi;
     const double b = m2[i] * b2 + (1.0 - b2) * gi * gi;
@@ -519,6 +523,8 @@ __global__ void k_adam(int n, double* __restrict__ p, double* __restrict__ m1, d
     p[i] = np;
     W[i] = (float)np;
 }
+
+__global__ void k_adam_tick(int* __restrict__ tdev) { *tdev += 1; }
 
 hipError_t train_step_impl(ann_ctx* ctx, int form, const double* Eref, const double* Fref, double wE, double wF,
                            double gamma, double* loss, double* grad) {
@@ -552,13 +558,11 @@ hipError_t train_step_impl(ann_ctx* ctx, int form, const double* Eref, const dou
     if ((e = symm_tangent(ctx, form, ctx->d_u, ctx->d_v)) != hipSuccess) return e;
     if ((e = net_bwd2(ctx, ctx->d_G, ctx->d_v, ctx->d_gE, g)) != hipSuccess) return e;
     if (gamma > 0) {
-        ctx->adam_t += 1;
-        const double b1ii = 1.0 / (1.0 - std::pow(0.9, ctx->adam_t));
-        const double b2ii = 1.0 / (1.0 - std::pow(0.999, ctx->adam_t));
         Launch L(ctx, "adam");
         k_adam<<<(ctx->nparams + 255) / 256, 256, 0, ctx->stream>>>(ctx->nparams, ctx->da.params, ctx->da.adam_m,
-                                                                    ctx->da.adam_v, g, gamma, b1ii, b2ii, 1e-16,
+                                                                    ctx->da.adam_v, g, gamma, ctx->d_adam_t, 1e-16,
                                                                     ctx->da.W);
+        k_adam_tick<<<1, 1, 0, ctx->stream>>>(ctx->d_adam_t);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     return hipSuccess;

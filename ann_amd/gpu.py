@@ -54,7 +54,7 @@ EXPORTS = ["ann_last_error", "ann_create", "ann_destroy", "ann_set_stream", "ann
            "ann_set_scaling", "ann_set_batch", "ann_neighbors", "ann_nlist_size", "ann_nlist_copy",
            "ann_symm_fwd", "ann_net_fwd", "ann_symm_bwd", "ann_force_reduce", "ann_energy_forces",
            "ann_symm_tangent", "ann_net_bwd2", "ann_train_step", "ann_get_params", "ann_profile",
-           "ann_profile_get", "ann_refresh_geometry", "ann_wrap_positions", "ann_md_step"]
+           "ann_profile_get", "ann_refresh_geometry", "ann_wrap_positions", "ann_md_step", "ann_train_graph"]
 
 
 def lib():
@@ -212,12 +212,20 @@ class Engine:
         return g
 
     def train_step(self, Eref: torch.Tensor, Fref: torch.Tensor, wE: float = 1.0, wF: float = 0.1,
-                   gamma: float = 0.0, form: int = FAST):
-        loss = torch.empty(2, dtype=torch.float64, device=self.device)
-        g = torch.empty(self.nparams, dtype=torch.float64, device=self.device)
+                   gamma: float = 0.0, form: int = FAST, out=None):
+        """One training step; ``out`` = (loss[2], grad[nparams]) FP64 buffers to reuse
+        (a captured step is replayed only while its buffers stay the same)."""
+        if out is None:
+            out = (torch.empty(2, dtype=torch.float64, device=self.device),
+                   torch.empty(self.nparams, dtype=torch.float64, device=self.device))
+        loss, g = out
         _check(lib().ann_train_step(self.h, form, _ptr(Eref.contiguous()), _ptr(Fref.contiguous()),
                                     C.c_double(wE), C.c_double(wF), C.c_double(gamma), _ptr(loss), _ptr(g)))
         return loss, g
+
+    def train_graph(self, enable: bool = True) -> None:
+        """Capture the training step as a HIP graph after its first call and replay it."""
+        _check(lib().ann_train_graph(self.h, int(enable)))
 
     # ------------------------------------------------------------ molecular dynamics
     def refresh_geometry(self) -> None:

@@ -137,7 +137,7 @@ def time_ef(eng, form, steps, warmup, out):
     return (time.perf_counter() - t0) / steps
 
 
-def run_train(steps, warmup, form):
+def run_train(steps, warmup, form, graph=False):
     import torch
     from ann_amd import configs
     from ann_amd.gpu import Engine
@@ -153,15 +153,19 @@ def run_train(steps, warmup, form):
     eng = Engine(model)
     eng.set_batch(batch)
     eng.neighbors()
+    if graph:
+        eng.train_graph(True)
+    out = (torch.empty(2, dtype=torch.float64, device=eng.device),
+           torch.empty(eng.nparams, dtype=torch.float64, device=eng.device))
     loss0 = None
-    for _ in range(warmup):
-        loss, _ = eng.train_step(Eref, Fref, 1.0, 0.1, 1e-3, form)
+    for _ in range(max(warmup, 2)):
+        loss, _ = eng.train_step(Eref, Fref, 1.0, 0.1, 1e-3, form, out=out)
         if loss0 is None:
             loss0 = loss.cpu().numpy().copy()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(steps):
-        loss, _ = eng.train_step(Eref, Fref, 1.0, 0.1, 1e-3, form)
+        loss, _ = eng.train_step(Eref, Fref, 1.0, 0.1, 1e-3, form, out=out)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
     return {"structs_per_s": batch.nstruc / dt, "ms_per_step": dt * 1e3, "batch": batch.nstruc,
@@ -350,6 +354,7 @@ def main():
     train = None
     if not args.no_train:
         train = run_train(max(3, args.steps // 2), max(1, args.warmup // 2), FAST)
+        train["graph_structs_per_s"] = run_train(max(3, args.steps // 2), 1, FAST, graph=True)["structs_per_s"]
         train_plain = run_train(max(3, args.steps // 4), 1, PLAIN)
         train["plain_structs_per_s"] = train_plain["structs_per_s"]
         train["triplet_structs_per_s"] = run_train(max(3, args.steps // 2), 1, TRIPLET)["structs_per_s"]

@@ -182,6 +182,12 @@ int ann_net_bwd2(ann_ctx* ctx, const float* d_G, const float* d_v, const double*
  * src/opt/algo.cpp:280-296) when gamma > 0.  d_loss [2] = (energy, force) */
 int ann_train_step(ann_ctx* ctx, int form, const double* d_Eref, const double* d_Fref, double wE,
                    double wF, double gamma, double* d_loss, double* d_grad);
+/* enable (1) or disable (0) HIP-graph capture of ann_train_step: after one
+ * eager call the step is captured on a private stream and later calls with the
+ * same form, buffers and weights replay it (ordered after prior work on the
+ * context stream).  A new batch, list or positions re-captures.  No
+ * counterpart in the reference (its trainer is host-driven). */
+int ann_train_graph(ann_ctx* ctx, int enable);
 /* copy the FP64 master parameters (device) to host */
 int ann_get_params(ann_ctx* ctx, double* params, int n);
 

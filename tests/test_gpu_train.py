@@ -102,3 +102,28 @@ def test_adam_update_matches_oracle(oracle_mod):
     pg = eng.get_params()
     assert np.allclose(pg, p, rtol=0, atol=1e-12)
     assert not np.allclose(pg, p0)
+
+
+def test_train_graph_replays_match_eager():
+    """Six Adam steps eagerly and through the captured graph (replays with
+    eager work and synchronisations between them): bitwise identical losses,
+    gradients and parameters after every step."""
+    import torch
+    from ann_amd.gpu import FAST
+    model, batch, e1 = _setup("lips3")
+    _, _, e2 = _setup("lips3")
+    e2.train_graph(True)
+    rng = np.random.default_rng(12)
+    Eref = torch.tensor(rng.normal(size=batch.nstruc), device="cuda:0")
+    Fref = torch.tensor(rng.normal(size=(batch.ntotal, 3)), device="cuda:0")
+    o1 = (torch.empty(2, dtype=torch.float64, device="cuda:0"),
+          torch.empty(e1.nparams, dtype=torch.float64, device="cuda:0"))
+    o2 = (torch.empty(2, dtype=torch.float64, device="cuda:0"),
+          torch.empty(e2.nparams, dtype=torch.float64, device="cuda:0"))
+    for step in range(6):
+        e1.train_step(Eref, Fref, 1.0, 0.1, 1e-3, FAST, out=o1)
+        e2.train_step(Eref, Fref, 1.0, 0.1, 1e-3, FAST, out=o2)
+        torch.cuda.synchronize()
+        assert torch.equal(o1[0], o2[0]), step
+        assert torch.equal(o1[1], o2[1]), step
+        assert np.array_equal(e1.get_params(), e2.get_params()), step
