@@ -22,6 +22,10 @@
 
 namespace ann {
 
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+hipError_t species_perm(ann_ctx* ctx);
+bool mfma_net_supported(const DevModel& M);
+
 hipError_t exclusive_scan(const int* in, int* out, int n, hipStream_t st);
 hipError_t ensure_work_buffers(ann_ctx* ctx);
 
@@ -191,6 +195,187 @@ __global__ __launch_bounds__(64) void k_net_bwd2(Bwd2Args A) {
         }
 #pragma unroll
         for (int j = 0; j < kBOut; ++j) { zb[j] = nb[j]; zdb[j] = ndb[j]; }
+    }
+}
+
+// The same second backward on the matrix cores: one wavefront per 16-atom
+// single-species tile (species-sorted permutation), every layer product as
+// v_mfma_f32_16x16x4_f32 (FP32 operands and accumulation, so no operand
+// splitting is needed):
+//   forward:   Z = X W^T (+ b), Zd = Xd W^T, a = sigma(Z), ad = sigma'(Z) Zd;
+//   backward:  zbar_{l-1} = (zbar_l W_l) sigma' + (zdbar_l W_l) sigma'' zd,
+//              zdbar_{l-1} = (zdbar_l W_l) sigma'.
+// Lane layout of 16x16x4: lane l supplies A[l % 16][l / 16] and B[l / 16][l % 16]
+// and receives D rows 4 (l / 16) .. +3 of column l % 16.
+constexpr int kB2Tile = 16;
+constexpr int kB2K = 128;      // widest input supported (same bound as the forward MFMA network)
+
+__global__ __launch_bounds__(64) void k_bwd2_mfma(Bwd2Args A, const int* __restrict__ perm,
+                                                  const int2* __restrict__ tiles, const int* __restrict__ spend) {
+    __shared__ float sx[kB2Tile][kB2K + 1], sxd[kB2Tile][kB2K + 1];
+    __shared__ float sdz[kMaxLayers - 1][kB2Tile][kMaxWidth + 1], sqz[kMaxLayers - 1][kB2Tile][kMaxWidth + 1];
+    __shared__ int64_t srow[kB2Tile];
+    __shared__ float sgE[kB2Tile];
+    const int lane = threadIdx.x;
+    const int2 tl = tiles[blockIdx.x];
+    const int a = tl.x;
+    const int nrow = min(kB2Tile, spend[a] - tl.y);
+    const NetDesc& D = A.dm->net[a];
+    const int L = D.nlayer;
+    const int stride = bwd2_stride(D);
+    const float* iw = A.inpw + D.inoff;
+    const float* ib = A.inpb + D.inoff;
+    if (lane < kB2Tile) {
+        const int ai = lane < nrow ? perm[tl.y + lane] : -1;
+        srow[lane] = ai >= 0 ? A.base[a] + (int64_t)A.rank[ai] * stride : -1;
+        sgE[lane] = ai >= 0 ? (float)A.gE[A.sof[ai]] : 0.0f;
+    }
+    int loff[kMaxLayers];
+    {
+        int s0 = 0, prev = D.nin;
+        for (int l = 0; l < L; ++l) { loff[l] = s0; s0 += 2 * (prev + D.nodes[l]); prev = D.nodes[l]; }
+    }
+    // layer-0 input x = inpw (G + inpb), xd = inpw v
+    for (int q = lane; q < kB2Tile * kB2K; q += kWave) {
+        const int r = q / kB2K, k = q % kB2K;
+        const int ai = r < nrow ? perm[tl.y + r] : -1;
+        float x = 0.0f, xd = 0.0f;
+        if (ai >= 0 && k < D.nin) {
+            x = iw[k] * (A.G[(size_t)ai * A.ldg + k] + ib[k]);
+            xd = A.v ? iw[k] * A.v[(size_t)ai * A.ldg + k] : 0.0f;
+        }
+        sx[r][k] = x;
+        sxd[r][k] = xd;
+    }
+    __syncthreads();
+    for (int q = lane; q < kB2Tile * D.nin; q += kWave) {
+        const int r = q / D.nin, k = q % D.nin;
+        if (srow[r] >= 0) {
+            float* row = A.scr + srow[r];
+            row[k] = sx[r][k];
+            row[D.nin + k] = sxd[r][k];
+        }
+    }
+    const int c16 = lane % 16, k4 = lane / 16, rb = 4 * k4;
+    constexpr int NT = kMaxWidth / 16;
+    int ni = D.nin;
+    for (int l = 0; l < L - 1; ++l) {
+        const int no = D.nodes[l];
+        const float* Wl = A.W + D.woff[l];
+        const float* bl = A.W + D.boff[l];
+        float oa[NT][4], od[NT][4];
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct) {
+            if (ct * 16 >= no) break;
+            f32x4 az = {0.f, 0.f, 0.f, 0.f}, ad = {0.f, 0.f, 0.f, 0.f};
+            const int n = ct * 16 + c16;
+            for (int k0 = 0; k0 < ni; k0 += 4) {
+                const int k = k0 + k4;
+                const bool ok = k < ni;
+                const float w = (ok && n < no) ? Wl[k * no + n] : 0.0f;
+                az = __builtin_amdgcn_mfma_f32_16x16x4f32(ok ? sx[c16][k] : 0.0f, w, az, 0, 0, 0);
+                ad = __builtin_amdgcn_mfma_f32_16x16x4f32(ok ? sxd[c16][k] : 0.0f, w, ad, 0, 0, 0);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float av = 0.f, dv = 0.f, d2 = 0.f;
+                if (n < no) act_f<true>(D.neuron, D.c, az[i] + bl[n], av, dv, d2);
+                oa[ct][i] = av;
+                od[ct][i] = dv * ad[i];
+                if (n < no) {
+                    sdz[l][rb + i][n] = dv;
+                    sqz[l][rb + i][n] = d2 * ad[i];
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct) {
+            if (ct * 16 >= no) break;
+            const int n = ct * 16 + c16;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int r = rb + i;
+                if (n < no) {
+                    sx[r][n] = oa[ct][i];
+                    sxd[r][n] = od[ct][i];
+                    if (srow[r] >= 0) {
+                        float* row = A.scr + srow[r] + loff[l + 1];
+                        row[n] = oa[ct][i];
+                        row[no + n] = od[ct][i];
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        ni = no;
+    }
+    // output layer (linear, one neuron): zbar = gE outw, zdbar = outw
+    const float zdbL = A.v ? D.outw : 0.0f;
+    const int nlast = L > 1 ? D.nodes[L - 2] : D.nin;
+    if (lane < kB2Tile && srow[lane] >= 0) {
+        float* row = A.scr + srow[lane] + loff[L - 1] + 2 * nlast;
+        row[0] = sgE[lane] * D.outw;
+        row[1] = zdbL;
+    }
+    if (L == 1) return;
+    // last hidden layer: abar = W_L^T zbar_L, then zbar / zdbar through sigma', sigma''
+    {
+        const float* WL = A.W + D.woff[L - 1];
+        __syncthreads();
+        for (int q = lane; q < kB2Tile * nlast; q += kWave) {
+            const int r = q / nlast, m = q % nlast;
+            const float d = sdz[L - 2][r][m], qq = sqz[L - 2][r][m];
+            const float ab = WL[m] * sgE[r] * D.outw, adb = WL[m] * zdbL;
+            sx[r][m] = ab * d + adb * qq;
+            sxd[r][m] = adb * d;
+        }
+        __syncthreads();
+    }
+    for (int l = L - 2; l >= 0; --l) {
+        const int no = D.nodes[l];
+        const int nprev = l == 0 ? D.nin : D.nodes[l - 1];
+        for (int q = lane; q < kB2Tile * no; q += kWave) {
+            const int r = q / no, n = q % no;
+            if (srow[r] >= 0) {
+                float* row = A.scr + srow[r] + loff[l] + 2 * nprev;
+                row[n] = sx[r][n];
+                row[no + n] = sxd[r][n];
+            }
+        }
+        if (l == 0) break;
+        const float* Wl = A.W + D.woff[l];
+        float nb[NT][4], ndb[NT][4];
+#pragma unroll
+        for (int mt = 0; mt < NT; ++mt) {
+            if (mt * 16 >= nprev) break;
+            f32x4 a1 = {0.f, 0.f, 0.f, 0.f}, a2 = {0.f, 0.f, 0.f, 0.f};
+            const int m = mt * 16 + c16;
+            for (int k0 = 0; k0 < no; k0 += 4) {
+                const int k = k0 + k4;
+                const bool ok = k < no;
+                const float w = (ok && m < nprev) ? Wl[m * no + k] : 0.0f;
+                a1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ok ? sx[c16][k] : 0.0f, w, a1, 0, 0, 0);
+                a2 = __builtin_amdgcn_mfma_f32_16x16x4f32(ok ? sxd[c16][k] : 0.0f, w, a2, 0, 0, 0);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int r = rb + i;
+                const float d = m < nprev ? sdz[l - 1][r][m] : 0.0f, qq = m < nprev ? sqz[l - 1][r][m] : 0.0f;
+                nb[mt][i] = a1[i] * d + a2[i] * qq;
+                ndb[mt][i] = a2[i] * d;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int mt = 0; mt < NT; ++mt) {
+            if (mt * 16 >= nprev) break;
+            const int m = mt * 16 + c16;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (m < nprev) { sx[rb + i][m] = nb[mt][i]; sxd[rb + i][m] = ndb[mt][i]; }
+        }
+        __syncthreads();
     }
 }
 
@@ -410,7 +595,13 @@ hipError_t net_bwd2(ann_ctx* ctx, const float* G, const float* v, const double* 
         A.base[s] = b;
         b += (int64_t)ctx->hm_counts[s] * bwd2_stride(ctx->hm.net[s]);
     }
-    {
+    if (mfma_net_supported(ctx->hm)) {
+        if ((e = species_perm(ctx)) != hipSuccess) return e;
+        Launch L(ctx, "net_bwd2");
+        if (ctx->ntiles > 0)
+            k_bwd2_mfma<<<ctx->ntiles, 64, 0, ctx->stream>>>(A, ctx->d_perm, reinterpret_cast<const int2*>(ctx->d_tiles),
+                                                            ctx->d_spend_end);
+    } else {
         Launch L(ctx, "net_bwd2");
         k_net_bwd2<<<(A.N + 3) / 4, 64, 0, ctx->stream>>>(A);
     }
