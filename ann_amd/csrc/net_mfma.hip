@@ -3,21 +3,14 @@
 // Same mathematics as ANN::fpbp + DODZ::grad (src/ml/nn.cpp:1367-1389,
 // 1801-1830).  Atoms are grouped by species (a species-sorted permutation of
 // the batch) into tiles of 16 rows; one wavefront owns a tile and runs every
-// layer as 16 x N x K products with v_mfma_f32_16x16x32_bf16 (BF16 operands,
-// FP32 accumulation).  BF16 alone carries 8 mantissa bits, so every operand
-// is split into three BF16 terms x = x0 + x1 + x2 (x0 = bf16(x),
-// x1 = bf16(x - x0), x2 = bf16(x - x0 - x1)) and the six products with
-// i + j <= 2 are accumulated: the truncation error is ~2^-24 relative, the
-// FP32 level that the 1e-5 force bar needs.
+// layer as 16 x N x K products on the matrix cores (BF16 operands split into
+// three terms, FP32 accumulation; mm_bf16x3 below):
 //   forward:  Z_l = A_{l-1} W_l^T (+ b_l), activation on the accumulator;
 //   backward: g_{L-1} = outw W_L^T . sigma'_{L-1}, g_{l-1} = (g_l W_l) . sigma'_{l-1},
 //             dE/dG = (g_0 W_0) . inpw.
 // Activations move between the accumulator layout and the operand layout
-// through LDS.  Weights are read as FP32 and split on the fly.
-//
-// MFMA 16x16x32 lane layout used here: lane l supplies row (l % 16) of A and
-// column (l % 16) of B for the k-block 8*(l / 16) .. +7; it receives D rows
-// 4*(l / 16) .. +3 of column (l % 16).
+// through LDS; weights are read as FP32 from the working copy and split on
+// the fly.
 #include "net_dev.h"
 
 namespace ann {
@@ -45,6 +38,16 @@ struct MfmaArgs {
     float* dEdG;
 };
 
+// acc[ct] += X (16 x K, rows = atoms, from LDS) * B (K x 16 columns of tile ct)
+// for every column tile at once on v_mfma_f32_16x16x32_bf16.  BF16 carries 8
+// mantissa bits, so both operands are split into three BF16 terms
+// x = x0 + x1 + x2 (x0 = bf16(x), x1 = bf16(x - x0), x2 = bf16(x - x0 - x1)) and
+// the six products with i + j <= 2 are accumulated in FP32: truncation
+// ~2^-24 relative, the FP32 level the 1e-5 force bar needs.  The activation
+// operand is split once per k block and reused by every column tile; the
+// tiles advance together so one block's weight loads are in flight together.
+// Lane l supplies row l % 16 of X and column l % 16 of B for k block
+// 8 (l / 16) .. +7 and receives D rows 4 (l / 16) .. +3 of column l % 16.
 __device__ __forceinline__ void split3(float x, __bf16& a, __bf16& b, __bf16& c) {
     a = (__bf16)x;
     const float r1 = x - (float)a;
@@ -52,29 +55,40 @@ __device__ __forceinline__ void split3(float x, __bf16& a, __bf16& b, __bf16& c)
     c = (__bf16)(r1 - (float)b);
 }
 
-// acc += A (16 x K, operand layout from fa(row, k)) * B (K x 16 col tile from fb(k, col))
-template <typename FA, typename FB>
-__device__ __forceinline__ f32x4 mm_tile(int K, FA fa, FB fb, f32x4 acc) {
+template <int NT, typename FB>
+__device__ __forceinline__ void mm_bf16x3(int K, int ncol, const float (*X)[kMaxK + 1], FB fb, f32x4 (&acc)[NT]) {
     const int lane = threadIdx.x % kWave;
     const int row = lane % 16, kb = 8 * (lane / 16);
     for (int k0 = 0; k0 < K; k0 += 32) {
-        bf16x8 a0, a1, a2, b0, b1, b2;
+        bf16x8 a0, a1, a2;
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-            __bf16 x0, x1, x2, y0, y1, y2;
-            split3(fa(row, k0 + kb + j), x0, x1, x2);
-            split3(fb(k0 + kb + j, row), y0, y1, y2);
+            const int k = k0 + kb + j;
+            __bf16 x0, x1, x2;
+            split3(k < K ? X[row][k] : 0.0f, x0, x1, x2);
             a0[j] = x0; a1[j] = x1; a2[j] = x2;
-            b0[j] = y0; b1[j] = y1; b2[j] = y2;
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, b0, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b2, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1, acc, 0, 0, 0);
-        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc, 0, 0, 0);
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct) {
+            if (ct * 16 >= ncol) break;
+            bf16x8 b0, b1, b2;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) {
+                const int k = k0 + kb + j;
+                __bf16 y0, y1, y2;
+                split3(k < K ? fb(k, ct * 16 + row) : 0.0f, y0, y1, y2);
+                b0[j] = y0; b1[j] = y1; b2[j] = y2;
+            }
+            f32x4 c = acc[ct];
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, b0, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b2, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1, c, 0, 0, 0);
+            c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, c, 0, 0, 0);
+            acc[ct] = c;
+        }
     }
-    return acc;
 }
 
 // one wavefront per 16-atom single-species tile
@@ -105,17 +119,15 @@ __global__ __launch_bounds__(64) void k_net_mfma(MfmaArgs A) {
         const int no = D.nodes[l];
         const float* Wl = A.W + D.woff[l];
         const float* bl = A.W + D.boff[l];
-        const int Kp = (ni + 31) / 32 * 32;
         float out[kMaxWidth / 16][4];
+        f32x4 accs[kMaxWidth / 16];
+#pragma unroll
+        for (int ct = 0; ct < kMaxWidth / 16; ++ct) accs[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+        mm_bf16x3<kMaxWidth / 16>(ni, no, act, [&](int k, int n) { return n < no ? Wl[k * no + n] : 0.0f; }, accs);
 #pragma unroll
         for (int ct = 0; ct < kMaxWidth / 16; ++ct) {
             if (ct * 16 >= no) break;
-            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-            acc = mm_tile(Kp, [&](int r, int k) { return act[r][k]; },
-                          [&](int k, int c) {
-                              const int n = ct * 16 + c;
-                              return (k < ni && n < no) ? Wl[k * no + n] : 0.0f;
-                          }, acc);
+            const f32x4 acc = accs[ct];
             const int n = ct * 16 + col;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
@@ -168,17 +180,15 @@ __global__ __launch_bounds__(64) void k_net_mfma(MfmaArgs A) {
         const int no = D.nodes[l];
         const int nprev = l == 0 ? D.nin : D.nodes[l - 1];
         const float* Wl = A.W + D.woff[l];
-        const int Kp = (no + 31) / 32 * 32;
         float out[kMaxK / 16][4];
+        f32x4 accs[kMaxK / 16];
+#pragma unroll
+        for (int ct = 0; ct < kMaxK / 16; ++ct) accs[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+        mm_bf16x3<kMaxK / 16>(no, nprev, act, [&](int k, int m) { return m < nprev ? Wl[m * no + k] : 0.0f; }, accs);
 #pragma unroll
         for (int ct = 0; ct < kMaxK / 16; ++ct) {
             if (ct * 16 >= nprev) break;
-            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-            acc = mm_tile(Kp, [&](int r, int k) { return act[r][k]; },
-                          [&](int k, int c) {
-                              const int m = ct * 16 + c;
-                              return (k < no && m < nprev) ? Wl[m * no + k] : 0.0f;
-                          }, acc);
+            const f32x4 acc = accs[ct];
             const int m = ct * 16 + col;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {

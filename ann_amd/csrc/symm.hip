@@ -66,10 +66,16 @@ __device__ __forceinline__ float wave_sum_multi(float (&v)[N], int& idx) {
 #pragma unroll
     for (int h = N / 2, bit = 32; h >= 1; h >>= 1, bit >>= 1) {
         const bool upper = (lane & bit) != 0;
+        // bitwise blend of two statically indexed registers: a conditional
+        // choice between v[q] and v[q + h] is otherwise rewritten by the
+        // compiler into a runtime-indexed register access (a compare/select
+        // chain over all N registers)
+        const unsigned m = upper ? 0xffffffffu : 0u;
 #pragma unroll
         for (int q = 0; q < h; ++q) {
-            const float send = upper ? v[q] : v[q + h];
-            const float keep = upper ? v[q + h] : v[q];
+            const unsigned lo = __float_as_uint(v[q]), hi = __float_as_uint(v[q + h]);
+            const float send = __uint_as_float((lo & m) | (hi & ~m));
+            const float keep = __uint_as_float((hi & m) | (lo & ~m));
             v[q] = keep + __shfl_xor(send, bit);
         }
         if (upper) base += h;

@@ -264,18 +264,30 @@ __global__ __launch_bounds__(64) void k_bwd2_mfma(Bwd2Args A, const int* __restr
         const float* Wl = A.W + D.woff[l];
         const float* bl = A.W + D.boff[l];
         float oa[NT][4], od[NT][4];
+        // all column tiles advance together over k, so the weight loads of one
+        // k step (independent across tiles and unrolled steps) are in flight at once
+        f32x4 AZ[NT], AD[NT];
+#pragma unroll
+        for (int ct = 0; ct < NT; ++ct) { AZ[ct] = f32x4{0.f, 0.f, 0.f, 0.f}; AD[ct] = f32x4{0.f, 0.f, 0.f, 0.f}; }
+#pragma unroll 4
+        for (int k0 = 0; k0 < ni; k0 += 4) {
+            const int k = k0 + k4;
+            const bool ok = k < ni;
+            const float xa = ok ? sx[c16][k] : 0.0f, xda = ok ? sxd[c16][k] : 0.0f;
+#pragma unroll
+            for (int ct = 0; ct < NT; ++ct) {
+                if (ct * 16 >= no) break;
+                const int n = ct * 16 + c16;
+                const float w = (ok && n < no) ? Wl[k * no + n] : 0.0f;
+                AZ[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa, w, AZ[ct], 0, 0, 0);
+                AD[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(xda, w, AD[ct], 0, 0, 0);
+            }
+        }
 #pragma unroll
         for (int ct = 0; ct < NT; ++ct) {
             if (ct * 16 >= no) break;
-            f32x4 az = {0.f, 0.f, 0.f, 0.f}, ad = {0.f, 0.f, 0.f, 0.f};
             const int n = ct * 16 + c16;
-            for (int k0 = 0; k0 < ni; k0 += 4) {
-                const int k = k0 + k4;
-                const bool ok = k < ni;
-                const float w = (ok && n < no) ? Wl[k * no + n] : 0.0f;
-                az = __builtin_amdgcn_mfma_f32_16x16x4f32(ok ? sx[c16][k] : 0.0f, w, az, 0, 0, 0);
-                ad = __builtin_amdgcn_mfma_f32_16x16x4f32(ok ? sxd[c16][k] : 0.0f, w, ad, 0, 0, 0);
-            }
+            const f32x4 az = AZ[ct], ad = AD[ct];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 float av = 0.f, dv = 0.f, d2 = 0.f;
@@ -346,18 +358,28 @@ __global__ __launch_bounds__(64) void k_bwd2_mfma(Bwd2Args A, const int* __restr
         if (l == 0) break;
         const float* Wl = A.W + D.woff[l];
         float nb[NT][4], ndb[NT][4];
+        f32x4 A1[NT], A2[NT];
+#pragma unroll
+        for (int mt = 0; mt < NT; ++mt) { A1[mt] = f32x4{0.f, 0.f, 0.f, 0.f}; A2[mt] = f32x4{0.f, 0.f, 0.f, 0.f}; }
+#pragma unroll 4
+        for (int k0 = 0; k0 < no; k0 += 4) {
+            const int k = k0 + k4;
+            const bool ok = k < no;
+            const float za = ok ? sx[c16][k] : 0.0f, zda = ok ? sxd[c16][k] : 0.0f;
+#pragma unroll
+            for (int mt = 0; mt < NT; ++mt) {
+                if (mt * 16 >= nprev) break;
+                const int m = mt * 16 + c16;
+                const float w = (ok && m < nprev) ? Wl[m * no + k] : 0.0f;
+                A1[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(za, w, A1[mt], 0, 0, 0);
+                A2[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(zda, w, A2[mt], 0, 0, 0);
+            }
+        }
 #pragma unroll
         for (int mt = 0; mt < NT; ++mt) {
             if (mt * 16 >= nprev) break;
-            f32x4 a1 = {0.f, 0.f, 0.f, 0.f}, a2 = {0.f, 0.f, 0.f, 0.f};
             const int m = mt * 16 + c16;
-            for (int k0 = 0; k0 < no; k0 += 4) {
-                const int k = k0 + k4;
-                const bool ok = k < no;
-                const float w = (ok && m < nprev) ? Wl[m * no + k] : 0.0f;
-                a1 = __builtin_amdgcn_mfma_f32_16x16x4f32(ok ? sx[c16][k] : 0.0f, w, a1, 0, 0, 0);
-                a2 = __builtin_amdgcn_mfma_f32_16x16x4f32(ok ? sxd[c16][k] : 0.0f, w, a2, 0, 0, 0);
-            }
+            const f32x4 a1 = A1[mt], a2 = A2[mt];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int r = rb + i;
