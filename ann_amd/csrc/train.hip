@@ -236,6 +236,7 @@ __global__ __launch_bounds__(64) void k_bwd2_mfma(Bwd2Args A, const int* __restr
         for (int l = 0; l < L; ++l) { loff[l] = s0; s0 += 2 * (prev + D.nodes[l]); prev = D.nodes[l]; }
     }
     // layer-0 input x = inpw (G + inpb), xd = inpw v
+#pragma unroll 8
     for (int q = lane; q < kB2Tile * kB2K; q += kWave) {
         const int r = q / kB2K, k = q % kB2K;
         const int ai = r < nrow ? perm[tl.y + r] : -1;
@@ -269,18 +270,30 @@ __global__ __launch_bounds__(64) void k_bwd2_mfma(Bwd2Args A, const int* __restr
         f32x4 AZ[NT], AD[NT];
 #pragma unroll
         for (int ct = 0; ct < NT; ++ct) { AZ[ct] = f32x4{0.f, 0.f, 0.f, 0.f}; AD[ct] = f32x4{0.f, 0.f, 0.f, 0.f}; }
-#pragma unroll 4
-        for (int k0 = 0; k0 < ni; k0 += 4) {
-            const int k = k0 + k4;
+        // this lane's weight operands of the whole layer, loaded up front so that
+        // one memory latency covers the layer (a wavefront per tile leaves too
+        // few wavefronts per SIMD to hide a latency per k step)
+        float wr[kB2K / 4][NT];
+#pragma unroll
+        for (int st = 0; st < kB2K / 4; ++st) {
+            const int k = 4 * st + k4;
+#pragma unroll
+            for (int ct = 0; ct < NT; ++ct) {
+                const int n = ct * 16 + c16;
+                wr[st][ct] = (4 * st < ni && k < ni && n < no) ? Wl[k * no + n] : 0.0f;
+            }
+        }
+#pragma unroll
+        for (int st = 0; st < kB2K / 4; ++st) {
+            if (4 * st >= ni) break;
+            const int k = 4 * st + k4;
             const bool ok = k < ni;
             const float xa = ok ? sx[c16][k] : 0.0f, xda = ok ? sxd[c16][k] : 0.0f;
 #pragma unroll
             for (int ct = 0; ct < NT; ++ct) {
                 if (ct * 16 >= no) break;
-                const int n = ct * 16 + c16;
-                const float w = (ok && n < no) ? Wl[k * no + n] : 0.0f;
-                AZ[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa, w, AZ[ct], 0, 0, 0);
-                AD[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(xda, w, AD[ct], 0, 0, 0);
+                AZ[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa, wr[st][ct], AZ[ct], 0, 0, 0);
+                AD[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(xda, wr[st][ct], AD[ct], 0, 0, 0);
             }
         }
 #pragma unroll
@@ -361,18 +374,27 @@ __global__ __launch_bounds__(64) void k_bwd2_mfma(Bwd2Args A, const int* __restr
         f32x4 A1[NT], A2[NT];
 #pragma unroll
         for (int mt = 0; mt < NT; ++mt) { A1[mt] = f32x4{0.f, 0.f, 0.f, 0.f}; A2[mt] = f32x4{0.f, 0.f, 0.f, 0.f}; }
-#pragma unroll 4
-        for (int k0 = 0; k0 < no; k0 += 4) {
-            const int k = k0 + k4;
+        float wr[kMaxWidth / 4][NT];       // no <= kMaxWidth: weights up front, as in the forward
+#pragma unroll
+        for (int st = 0; st < kMaxWidth / 4; ++st) {
+            const int k = 4 * st + k4;
+#pragma unroll
+            for (int mt = 0; mt < NT; ++mt) {
+                const int m = mt * 16 + c16;
+                wr[st][mt] = (4 * st < no && k < no && m < nprev) ? Wl[m * no + k] : 0.0f;
+            }
+        }
+#pragma unroll
+        for (int st = 0; st < kMaxWidth / 4; ++st) {
+            if (4 * st >= no) break;
+            const int k = 4 * st + k4;
             const bool ok = k < no;
             const float za = ok ? sx[c16][k] : 0.0f, zda = ok ? sxd[c16][k] : 0.0f;
 #pragma unroll
             for (int mt = 0; mt < NT; ++mt) {
                 if (mt * 16 >= nprev) break;
-                const int m = mt * 16 + c16;
-                const float w = (ok && m < nprev) ? Wl[m * no + k] : 0.0f;
-                A1[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(za, w, A1[mt], 0, 0, 0);
-                A2[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(zda, w, A2[mt], 0, 0, 0);
+                A1[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(za, wr[st][mt], A1[mt], 0, 0, 0);
+                A2[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(zda, wr[st][mt], A2[mt], 0, 0, 0);
             }
         }
 #pragma unroll
