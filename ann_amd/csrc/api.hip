@@ -746,6 +746,101 @@ extern "C" int ann_train_graph(ann_ctx* ctx, int enable) {
     return ANN_OK;
 }
 
+// ---- autotuning ----
+static double time_ms(ann_ctx* ctx, hipError_t (*f)(ann_ctx*, int), int arg, int reps, hipError_t& err) {
+    hipEvent_t a = nullptr, b = nullptr;
+    float ms = 0.0f;
+    err = hipEventCreate(&a);
+    if (err == hipSuccess) err = hipEventCreate(&b);
+    if (err == hipSuccess) err = f(ctx, arg);                       // warm-up
+    if (err == hipSuccess) err = hipEventRecord(a, ctx->stream);
+    for (int r = 0; r < reps && err == hipSuccess; ++r) err = f(ctx, arg);
+    if (err == hipSuccess) err = hipEventRecord(b, ctx->stream);
+    if (err == hipSuccess) err = hipEventSynchronize(b);
+    if (err == hipSuccess) err = hipEventElapsedTime(&ms, a, b);
+    if (a) hipEventDestroy(a);
+    if (b) hipEventDestroy(b);
+    return (double)ms / reps;
+}
+static hipError_t run_fwd(ann_ctx* c, int form) { return symm_fwd(c, form, c->d_G); }
+static hipError_t run_bwd(ann_ctx* c, int form) { return symm_bwd(c, form, c->d_dEdG, c->d_P); }
+static hipError_t run_tan(ann_ctx* c, int form) { return symm_tangent(c, form, c->d_u, c->d_v); }
+static hipError_t run_nb(ann_ctx* c, int) {
+    c->grid_valid = false;      // the blocking policy is part of the cached grid
+    return neighbors_build(c, c->nl_rc);
+}
+
+extern "C" int ann_autotune(ann_ctx* ctx, int form) {
+    int rc = need_nlist(ctx); if (rc) return rc;
+    if ((rc = check_form(form))) return rc;
+    RET_HIP(ensure_work(ctx), "allocation");
+    hipError_t e;
+    RET_HIP(symm_fwd(ctx, form, ctx->d_G), "symm_fwd");
+    RET_HIP(net_fwd(ctx, form, ctx->d_G, ctx->d_Eatom, ctx->d_dEdG), "net_fwd");
+    RET_HIP(hipMemsetAsync(ctx->d_u, 0, sizeof(float) * 3 * (size_t)ctx->natoms, ctx->stream), "memset");
+    const DevModel& M = ctx->hm;
+    if (form == ANN_FORM_FAST && M.all_grid && M.moment_z > 0) {
+        hipError_t (*runs[3])(ann_ctx*, int) = {run_fwd, run_bwd, run_tan};
+        double best_total = 0.0;
+        ctx->tune_angular = 0;
+        for (int mode = 0; mode < 3; ++mode) {
+            double best = 1e30;
+            int pick = 1;
+            for (int apb : {1, 2, 4}) {
+                ctx->tune_apb[mode] = apb;
+                const double t = time_ms(ctx, runs[mode], form, 3, e);
+                if (e != hipSuccess) return fail(ANN_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
+                if (t < best) { best = t; pick = apb; }
+            }
+            ctx->tune_apb[mode] = pick;
+            if (mode < 2) best_total += best;
+        }
+        // moment form against the triplet tiles (G + adjoint)
+        ctx->tune_angular = 1;
+        const double tt = time_ms(ctx, run_fwd, form, 3, e) + time_ms(ctx, run_bwd, form, 3, e);
+        if (e != hipSuccess) return fail(ANN_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
+        ctx->tune_angular = tt < best_total ? 1 : 0;
+    }
+    // neighbour search blocking
+    {
+        double t[2];
+        for (int sb = 0; sb < 2; ++sb) {
+            ctx->tune_sb = sb;
+            t[sb] = time_ms(ctx, run_nb, 0, 2, e);
+            if (e != hipSuccess) return fail(ANN_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
+        }
+        ctx->tune_sb = t[1] < t[0] ? 1 : 0;
+        ctx->grid_valid = false;
+        RET_HIP(neighbors_build(ctx, ctx->nl_rc), "neighbour build");
+    }
+    ctx->G_valid = false;
+    ctx->pos_ref_valid = false;
+    ctx->gen++;
+    return ANN_OK;
+}
+
+extern "C" int ann_get_tuning(ann_ctx* ctx, int* t, int n) {
+    CHECK_CTX(ctx);
+    if (!t || n < 5) return fail(ANN_ERR_ARG, "tuning needs 5 entries");
+    t[0] = ctx->tune_apb[0]; t[1] = ctx->tune_apb[1]; t[2] = ctx->tune_apb[2];
+    t[3] = ctx->tune_angular; t[4] = ctx->tune_sb;
+    return ANN_OK;
+}
+
+extern "C" int ann_set_tuning(ann_ctx* ctx, const int* t, int n) {
+    CHECK_CTX(ctx);
+    if (!t || n < 5) return fail(ANN_ERR_ARG, "tuning needs 5 entries");
+    for (int q = 0; q < 3; ++q)
+        if (t[q] != 1 && t[q] != 2 && t[q] != 4) return fail(ANN_ERR_ARG, "atoms per workgroup must be 1, 2 or 4");
+    if ((t[3] != 0 && t[3] != 1) || (t[4] != 0 && t[4] != 1)) return fail(ANN_ERR_ARG, "invalid tuning switch");
+    for (int q = 0; q < 3; ++q) ctx->tune_apb[q] = t[q];
+    ctx->tune_angular = t[3];
+    if (ctx->tune_sb != t[4]) ctx->grid_valid = false;
+    ctx->tune_sb = t[4];
+    ctx->gen++;
+    return ANN_OK;
+}
+
 extern "C" int ann_profile(ann_ctx* ctx, int enable) {
     CHECK_CTX(ctx);
     int rc = drain_timers(ctx);

@@ -188,6 +188,10 @@ struct ann_ctx {
     int64_t gpart_cap = 0;
     int64_t work_atoms = 0, work_pairs = 0;
     int* d_adam_t = nullptr;          // Adam step counter (device, so a captured step can advance it)
+    // tuning (set by ann_autotune or ann_set_tuning)
+    int tune_apb[3] = {1, 1, 1};      // atoms (wavefronts) per workgroup of the moment kernels, per mode
+    int tune_angular = 0;             // 0: moment form where it applies, 1: triplet tiles
+    int tune_sb = 0;                  // neighbour search blocks: 0 heuristic (2x2x2 bins where they fit), 1 single bins
     // training step captured as a HIP graph and replayed
     bool graph_on = false, capturing = false, train_warm = false;
     hipStream_t gstream = nullptr;    // private non-blocking stream the graph is captured on and replayed

@@ -544,7 +544,7 @@ static hipError_t grid_setup(ann_ctx* ctx, double rc) {
         const double per_bin = (double)(ctx->atom_off[s + 1] - ctx->atom_off[s]) / c.nbins;
         const double box2 = (double)(std::min(2, c.nb[0]) + 2 * c.k[0]) * (std::min(2, c.nb[1]) + 2 * c.k[1]) *
                             (std::min(2, c.nb[2]) + 2 * c.k[2]);
-        c.sb = (box2 * per_bin <= 0.7 * kStageCap && box2 <= kStageImages) ? 2 : 1;
+        c.sb = (ctx->tune_sb != 1 && box2 * per_bin <= 0.7 * kStageCap && box2 <= kStageImages) ? 2 : 1;
         for (int a = 0; a < 3; ++a) c.nsb[a] = (c.nb[a] + c.sb - 1) / c.sb;
         const int nsbins = c.nsb[0] * c.nsb[1] * c.nsb[2];
         c.sboff = (int)bin_struc.size();

@@ -42,6 +42,7 @@ struct SymmArgs {
     const float* ang_zeta;
     const float* ang_lam;
     int N, ldg, maxn;
+    int smem_wave;            // bytes of dynamic LDS per wavefront (moment kernels: one atom per wavefront)
     const int* species;
     const int* off;
     const float4* geo;
@@ -87,6 +88,14 @@ __device__ __forceinline__ float wave_sum_multi(float (&v)[N], int& idx) {
     for (int q = 0; q < steps; ++q, bit <<= 1) s += __shfl_xor(s, bit);
     idx = base;
     return s;
+}
+
+// LDS ordering between the lanes of one wavefront (each wavefront of a
+// moment-kernel block owns its own LDS slice, so no block barrier is needed)
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 __device__ __forceinline__ float wave_sum(float v) {
@@ -275,7 +284,7 @@ __device__ __forceinline__ float pow_m1(float w, int zi, float zeta) {
 // are written as zero when zero_P is set.  Returns the number kept.
 __device__ __forceinline__ int fast_load(const SymmArgs& A, const DevModel& M, int i, int a, int o, int n, FastSmem& sm,
                          bool need_deriv, bool zero_P) {
-    const int lane = threadIdx.x, S = M.S, U = M.nunion[a];
+    const int lane = threadIdx.x % kWave, S = M.S, U = M.nunion[a];
     const float rc = M.rc;
     int cnt[kMaxSpecies];
 #pragma unroll
@@ -346,7 +355,7 @@ __device__ __forceinline__ int fast_load(const SymmArgs& A, const DevModel& M, i
 template <int NG, int NK, int MODE>
 __device__ __forceinline__ void fast_block_flat(const AngBlock& B, const FastSmem& sm, int sb, int nb, int sc, int nc, bool tri,
                                 const float4* tv, float* out) {
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x % kWave;
     float lamh[NK], zeta[NK];
     int zi[NK], gu[NG];
 #pragma unroll
@@ -442,7 +451,7 @@ __device__ __forceinline__ void fast_block_flat(const AngBlock& B, const FastSme
 template <int NG, int NK>
 __device__ __forceinline__ void fast_block_adj(const AngBlock& B, const FastSmem& sm, int sb, int nb, int sc, int nc, bool tri,
                                const float* sd, float* sP) {
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x % kWave;
     float lamh[NK], zeta[NK], dl[NG][NK];
     int zi[NK], gu[NG];
 #pragma unroll
@@ -542,7 +551,7 @@ constexpr int kRadChunk = 16;
 template <int MODE>
 __device__ __forceinline__ void fast_radial(const SymmArgs& A, const DevModel& M, int a, const FastSmem& sm, const float4* tv,
                             const float* sd, float* sP, float* out) {
-    const int lane = threadIdx.x, S = M.S;
+    const int lane = threadIdx.x % kWave, S = M.S;
     for (int b = 0; b < S; ++b) {
         const RadBlock rb = M.rad[a * S + b];
         if (rb.n == 0) continue;
@@ -721,16 +730,20 @@ __constant__ MonoTable c_mono = MonoTable();
 
 __device__ __forceinline__ int pair_of(int b, int c) { return b <= c ? c * (c + 1) / 2 + b : b * (b + 1) / 2 + c; }
 
+// One wavefront per atom; a block holds blockDim.x / 64 atoms (the "atoms per
+// workgroup" tuning), each wavefront with its own LDS slice.
 template <int MODE, int Z>
-__global__ __launch_bounds__(64) void k_moment(SymmArgs A) {
+__global__ __launch_bounds__(256) void k_moment(SymmArgs A) {
     constexpr int NM = nmono(Z);
     constexpr int ZP = Z + 1;
     constexpr int PWS = (3 * ZP * (MODE == 2 ? 2 : 1)) | 1;   // powers per neighbour, odd stride
     constexpr int U4 = kMaxUnionEta;
     constexpr int NG = kWave / NM;                           // lane groups over the neighbours
-    extern __shared__ float4 smem4[];
-    const int i = blockIdx.x, lane = threadIdx.x;
+    extern __shared__ float4 smem_all[];
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    const int i = blockIdx.x * (blockDim.x / kWave) + wave;
     if (i >= A.N) return;
+    float4* smem4 = smem_all + (size_t)wave * (A.smem_wave / 16);
     const DevModel& Mo = *A.dm;
     const int a = A.species[i], o = A.off[i];
     int n = A.off[i + 1] - o;
@@ -762,7 +775,7 @@ __global__ __launch_bounds__(64) void k_moment(SymmArgs A) {
     } else {
         for (int m = lane; m < ldg; m += kWave) rowbuf[m] = 0.0f;
     }
-    __syncthreads();
+    wave_sync();
     if (MODE == 2) {
         for (int J = lane; J < n; J += kWave) {
             const int j = A.nidx[o + sm.canon[J]];
@@ -771,7 +784,7 @@ __global__ __launch_bounds__(64) void k_moment(SymmArgs A) {
             const float4 gq = sm.g[J];
             tv[J] = make_float4(tx, ty, tz, gq.x * tx + gq.y * ty + gq.z * tz);
         }
-        __syncthreads();
+        wave_sync();
     }
     fast_radial<MODE>(A, Mo, a, sm, tv, rowbuf, sP, rowbuf);
     // powers of the unit vector (and, for the tangent, their derivatives along
@@ -797,7 +810,7 @@ __global__ __launch_bounds__(64) void k_moment(SymmArgs A) {
             }
         }
     }
-    __syncthreads();
+    wave_sync();
     // moments T_b[u][alpha]: lane = (alpha, group); groups stride the neighbours
     {
         const int al = lane % NM, grp = lane / NM;
@@ -854,7 +867,7 @@ __global__ __launch_bounds__(64) void k_moment(SymmArgs A) {
                         if (MODE == 2) part[NG * NM * U4 + (grp * NM + al) * U4 + u] = dacc[u];
                     }
                 }
-                __syncthreads();
+                wave_sync();
                 for (int q = lane; q < NM * U4; q += kWave) {
                     float s = 0.0f, ds = 0.0f;
                     for (int gq = 0; gq < NG; ++gq) {
@@ -865,7 +878,7 @@ __global__ __launch_bounds__(64) void k_moment(SymmArgs A) {
                     T[(b * U4 + u) * NM + al2] = s;
                     if (MODE == 2) dT[(b * U4 + u) * NM + al2] = ds;
                 }
-                __syncthreads();
+                wave_sync();
             }
         }
         // diagonal sums D_b[u] (mode 2 also their tangent): lanes over the
@@ -895,7 +908,7 @@ __global__ __launch_bounds__(64) void k_moment(SymmArgs A) {
             }
         }
     }
-    __syncthreads();
+    wave_sync();
     // block contractions (modes 0, 2) or their adjoint seeds (mode 1)
     if (MODE == 1) {
         for (int p = 0; p < NP; ++p) {
@@ -929,7 +942,7 @@ __global__ __launch_bounds__(64) void k_moment(SymmArgs A) {
         }
         Am[it] = s;
     }
-    __syncthreads();
+    wave_sync();
     if (MODE != 1) {
         for (int p = 0; p < NP; ++p) {
             const AngBlock& B = Mo.ang[a * NP + p];
@@ -941,7 +954,7 @@ __global__ __launch_bounds__(64) void k_moment(SymmArgs A) {
                 rowbuf[B.goff + B.outm[q]] = v;
             }
         }
-        __syncthreads();
+        wave_sync();
         float* out = (MODE == 0 ? A.G : A.v) + (size_t)i * ldg;
         for (int m = lane; m < ldg; m += kWave) out[m] = rowbuf[m];
         return;
@@ -960,7 +973,7 @@ __global__ __launch_bounds__(64) void k_moment(SymmArgs A) {
         for (int m = 0; m < ZP; ++m) s += Am[(pair_of(b, b) * U4 + u) * ZP + m];
         dD[lane] = -0.5f * s;
     }
-    __syncthreads();
+    wave_sync();
     // per neighbour: Q(r^) = sum_alpha W_alpha r^^alpha and grad of sum V_alpha r^^alpha
     constexpr MonoTable mt{};
     for (int J = lane; J < n; J += kWave) {
@@ -994,7 +1007,7 @@ __global__ __launch_bounds__(64) void k_moment(SymmArgs A) {
         sP[3 * J + 1] += radial * gq.y + (g1 - gq.y * rg) * gq.w;
         sP[3 * J + 2] += radial * gq.z + (g2 - gq.z * rg) * gq.w;
     }
-    __syncthreads();
+    wave_sync();
     for (int J = lane; J < n; J += kWave) {
         const int t = sm.canon[J];
         A.P[3 * ((size_t)o + t)] = sP[3 * J];
@@ -1058,18 +1071,23 @@ static hipError_t run(ann_ctx* ctx, int form, SymmArgs A) {
     if (A.N == 0) return hipSuccess;
     const DevModel& M = ctx->hm;
     const bool fast = form != ANN_FORM_PLAIN && M.all_grid && M.fast_ng > 0;
-    const bool moment = fast && form == ANN_FORM_FAST && M.moment_z > 0;
+    const bool moment = fast && form == ANN_FORM_FAST && M.moment_z > 0 && ctx->tune_angular == 0;
     const char* names[3][3] = {{"symm_fwd_plain", "symm_bwd_plain", "symm_tan_plain"},
                                {"symm_fwd_fast", "symm_bwd_fast", "symm_tan_fast"},
                                {"symm_fwd_mom", "symm_bwd_mom", "symm_tan_mom"}};
     Launch L(ctx, names[moment ? 2 : (fast ? 1 : 0)][MODE]);
     if (moment) {
-        const size_t sm = moment_smem(A, MODE, M.moment_z, M.S, M.NP);
+        const size_t sw = (moment_smem(A, MODE, M.moment_z, M.S, M.NP) + 15) / 16 * 16;
+        int apb = ctx->tune_apb[MODE];
+        while (apb > 1 && sw * apb > 65536) apb /= 2;          // dynamic LDS of one workgroup
+        A.smem_wave = (int)sw;
+        const unsigned grid = (unsigned)((A.N + apb - 1) / apb), block = 64u * apb;
+        const size_t sm = sw * apb;
         switch (M.moment_z) {
-            case 1: k_moment<MODE, 1><<<A.N, 64, sm, ctx->stream>>>(A); break;
-            case 2: k_moment<MODE, 2><<<A.N, 64, sm, ctx->stream>>>(A); break;
-            case 3: k_moment<MODE, 3><<<A.N, 64, sm, ctx->stream>>>(A); break;
-            default: k_moment<MODE, 4><<<A.N, 64, sm, ctx->stream>>>(A); break;
+            case 1: k_moment<MODE, 1><<<grid, block, sm, ctx->stream>>>(A); break;
+            case 2: k_moment<MODE, 2><<<grid, block, sm, ctx->stream>>>(A); break;
+            case 3: k_moment<MODE, 3><<<grid, block, sm, ctx->stream>>>(A); break;
+            default: k_moment<MODE, 4><<<grid, block, sm, ctx->stream>>>(A); break;
         }
         return hipGetLastError();
     }

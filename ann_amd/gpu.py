@@ -54,7 +54,8 @@ EXPORTS = ["ann_last_error", "ann_create", "ann_destroy", "ann_set_stream", "ann
            "ann_set_scaling", "ann_set_batch", "ann_neighbors", "ann_nlist_size", "ann_nlist_copy",
            "ann_symm_fwd", "ann_net_fwd", "ann_symm_bwd", "ann_force_reduce", "ann_energy_forces",
            "ann_symm_tangent", "ann_net_bwd2", "ann_train_step", "ann_get_params", "ann_profile",
-           "ann_profile_get", "ann_refresh_geometry", "ann_wrap_positions", "ann_md_step", "ann_train_graph"]
+           "ann_profile_get", "ann_refresh_geometry", "ann_wrap_positions", "ann_md_step", "ann_train_graph",
+           "ann_autotune", "ann_get_tuning", "ann_set_tuning"]
 
 
 def lib():
@@ -246,6 +247,20 @@ class Engine:
         _check(lib().ann_md_step(self.h, form, C.c_double(dt), _ptr(mass), _ptr(vel), _ptr(F), _ptr(E), _ptr(KE),
                                  _ptr(V), C.byref(moved) if check else None))
         return bool(moved.value)
+
+    # ------------------------------------------------------------ tuning
+    def autotune(self, form: int = FAST) -> list:
+        _check(lib().ann_autotune(self.h, form))
+        return self.tuning()
+
+    def tuning(self) -> list:
+        t = (C.c_int * 5)()
+        _check(lib().ann_get_tuning(self.h, t, 5))
+        return list(t)
+
+    def set_tuning(self, t) -> None:
+        v = (C.c_int * 5)(*[int(x) for x in t])
+        _check(lib().ann_set_tuning(self.h, v, 5))
 
     # ------------------------------------------------------------ timing
     def profile(self, enable: bool = True) -> None:

@@ -140,7 +140,7 @@ def time_ef(eng, form, steps, warmup, out):
 def run_train(steps, warmup, form, graph=False):
     import torch
     from ann_amd import configs
-    from ann_amd.gpu import Engine
+    from ann_amd.gpu import Engine, FAST
     model = configs.model_lips()
     teacher = configs.model_lips(seed=33)
     batch = configs.batch_lips(128, seed=34)
@@ -153,6 +153,8 @@ def run_train(steps, warmup, form, graph=False):
     eng = Engine(model)
     eng.set_batch(batch)
     eng.neighbors()
+    if form == FAST:
+        eng.autotune(FAST)
     if graph:
         eng.train_graph(True)
     out = (torch.empty(2, dtype=torch.float64, device=eng.device),
@@ -185,12 +187,13 @@ def run_uneven(steps, warmup):
     eng = Engine(model)
     eng.set_batch(batch)
     eng.neighbors()
+    tuning = eng.autotune(FAST)
     P, maxn = eng.nlist_size()
     dev = eng.device
     out = (torch.empty(batch.nstruc, dtype=torch.float64, device=dev),
            torch.empty((batch.ntotal, 3), dtype=torch.float64, device=dev),
            torch.empty((batch.nstruc, 3, 3), dtype=torch.float64, device=dev))
-    res = {"structures": batch.nstruc, "atoms": batch.ntotal, "pairs": P}
+    res = {"structures": batch.nstruc, "atoms": batch.ntotal, "pairs": P, "tuning": tuning}
     for form, name, k in ((FAST, "fast", steps), (PLAIN, "plain", max(2, steps // 5))):
         dt = time_ef(eng, form, k, max(1, warmup // 2), out)
         res[name] = {"atoms_per_s": batch.ntotal / dt, "ms_per_step": dt * 1e3, "steps": k}
@@ -313,6 +316,7 @@ def main():
     eng = Engine(model)
     eng.set_batch(batch)
     eng.neighbors()
+    tuning = eng.autotune(FAST)          # atoms per workgroup, kernel family, search blocking
     nl = eng.nlist()
     flops = work_model(model, batch, nl)
     dev = eng.device
@@ -394,6 +398,8 @@ def main():
         "kernels": {name: res[name]["kernels"] for name in res},
         "roofline": roof,
         "kernel_rooflines": roofs,
+        "tuning": {"atoms_per_workgroup": {"G": tuning[0], "adjoint": tuning[1], "tangent": tuning[2]},
+                   "angular": "triplet" if tuning[3] else "moment", "search_blocks": "single" if tuning[4] else "2x2x2"},
         "train": train,
         "uneven_batch": uneven,
         "md": md,

@@ -191,6 +191,19 @@ int ann_train_graph(ann_ctx* ctx, int enable);
 /* copy the FP64 master parameters (device) to host */
 int ann_get_params(ann_ctx* ctx, double* params, int n);
 
+/* ---- tuning ----
+ * ann_autotune times the variants on the current batch and keeps the fastest
+ * (synchronous): atoms per workgroup (1, 2, 4) of the moment-form kernels
+ * for G, adjoint and tangent; moment form against triplet tiles; neighbour
+ * search in 2x2x2-bin or single-bin blocks.  Results do not depend on the
+ * choice beyond FP32 summation order.  Tuning vector (5 ints): atoms per
+ * workgroup for G, adjoint, tangent; angular kernels (0 moment, 1 triplet);
+ * search blocks (0 heuristic, 1 single bins).  No counterpart in the
+ * reference. */
+int ann_autotune(ann_ctx* ctx, int form);
+int ann_get_tuning(ann_ctx* ctx, int* tuning, int n);
+int ann_set_tuning(ann_ctx* ctx, const int* tuning, int n);
+
 /* ---- kernel timing with HIP events on the context stream ---- */
 int ann_profile(ann_ctx* ctx, int enable);
 /* accumulated time (ms) and launch count of the kernel family `name`;

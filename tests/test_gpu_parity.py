@@ -172,3 +172,34 @@ def test_network_parity(cases, oracle_mod, name, form):
     err = np.sqrt(((dE[:, :orc.ld] - dEo) ** 2).sum() / (dEo ** 2).sum())
     assert err <= 1e-5, err
     assert np.all(dE[:, orc.ld:] == 0)
+
+
+@pytest.mark.parametrize("tuning", [[2, 2, 2, 0, 0], [4, 4, 4, 0, 1], [1, 1, 1, 1, 0]])
+@pytest.mark.parametrize("name", ["water4", "lips3"])
+def test_tuning_variants_keep_parity(cases, oracle_mod, name, tuning):
+    """Every tuning choice (atoms per workgroup, moment/triplet kernels,
+    search blocking) gives the same list bit for bit and E/F within the bars."""
+    model, batch = cases[name]
+    from ann_amd.gpu import Engine
+    eng = Engine(model, "cuda:0")
+    eng.set_tuning(tuning)
+    eng.set_batch(batch)
+    eng.neighbors()
+    ref = _engine(model, batch).nlist()
+    nl = eng.nlist()
+    for k in ("off", "idx", "img", "rev"):
+        assert np.array_equal(nl[k], ref[k]), k
+    E, F, _ = eng.energy_forces(1)
+    Eo, Fo, _ = oracle_mod.Oracle(model).compute_batch(batch, nthreads=4)
+    assert np.max(np.abs(E.cpu().numpy() - Eo) / batch.natoms) <= 1e-6
+    assert np.sqrt(((F.cpu().numpy() - Fo) ** 2).sum() / (Fo ** 2).sum()) <= 1e-5
+
+
+def test_autotune_picks_valid_settings(cases, oracle_mod):
+    model, batch = cases["water4"]
+    eng = _engine(model, batch)
+    t = eng.autotune(1)
+    assert t[0] in (1, 2, 4) and t[1] in (1, 2, 4) and t[2] in (1, 2, 4) and t[3] in (0, 1) and t[4] in (0, 1)
+    E, F, _ = eng.energy_forces(1)
+    Eo, Fo, _ = oracle_mod.Oracle(model).compute_batch(batch, nthreads=4)
+    assert np.sqrt(((F.cpu().numpy() - Fo) ** 2).sum() / (Fo ** 2).sum()) <= 1e-5
