@@ -708,6 +708,10 @@ __global__ __launch_bounds__(64) void k_fast(SymmArgs A) {
 __host__ __device__ constexpr int nmono(int Z) { return (Z + 1) * (Z + 2) * (Z + 3) / 6; }
 __host__ __device__ constexpr int mono_start(int m) { return m * (m + 1) * (m + 2) / 6; }
 __host__ __device__ constexpr int ifact(int n) { return n <= 1 ? 1 : n * ifact(n - 1); }
+// position of x^a y^b z^c in the monomial order below
+__host__ __device__ constexpr int mono_index(int a, int b, int c) {
+    return mono_start(a + b + c) + (b + c) * (b + c + 1) / 2 + c;
+}
 
 // monomials of total degree <= kMomZ ordered by degree, then exponent of x and
 // y descending; every smaller degree bound is a prefix
@@ -981,10 +985,17 @@ __global__ __launch_bounds__(256) void k_moment(SymmArgs A) {
         while (b + 1 < S && J >= sm.seg[b + 1]) ++b;
         const float4 gq = sm.g[J];
         const float4 e4 = sm.ef[J], f4 = sm.efd[J];
-        float px[ZP], py[ZP], pz[ZP];
-        px[0] = py[0] = pz[0] = 1.0f;
+        // every monomial of r^ once (one product each from a lower one); the
+        // gradient of x^a y^b z^c is (a, b, c) times monomials of one degree less
+        float mono[NM];
+        mono[0] = 1.0f;
 #pragma unroll
-        for (int k = 1; k < ZP; ++k) { px[k] = px[k - 1] * gq.x; py[k] = py[k - 1] * gq.y; pz[k] = pz[k - 1] * gq.z; }
+        for (int q = 1; q < NM; ++q) {
+            const int ea = mt.a[q], eb = mt.b[q], ec = mt.c[q];
+            mono[q] = ea > 0 ? mono[mono_index(ea > 0 ? ea - 1 : 0, eb, ec)] * gq.x
+                             : (eb > 0 ? mono[mono_index(ea, eb > 0 ? eb - 1 : 0, ec)] * gq.y
+                                       : mono[mono_index(ea, eb, ec > 0 ? ec - 1 : 0)] * gq.z);
+        }
         float Qd = 0.0f, g0 = 0.0f, g1 = 0.0f, g2 = 0.0f;
         const float4* Ubb = reinterpret_cast<const float4*>(Ub + b * NM * U4);
 #pragma unroll
@@ -993,11 +1004,10 @@ __global__ __launch_bounds__(256) void k_moment(SymmArgs A) {
             const float4 w4 = Ubb[q];
             const float V = e4.x * w4.x + e4.y * w4.y + e4.z * w4.z + e4.w * w4.w;
             const float W = f4.x * w4.x + f4.y * w4.y + f4.z * w4.z + f4.w * w4.w;
-            const float yz = py[eb] * pz[ec];
-            Qd += W * (px[ea] * yz);
-            if (ea > 0) g0 += V * ((float)ea * px[ea > 0 ? ea - 1 : 0] * yz);
-            if (eb > 0) g1 += V * ((float)eb * px[ea] * py[eb > 0 ? eb - 1 : 0] * pz[ec]);
-            if (ec > 0) g2 += V * ((float)ec * px[ea] * py[eb] * pz[ec > 0 ? ec - 1 : 0]);
+            Qd += W * mono[q];
+            if (ea > 0) g0 += (V * (float)ea) * mono[mono_index(ea > 0 ? ea - 1 : 0, eb, ec)];
+            if (eb > 0) g1 += (V * (float)eb) * mono[mono_index(ea, eb > 0 ? eb - 1 : 0, ec)];
+            if (ec > 0) g2 += (V * (float)ec) * mono[mono_index(ea, eb, ec > 0 ? ec - 1 : 0)];
         }
         const float dd = 2.0f * (dD[b * U4] * e4.x * f4.x + dD[b * U4 + 1] * e4.y * f4.y +
                                  dD[b * U4 + 2] * e4.z * f4.z + dD[b * U4 + 3] * e4.w * f4.w);
