@@ -556,6 +556,38 @@ __device__ __forceinline__ void fast_radial(const SymmArgs& A, const DevModel& M
         const RadBlock rb = M.rad[a * S + b];
         if (rb.n == 0) continue;
         const int s0 = sm.seg[b], s1 = sm.seg[b + 1];
+        if (MODE == 1 && rb.name == RAD_GAUSSIAN) {
+            // 16 functions per pass: widths, centres and dE/dG held in registers,
+            // d/dr [e fc] = e (gc - 2 eta x fc) split into two running sums
+            for (int m0 = 0; m0 < rb.n; m0 += kRadChunk) {
+                float eta[kRadChunk], rs[kRadChunk], gs[kRadChunk];
+#pragma unroll
+                for (int m = 0; m < kRadChunk; ++m) {
+                    const bool in = m0 + m < rb.n;
+                    eta[m] = in ? A.rad_eta[rb.par + m0 + m] : 0.0f;
+                    rs[m] = in ? A.rad_rs[rb.par + m0 + m] : 0.0f;
+                    gs[m] = in ? sd[rb.goff + m0 + m] : 0.0f;
+                }
+                for (int J = s0 + lane; J < s1; J += kWave) {
+                    const float r = sm.r[J];
+                    float fc, gc;
+                    cutoff_fg(M.cutoff, M.rc, M.rci, M.pirc, r, fc, gc);
+                    float se = 0.0f, sx = 0.0f;
+#pragma unroll
+                    for (int m = 0; m < kRadChunk; ++m) {
+                        const float x = eta[m] * (r - rs[m]), ge = gs[m] * __expf(-x * x);
+                        se += ge;
+                        sx += ge * eta[m] * x;
+                    }
+                    const float amp = se * gc - 2.0f * fc * sx;
+                    const float4 gj = sm.g[J];
+                    sP[3 * J] += amp * gj.x;
+                    sP[3 * J + 1] += amp * gj.y;
+                    sP[3 * J + 2] += amp * gj.z;
+                }
+            }
+            continue;
+        }
         if (MODE == 1) {
             for (int J = s0 + lane; J < s1; J += kWave) {
                 const float4 gj = sm.g[J];
