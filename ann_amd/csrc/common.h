@@ -24,6 +24,17 @@ constexpr int kMomZ = 4;           // largest integer zeta of the moment form
 constexpr int kSumChunk = 4096;    // atoms per block of the per-structure sums
 constexpr int kSumQ = 11;          // summed quantities: energy, virial[9], kinetic energy
 
+// Workgroups are dispatched round robin over the 8 XCDs, each with its own L2.
+// Map dispatch order b of nb workgroups to a logical index so that XCD x runs
+// one contiguous range: kernels whose workgroups read their neighbours' data
+// (partner slots, partner positions) then find it in their own XCD's L2.
+// A bijection on [0, nb) for every nb.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+    const int x = b % kXcds, q = nb / kXcds, r = nb % kXcds;
+    return x * q + (x < r ? x : r) + b / kXcds;
+}
+
 // per (centre, neighbour) radial block
 struct RadBlock {
     int name, n, goff, par;        // par: start in rad_rs / rad_eta

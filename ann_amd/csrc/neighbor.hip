@@ -219,6 +219,8 @@ constexpr int kStageCap = 1280;
 constexpr int kStageImages = 216;
 constexpr int kStageThreads = 256;
 constexpr int kStageWaves = kStageThreads / kWave;
+constexpr int kStageIt = (kStageCap + kStageThreads - 1) / kStageThreads;
+constexpr int kShiftT = 3, kShiftW = 2 * kShiftT + 1, kShiftN = kShiftW * kShiftW * kShiftW;
 
 __device__ __forceinline__ void cswap_u64(uint64_t& a, uint64_t& b, bool asc) {
     const uint64_t lo = a < b ? a : b, hi = a < b ? b : a;
@@ -285,9 +287,10 @@ __global__ __launch_bounds__(kStageThreads) void k_search_bins(
     __shared__ double cx[kStageCap], cy[kStageCap], cz[kStageCap];
     __shared__ uint64_t cjd[kStageCap];                     // j << 32 | packed (image - shift_j)
     __shared__ uint64_t wkeys[kStageWaves][kMaxNeigh];
-    __shared__ int cen_bin[8], cen_pre[9];
+    __shared__ int cen_bin[8], cen_pre[9], cen_img[8];
+    __shared__ double shv[3][kShiftN];                      // image shift vectors R^T n, |n_c| <= kShiftT
     __shared__ int s_total, s_max;
-    const int g = blockIdx.x, t = threadIdx.x;
+    const int g = xcd_block(blockIdx.x, gridDim.x), t = threadIdx.x;
     const int lane = t % kWave, wave = t / kWave;
     const int s = bin_struc[g];                             // structure of this super-bin
     const CellInfo c = ci[s];
@@ -305,9 +308,19 @@ __global__ __launch_bounds__(kStageThreads) void k_search_bins(
                     const int bg = c.boff + ((lo0 + d0) * c.nb[1] + (lo1 + d1)) * c.nb[2] + (lo2 + d2);
                     cen_bin[q] = bg;
                     cen_pre[q] = acc;
+                    cen_img[q] = ((c.k[0] + d0) * (n1b + 2 * c.k[1]) + (c.k[1] + d1)) * (n2b + 2 * c.k[2]) + (c.k[2] + d2);
                     acc += bin_start[bg + 1] - bin_start[bg];
                 }
         cen_pre[q] = acc;
+    }
+    for (int q = t; q < kShiftN; q += kStageThreads) {
+        // the same FP64 expression as pair_vec_v, so a table entry is bitwise its value
+#pragma clang fp contract(off)
+        const double m0 = q / (kShiftW * kShiftW) - kShiftT, m1 = (q / kShiftW) % kShiftW - kShiftT,
+                     m2 = q % kShiftW - kShiftT;
+        shv[0][q] = __dadd_rn(__dadd_rn(__dmul_rn(R[0], m0), __dmul_rn(R[1], m1)), __dmul_rn(R[2], m2));
+        shv[1][q] = __dadd_rn(__dadd_rn(__dmul_rn(R[3], m0), __dmul_rn(R[4], m1)), __dmul_rn(R[5], m2));
+        shv[2][q] = __dadd_rn(__dadd_rn(__dmul_rn(R[6], m0), __dmul_rn(R[7], m1)), __dmul_rn(R[8], m2));
     }
     const int ncb = n0b * n1b * n2b;
     const int e0 = n0b + 2 * c.k[0], e1 = n1b + 2 * c.k[1], e2 = n2b + 2 * c.k[2];
@@ -354,21 +367,40 @@ __global__ __launch_bounds__(kStageThreads) void k_search_bins(
         if (t == 0) atomicOr(flag, 8);
         return;
     }
-    for (int q = t; q < total; q += kStageThreads) {
-        int lo = 0, hi = nimg;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (im_pre[mid] <= q) lo = mid; else hi = mid;
+    {
+        // every thread resolves all its candidates first, then issues their loads together
+        int src[kStageIt], mq[kStageIt];
+#pragma unroll
+        for (int r = 0; r < kStageIt; ++r) {
+            const int q = t + r * kStageThreads;
+            src[r] = -1;
+            if (q < total) {
+                int lo = 0, hi = nimg;
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (im_pre[mid] <= q) lo = mid; else hi = mid;
+                }
+                src[r] = im_first[lo] + (q - im_pre[lo]);
+                mq[r] = im_m[lo];
+            }
         }
-        const double4 r4 = rec[im_first[lo] + (q - im_pre[lo])];
-        const uint64_t js = (uint64_t)__double_as_longlong(r4.w);
-        int m0, m1, m2, s0, s1, s2;
-        unpack_img((uint32_t)im_m[lo], m0, m1, m2);
-        unpack_img((uint32_t)(js & 0xffffffffu), s0, s1, s2);
-        cx[q] = r4.x;
-        cy[q] = r4.y;
-        cz[q] = r4.z;
-        cjd[q] = (js & 0xffffffff00000000ull) | (uint64_t)pack_img(m0 - s0, m1 - s1, m2 - s2);
+        double4 r4[kStageIt];
+#pragma unroll
+        for (int r = 0; r < kStageIt; ++r)
+            if (src[r] >= 0) r4[r] = rec[src[r]];
+#pragma unroll
+        for (int r = 0; r < kStageIt; ++r) {
+            const int q = t + r * kStageThreads;
+            if (src[r] < 0) continue;
+            const uint64_t js = (uint64_t)__double_as_longlong(r4[r].w);
+            int m0, m1, m2, s0, s1, s2;
+            unpack_img((uint32_t)mq[r], m0, m1, m2);
+            unpack_img((uint32_t)(js & 0xffffffffu), s0, s1, s2);
+            cx[q] = r4[r].x;
+            cy[q] = r4[r].y;
+            cz[q] = r4[r].z;
+            cjd[q] = (js & 0xffffffff00000000ull) | (uint64_t)pack_img(m0 - s0, m1 - s1, m2 - s2);
+        }
     }
     __syncthreads();
     uint64_t* wk = wkeys[wave];
@@ -377,11 +409,16 @@ __global__ __launch_bounds__(kStageThreads) void k_search_bins(
     for (int cq = wave; cq < ncen; cq += kStageWaves) {
         int cb = 0;
         while (cb + 1 < ncb && cq >= cen_pre[cb + 1]) ++cb;
-        const double4 ri = rec[bin_start[cen_bin[cb]] + (cq - cen_pre[cb])];
-        const uint64_t jsi = (uint64_t)__double_as_longlong(ri.w);
+        // the centre is staged too: its own bin's zero image, where (image - shift) = -shift
+        const int qc = im_pre[cen_img[cb]] + (cq - cen_pre[cb]);
+        const double rix = cx[qc], riy = cy[qc], riz = cz[qc];
+        const uint64_t jsi = cjd[qc];
         const int i = (int)(jsi >> 32);
         int si0, si1, si2;
         unpack_img((uint32_t)(jsi & 0xffffffffu), si0, si1, si2);
+        si0 = -si0;
+        si1 = -si1;
+        si2 = -si2;
         int count = 0;
         for (int base = 0; base < total; base += kWave) {
             const int q = base + lane;
@@ -394,7 +431,16 @@ __global__ __launch_bounds__(kStageThreads) void k_search_bins(
                 unpack_img((uint32_t)(jd & 0xffffffffu), d0, d1, d2i);
                 const int n0 = si0 + d0, n1 = si1 + d1, n2 = si2 + d2i;
                 double x, y, z, d2;
-                pair_vec_v(ri.x, ri.y, ri.z, cx[q], cy[q], cz[q], R, n0, n1, n2, x, y, z, d2);
+                if (abs(n0) <= kShiftT && abs(n1) <= kShiftT && abs(n2) <= kShiftT) {
+#pragma clang fp contract(off)
+                    const int ti = ((n0 + kShiftT) * kShiftW + (n1 + kShiftT)) * kShiftW + (n2 + kShiftT);
+                    x = __dsub_rn(__dsub_rn(rix, cx[q]), shv[0][ti]);
+                    y = __dsub_rn(__dsub_rn(riy, cy[q]), shv[1][ti]);
+                    z = __dsub_rn(__dsub_rn(riz, cz[q]), shv[2][ti]);
+                    d2 = __dadd_rn(__dadd_rn(__dmul_rn(x, x), __dmul_rn(y, y)), __dmul_rn(z, z));
+                } else {
+                    pair_vec_v(rix, riy, riz, cx[q], cy[q], cz[q], R, n0, n1, n2, x, y, z, d2);
+                }
                 ok = d2 > 0.0 && d2 < rc2 && !(j == i && n0 == 0 && n1 == 0 && n2 == 0);
                 key = ((uint64_t)j << 30) | (uint64_t)pack_img(n0, n1, n2);
             }
@@ -431,9 +477,9 @@ __global__ __launch_bounds__(kStageThreads) void k_search_bins(
 __global__ void k_compact(int N, const double* __restrict__ pos, const int* __restrict__ sof,
                           const double* __restrict__ cells, const int* __restrict__ species,
                           const uint64_t* __restrict__ keys, const int* __restrict__ off,
-                          int* __restrict__ idx, uint32_t* __restrict__ img, float4* __restrict__ geo,
-                          uint8_t* __restrict__ nsp) {
-    const int i = blockIdx.x;
+                          int* __restrict__ idx, uint32_t* __restrict__ img, int* __restrict__ rev,
+                          float4* __restrict__ geo, uint8_t* __restrict__ nsp) {
+    const int i = xcd_block(blockIdx.x, gridDim.x);
     if (i >= N) return;
     const int o = off[i], n = off[i + 1] - o;
     const double* R = cells + 9 * sof[i];
@@ -447,31 +493,45 @@ __global__ void k_compact(int N, const double* __restrict__ pos, const int* __re
         pair_vec(pos, i, j, R, n0, n1, n2, x, y, z, d2);
         idx[o + t] = j;
         img[o + t] = p;
+        rev[o + t] = o + t;
         geo[o + t] = make_float4((float)x, (float)y, (float)z, (float)sqrt(d2));
         nsp[o + t] = (uint8_t)species[j];
     }
 }
 
-// reverse slot by binary search in the partner's sorted key list
-__global__ void k_rev(int N, const uint64_t* __restrict__ keys, const int* __restrict__ off,
-                      int* __restrict__ rev, int* __restrict__ flag) {
-    const int i = blockIdx.x;
+// Reverse slots.  rev is an involution, so each pair is resolved once, from its
+// upper slot (partner index above the centre's, or the same atom with the
+// larger packed image): a binary search in the partner's CSR run, which is
+// sorted by (partner index, packed image) like the keys, on the 4-byte partner
+// indices, then a scan over the few images of the same partner; the result
+// is written to both slots.  k_compact has set every slot to itself before,
+// so a slot without a partner (an asymmetric list, impossible for the
+// symmetric FP64 test) stays in range; the missing upper match is flagged.
+__global__ void k_rev(int N, const int* __restrict__ idx, const uint32_t* __restrict__ img,
+                      const int* __restrict__ off, int* __restrict__ rev, int* __restrict__ flag) {
+    const int i = xcd_block(blockIdx.x, gridDim.x);
     if (i >= N) return;
     const int o = off[i], n = off[i + 1] - o;
     for (int t = threadIdx.x; t < n; t += blockDim.x) {
-        const uint64_t key = keys[(size_t)i * kMaxNeigh + t];
-        const int j = (int)(key >> 30);
+        const int j = idx[o + t];
+        const uint32_t p = img[o + t];
         int n0, n1, n2;
-        unpack_img((uint32_t)(key & 0x3fffffffu), n0, n1, n2);
-        const uint64_t want = ((uint64_t)i << 30) | (uint64_t)pack_img(-n0, -n1, -n2);
-        const uint64_t* kj = keys + (size_t)j * kMaxNeigh;
-        int lo = 0, hi = off[j + 1] - off[j];
+        unpack_img(p, n0, n1, n2);
+        const uint32_t want = pack_img(-n0, -n1, -n2);
+        if (j < i || (j == i && p < want)) continue;       // lower slot: written by its partner
+        const int b = off[j], e = off[j + 1];
+        int lo = b, hi = e;
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
-            if (kj[mid] < want) lo = mid + 1; else hi = mid;
+            if (idx[mid] < i) lo = mid + 1; else hi = mid;
         }
-        if (lo < off[j + 1] - off[j] && kj[lo] == want) rev[o + t] = off[j] + lo;
-        else { rev[o + t] = o + t; atomicOr(flag, 4); }
+        while (lo < e && idx[lo] == i && img[lo] < want) ++lo;
+        if (lo < e && idx[lo] == i && img[lo] == want) {
+            rev[o + t] = lo;
+            rev[lo] = o + t;
+        } else {
+            atomicOr(flag, 4);
+        }
     }
 }
 
@@ -645,10 +705,10 @@ hipError_t neighbors_build(ann_ctx* ctx, double rc) {
     {
         Launch L(ctx, "nb_compact");
         k_compact<<<N, 128, 0, st>>>(N, ctx->d_pos, ctx->d_struc_of, ctx->d_cells, ctx->d_species, ctx->d_keys,
-                                     nl.off, nl.idx, nl.img, nl.geo, nl.nsp);
+                                     nl.off, nl.idx, nl.img, nl.rev, nl.geo, nl.nsp);
         // a missing reverse slot (impossible for the symmetric FP64 test) is
         // reported by the next synchronising call (ann_nlist_size, next build)
-        k_rev<<<N, 128, 0, st>>>(N, ctx->d_keys, nl.off, nl.rev, ctx->d_flag + 2);
+        k_rev<<<N, 128, 0, st>>>(N, nl.idx, nl.img, nl.off, nl.rev, ctx->d_flag + 2);
     }
     return hipGetLastError();
 }

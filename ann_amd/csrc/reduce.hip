@@ -27,7 +27,7 @@ __global__ __launch_bounds__(256) void k_force_atoms(int N, const int* __restric
                                                      const int* __restrict__ rev, const float4* __restrict__ geo,
                                                      const float* __restrict__ P, double* __restrict__ F,
                                                      double* __restrict__ avir) {
-    const int a = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+    const int a = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
     if (a >= N) return;
     double fx = 0, fy = 0, fz = 0;
