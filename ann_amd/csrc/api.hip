@@ -291,6 +291,8 @@ static int upload_model(ann_ctx* ctx, const ann_model_desc* d) {
     ANN_HIP(hipMemset(ctx->da.adam_v, 0, sizeof(double) * ptot));
     ANN_HIP(hipMalloc(&ctx->d_flag, sizeof(int) * 4));
     ANN_HIP(hipMemset(ctx->d_flag, 0, sizeof(int) * 4));
+    ANN_HIP(hipHostMalloc((void**)&ctx->h_rb, sizeof(int) * kReadbackWords, hipHostMallocDefault));
+    ANN_HIP(hipHostGetDevicePointer((void**)&ctx->d_rb, ctx->h_rb, 0));
     ANN_HIP(hipMalloc(&ctx->d_adam_t, sizeof(int)));
     ANN_HIP(hipMemset(ctx->d_adam_t, 0, sizeof(int)));
     return ANN_OK;
@@ -326,6 +328,7 @@ extern "C" int ann_destroy(ann_ctx* ctx) {
                     ctx->d_bin_struc, ctx->d_binrec, ctx->d_adam_t};
     for (void* p : ptrs)
         if (p) hipFree(p);
+    if (ctx->h_rb) hipHostFree(ctx->h_rb);
     for (hipEvent_t e : ctx->evpool) hipEventDestroy(e);
     if (ctx->gexec) hipGraphExecDestroy(ctx->gexec);
     for (hipEvent_t e : ctx->gev)
@@ -656,10 +659,9 @@ extern "C" int ann_md_step(ann_ctx* ctx, int form, double dt, const double* mass
     RET_HIP(md_kick(ctx, dt, mass, F, vel), "md_kick");
     if (KE) RET_HIP(struc_sums(ctx, nullptr, 0, nullptr, ctx->d_keatom, nullptr, nullptr, KE), "kinetic energy");
     if (moved) {
-        int flag = 0;
-        RET_HIP(hipMemcpyAsync(&flag, ctx->d_md_flag, sizeof(int), hipMemcpyDeviceToHost, ctx->stream), "flag");
+        RET_HIP(readback(ctx->d_md_flag, 1, ctx->d_rb + 8, ctx->stream), "flag");
         RET_HIP(hipStreamSynchronize(ctx->stream), "synchronize");
-        *moved = flag;
+        *moved = ((volatile const int*)ctx->h_rb)[8];
     }
     return ANN_OK;
 }

@@ -23,6 +23,7 @@ constexpr int kGridK = 8;          // fast angular tile: (lambda, zeta) combinat
 constexpr int kMomZ = 4;           // largest integer zeta of the moment form
 constexpr int kSumChunk = 4096;    // atoms per block of the per-structure sums
 constexpr int kSumQ = 11;          // summed quantities: energy, virial[9], kinetic energy
+constexpr int kReadbackWords = 16; // pinned host words of a context (flags and counts for the host)
 
 // Workgroups are dispatched round robin over the 8 XCDs, each with its own L2.
 // Map dispatch order b of nb workgroups to a logical index so that XCD x runs
@@ -181,6 +182,8 @@ struct ann_ctx {
     int64_t binrec_cap = 0;
     int* d_cnt = nullptr;             // [N] neighbour counts
     int* d_flag = nullptr;            // error flags
+    int* h_rb = nullptr;              // pinned host words written by kernels for the host to read after a sync
+    int* d_rb = nullptr;              // the same words as seen from the device
     int64_t keys_cap = 0;
     // work buffers
     float* d_G = nullptr;
@@ -252,6 +255,12 @@ struct Launch {
     ~Launch();
 };
 hipError_t neighbors_build(ann_ctx* ctx, double rc);
+// exclusive scan of n ints (out[n-1] = sum of in[0..n-2]); with rb set, also
+// copies nflags words of flags, then out[n-1], to the pinned words rb
+hipError_t exclusive_scan(const int* in, int* out, int n, hipStream_t st, const int* flags = nullptr,
+                          int nflags = 0, int* rb = nullptr);
+// copy n device ints to the pinned words rb (stream ordered; read after a sync)
+hipError_t readback(const int* src, int n, int* rb, hipStream_t st);
 hipError_t symm_fwd(ann_ctx* ctx, int form, float* G);
 hipError_t symm_bwd(ann_ctx* ctx, int form, const float* dEdG, float* P);
 hipError_t symm_tangent(ann_ctx* ctx, int form, const float* u, float* v);

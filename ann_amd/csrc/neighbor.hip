@@ -555,7 +555,91 @@ static hipError_t grow(T*& p, int64_t& cap, int64_t want) {
     return e;
 }
 
-hipError_t exclusive_scan(const int* in, int* out, int n, hipStream_t st) {
+// Small exclusive scans in one workgroup: tiles of kScanThreads x 4 values,
+// a wavefront scan of the per-thread sums, the wavefront totals scanned by the
+// first wavefront, a carry across tiles.  One launch instead of the two of a
+// decoupled device scan, and the host-visible words written by the same kernel.
+constexpr int kScanThreads = 1024;
+constexpr int kScanMax = 1 << 16;
+
+__global__ void k_clear2(int* __restrict__ a, int na, int* __restrict__ b, int nb) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < na) a[t] = 0;
+    else if (t < na + nb) b[t - na] = 0;
+}
+
+__device__ __forceinline__ void store_host(int* rb, int q, int v) {
+    __hip_atomic_store(rb + q, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_scan_small(const int* __restrict__ in, int* __restrict__ out, int n,
+                                                             const int* __restrict__ flags, int nflags, int* rb) {
+    __shared__ int wsum[kScanThreads / kWave];
+    __shared__ int s_carry;
+    const int t = threadIdx.x, lane = t % kWave, w = t / kWave;
+    if (t == 0) s_carry = 0;
+    __syncthreads();
+    for (int base = 0; base < n; base += 4 * kScanThreads) {
+        int v[4], sum = 0;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = base + 4 * t + q;
+            v[q] = e < n ? in[e] : 0;
+            sum += v[q];
+        }
+        int inc = sum;   // inclusive scan over the wavefront
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const int u = __shfl_up(inc, o);
+            if (lane >= o) inc += u;
+        }
+        if (lane == kWave - 1) wsum[w] = inc;
+        __syncthreads();
+        if (w == 0) {
+            int x = lane < kScanThreads / kWave ? wsum[lane] : 0;
+#pragma unroll
+            for (int o = 1; o < kScanThreads / kWave; o <<= 1) {
+                const int u = __shfl_up(x, o);
+                if (lane >= o) x += u;
+            }
+            if (lane < kScanThreads / kWave) wsum[lane] = x;   // inclusive over wavefronts
+        }
+        __syncthreads();
+        int run = s_carry + (w > 0 ? wsum[w - 1] : 0) + inc - sum;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = base + 4 * t + q;
+            if (e < n) out[e] = run;
+            run += v[q];
+        }
+        __syncthreads();
+        if (t == 0) s_carry += wsum[kScanThreads / kWave - 1];
+        __syncthreads();
+    }
+    if (rb && t == kScanThreads - 1) {
+        // the last element's exclusive prefix is the carry minus the last value
+        for (int q = 0; q < nflags; ++q) store_host(rb, q, flags[q]);
+        store_host(rb, nflags, n > 0 ? s_carry - in[n - 1] : 0);
+        __threadfence_system();
+    }
+}
+
+__global__ void k_readback(const int* __restrict__ src, int n, int* rb) {
+    const int t = threadIdx.x;
+    if (t < n) store_host(rb, t, src[t]);
+    __threadfence_system();
+}
+
+hipError_t readback(const int* src, int n, int* rb, hipStream_t st) {
+    k_readback<<<1, 64, 0, st>>>(src, n, rb);
+    return hipGetLastError();
+}
+
+hipError_t exclusive_scan(const int* in, int* out, int n, hipStream_t st, const int* flags, int nflags, int* rb) {
+    if (n <= kScanMax) {
+        k_scan_small<<<1, kScanThreads, 0, st>>>(in, out, n, flags, nflags, rb);
+        return hipGetLastError();
+    }
     size_t bytes = 0;
     hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n, st);
     if (e != hipSuccess) return e;
@@ -567,7 +651,12 @@ hipError_t exclusive_scan(const int* in, int* out, int n, hipStream_t st) {
         if (e != hipSuccess) { tmp = nullptr; tmp_cap = 0; return e; }
         tmp_cap = bytes;
     }
-    return hipcub::DeviceScan::ExclusiveSum(tmp, bytes, in, out, n, st);
+    if ((e = hipcub::DeviceScan::ExclusiveSum(tmp, bytes, in, out, n, st)) != hipSuccess) return e;
+    if (rb) {
+        if (nflags > 0 && (e = readback(flags, nflags, rb, st)) != hipSuccess) return e;
+        e = readback(out + n - 1, 1, rb + nflags, st);
+    }
+    return e;
 }
 
 #define HIPCHK(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) return e_; } while (0)
@@ -649,8 +738,11 @@ hipError_t neighbors_build(ann_ctx* ctx, double rc) {
     ctx->binrec_cap = rcap;
     const CellInfo* dci = reinterpret_cast<const CellInfo*>(ctx->d_cellinfo);
     int* bin_fill = ctx->d_bin_cnt + (nbins + 1);
-    HIPCHK(hipMemsetAsync(ctx->d_bin_cnt, 0, sizeof(int) * 3 * (size_t)ctx->bin_cap, st));
-    HIPCHK(hipMemsetAsync(ctx->d_flag, 0, sizeof(int) * 2, st));   // flag[2]: sticky reverse-slot error
+    {
+        // bin counters, fill cursors, overflow marks and flag[0..1] (flag[2]: sticky reverse-slot error)
+        const int nc = 3 * ctx->bin_cap;
+        k_clear2<<<(nc + 2 + 255) / 256, 256, 0, st>>>(ctx->d_bin_cnt, nc, ctx->d_flag, 2);
+    }
     {
         Launch L(ctx, "nb_bin");
         k_bin<<<(N + 255) / 256, 256, 0, st>>>(N, ctx->d_pos, ctx->d_struc_of, dci, ctx->d_cellmat,
@@ -666,12 +758,12 @@ hipError_t neighbors_build(ann_ctx* ctx, double rc) {
                                                       ctx->d_bin_ovf);
     }
     HIPCHK(hipGetLastError());
-    HIPCHK(exclusive_scan(ctx->d_cnt, ctx->nl.off, N + 1, st));
-    int flags[4] = {0, 0, 0, 0};
-    int total = 0;
-    HIPCHK(hipMemcpyAsync(flags, ctx->d_flag, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(&total, ctx->nl.off + N, sizeof(int), hipMemcpyDeviceToHost, st));
+    // offsets, then flags and pair count to pinned host words: the build's one synchronisation
+    HIPCHK(exclusive_scan(ctx->d_cnt, ctx->nl.off, N + 1, st, ctx->d_flag, 4, ctx->d_rb));
     HIPCHK(hipStreamSynchronize(st));
+    volatile const int* rb = ctx->h_rb;
+    int flags[4] = {rb[0], rb[1], rb[2], rb[3]};
+    int total = rb[4];
     if (flags[2]) { set_error("the previous neighbour list was not symmetric (reverse slot missing)"); return hipErrorInvalidValue; }
     if (flags[0] & 8) {
         // bins with a search box too large to stage: one wavefront per centre atom
@@ -682,10 +774,10 @@ hipError_t neighbors_build(ann_ctx* ctx, double rc) {
                                        ctx->d_cnt, ctx->d_flag, ctx->d_bin_ovf);
         }
         HIPCHK(hipGetLastError());
-        HIPCHK(exclusive_scan(ctx->d_cnt, ctx->nl.off, N + 1, st));
-        HIPCHK(hipMemcpyAsync(flags, ctx->d_flag, sizeof(int) * 4, hipMemcpyDeviceToHost, st));
-        HIPCHK(hipMemcpyAsync(&total, ctx->nl.off + N, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(exclusive_scan(ctx->d_cnt, ctx->nl.off, N + 1, st, ctx->d_flag, 4, ctx->d_rb));
         HIPCHK(hipStreamSynchronize(st));
+        for (int q = 0; q < 4; ++q) flags[q] = rb[q];
+        total = rb[4];
     }
     if (flags[0] & 2) { set_error("neighbour search box exceeds the image limit (cell far narrower than rc)"); return hipErrorInvalidValue; }
     if (flags[0] & 1) {

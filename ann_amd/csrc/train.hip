@@ -26,7 +26,6 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 hipError_t species_perm(ann_ctx* ctx);
 bool mfma_net_supported(const DevModel& M);
 
-hipError_t exclusive_scan(const int* in, int* out, int n, hipStream_t st);
 hipError_t ensure_work_buffers(ann_ctx* ctx);
 
 // per-atom stride of the second-backward scratch: per layer x, xd (n_in), zbar, zdbar (n_out)
