@@ -5,8 +5,8 @@
 // (j -> i, -n).  This folds the reaction of every periodic image back onto
 // its owner, i.e. it is the exact adjoint of the reference force loops
 // (src/nnp/nnp.cpp:489-516, which apply reactions only to min() entries).
-// Each atom gathers its own sum in FP64 (a wavefront per atom, lanes over
-// slots, fixed butterfly; no atomics, deterministic).
+// Each atom gathers its own sum in FP64 (16 lanes per atom, lanes over
+// slots, fixed transposing butterfly; no atomics, deterministic).
 // Energy per structure (NNP::energy src/nnp/nnp.cpp:436-447, vacuum energies
 // optional as in NNH::energy src/nnp/nnh.cpp:118-121) and the virial
 // W = -sum_slots P (x) r are FP64 tree reductions over fixed atom chunks.
@@ -16,53 +16,73 @@
 
 namespace ann {
 
-__device__ __forceinline__ double wave_sum_d(double v) {
+// Sum NV per-lane doubles over groups of 16 lanes with a transposing
+// butterfly (halving steps over lane bits 8, 4, 2, 1): afterwards lane l of a
+// group holds the group total of value index vidx(l) for l < NV rounded up to
+// a power of two (16 lanes over NV <= 16 values, then plain steps).
+template <int NV>
+__device__ __forceinline__ double group16_sum_multi(double (&v)[NV], int& idx) {
+    const int lane = threadIdx.x % 16;
+    int base = 0;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-    return v;
+    for (int h = NV / 2, bit = 8; h >= 1; h >>= 1, bit >>= 1) {
+        const bool upper = (lane & bit) != 0;
+#pragma unroll
+        for (int q = 0; q < h; ++q) {
+            const double send = upper ? v[q] : v[q + h];
+            const double keep = upper ? v[q + h] : v[q];
+            v[q] = keep + __shfl_xor(send, bit);
+        }
+        if (upper) base += h;
+    }
+    double s = v[0];
+    constexpr int levels = NV >= 16 ? 0 : (NV >= 8 ? 1 : (NV >= 4 ? 2 : (NV >= 2 ? 3 : 4)));
+    int bit = 1;
+#pragma unroll
+    for (int q = 0; q < levels; ++q, bit <<= 1) s += __shfl_xor(s, bit);
+    idx = base;
+    return s;
 }
 
-// one wavefront per atom: lanes over its slots, fixed-order FP64 butterfly
+// 16 lanes per atom (four atoms per wavefront): lanes over the atom's slots,
+// then one fixed-order transposing FP64 reduction of the 3 force (and 9
+// virial) sums
+template <bool VIR>
 __global__ __launch_bounds__(256) void k_force_atoms(int N, const int* __restrict__ off,
                                                      const int* __restrict__ rev, const float4* __restrict__ geo,
                                                      const float* __restrict__ P, double* __restrict__ F,
                                                      double* __restrict__ avir) {
-    const int a = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
-    const int lane = threadIdx.x % kWave;
-    if (a >= N) return;
-    double fx = 0, fy = 0, fz = 0;
-    double v[9];
+    constexpr int NV = VIR ? 16 : 4;           // 3 (+ 9) sums padded to a power of two
+    const int a = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x / 16) + threadIdx.x / 16;
+    const int sub = threadIdx.x % 16;
+    // every lane takes part in the shuffles; atoms beyond N sum nothing
+    const int s0 = a < N ? off[a] : 0, s1 = a < N ? off[a + 1] : 0;
+    double v[NV];
 #pragma unroll
-    for (int q = 0; q < 9; ++q) v[q] = 0.0;
-    const int s0 = off[a], s1 = off[a + 1];
-    for (int s = s0 + lane; s < s1; s += kWave) {
+    for (int q = 0; q < NV; ++q) v[q] = 0.0;
+    for (int s = s0 + sub; s < s1; s += 16) {
         const int r = rev[s];
         const double px = P[3 * (size_t)s], py = P[3 * (size_t)s + 1], pz = P[3 * (size_t)s + 2];
-        fx += (double)P[3 * (size_t)r] - px;
-        fy += (double)P[3 * (size_t)r + 1] - py;
-        fz += (double)P[3 * (size_t)r + 2] - pz;
-        if (avir) {
+        v[0] += (double)P[3 * (size_t)r] - px;
+        v[1] += (double)P[3 * (size_t)r + 1] - py;
+        v[2] += (double)P[3 * (size_t)r + 2] - pz;
+        if (VIR) {
             const float4 g = geo[s];
             const double rr[3] = {g.x, g.y, g.z}, pp[3] = {px, py, pz};
 #pragma unroll
             for (int x = 0; x < 3; ++x)
 #pragma unroll
-                for (int y = 0; y < 3; ++y) v[3 * x + y] -= pp[x] * rr[y];
+                for (int y = 0; y < 3; ++y) v[3 + 3 * x + y] -= pp[x] * rr[y];
         }
     }
-    fx = wave_sum_d(fx);
-    fy = wave_sum_d(fy);
-    fz = wave_sum_d(fz);
-    if (F && lane == 0) {
-        F[3 * (size_t)a] = fx;
-        F[3 * (size_t)a + 1] = fy;
-        F[3 * (size_t)a + 2] = fz;
-    }
-    if (avir) {
-#pragma unroll
-        for (int q = 0; q < 9; ++q) {
-            const double t = wave_sum_d(v[q]);
-            if (lane == 0) avir[9 * (size_t)a + q] = t;
+    int idx;
+    const double t = group16_sum_multi<NV>(v, idx);
+    constexpr int kRep = 16 / NV;              // lanes holding the same value index
+    if (a < N && sub % kRep == 0) {
+        if (idx < 3) {
+            if (F) F[3 * (size_t)a + idx] = t;
+        } else if (VIR && idx < 12) {
+            avir[9 * (size_t)a + idx - 3] = t;
         }
     }
 }
@@ -141,8 +161,12 @@ hipError_t force_reduce(ann_ctx* ctx, const float* P, const float* Eatom, int wi
     if (N == 0) return hipSuccess;
     {
         Launch L(ctx, "force_gather");
-        k_force_atoms<<<(N + 3) / 4, 256, 0, ctx->stream>>>(N, ctx->nl.off, ctx->nl.rev, ctx->nl.geo, P, F,
-                                                                 V ? ctx->d_atomvir : nullptr);
+        if (V)
+            k_force_atoms<true><<<(N + 15) / 16, 256, 0, ctx->stream>>>(N, ctx->nl.off, ctx->nl.rev, ctx->nl.geo, P,
+                                                                        F, ctx->d_atomvir);
+        else
+            k_force_atoms<false><<<(N + 15) / 16, 256, 0, ctx->stream>>>(N, ctx->nl.off, ctx->nl.rev, ctx->nl.geo, P,
+                                                                         F, nullptr);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
