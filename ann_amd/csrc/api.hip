@@ -755,14 +755,18 @@ static double time_ms(ann_ctx* ctx, hipError_t (*f)(ann_ctx*, int), int arg, int
     err = hipEventCreate(&a);
     if (err == hipSuccess) err = hipEventCreate(&b);
     if (err == hipSuccess) err = f(ctx, arg);                       // warm-up
-    if (err == hipSuccess) err = hipEventRecord(a, ctx->stream);
-    for (int r = 0; r < reps && err == hipSuccess; ++r) err = f(ctx, arg);
-    if (err == hipSuccess) err = hipEventRecord(b, ctx->stream);
-    if (err == hipSuccess) err = hipEventSynchronize(b);
-    if (err == hipSuccess) err = hipEventElapsedTime(&ms, a, b);
+    double best = 1e30;
+    for (int trial = 0; trial < 3 && err == hipSuccess; ++trial) {  // the fastest of three timings
+        err = hipEventRecord(a, ctx->stream);
+        for (int r = 0; r < reps && err == hipSuccess; ++r) err = f(ctx, arg);
+        if (err == hipSuccess) err = hipEventRecord(b, ctx->stream);
+        if (err == hipSuccess) err = hipEventSynchronize(b);
+        if (err == hipSuccess) err = hipEventElapsedTime(&ms, a, b);
+        if (err == hipSuccess) best = std::min(best, (double)ms / reps);
+    }
     if (a) hipEventDestroy(a);
     if (b) hipEventDestroy(b);
-    return (double)ms / reps;
+    return best;
 }
 static hipError_t run_fwd(ann_ctx* c, int form) { return symm_fwd(c, form, c->d_G); }
 static hipError_t run_bwd(ann_ctx* c, int form) { return symm_bwd(c, form, c->d_dEdG, c->d_P); }

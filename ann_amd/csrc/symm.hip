@@ -241,6 +241,7 @@ struct FastSmem {
     float4* g;      // [maxn] (hx, hy, hz, 1/r), species-sorted
     float4* ef;     // [maxn] exp(-r^2 ieta2_u) fc for u = 0..3
     float4* efd;    // [maxn] d/dr of ef
+    float2* fg;     // [maxn] cutoff and its derivative
     float* r;       // [maxn]
     int* canon;     // [maxn] canonical slot of each sorted entry
     int* seg;       // [kMaxSpecies + 1]
@@ -337,6 +338,7 @@ __device__ __forceinline__ int fast_load(const SymmArgs& A, const DevModel& M, i
             sm.canon[pos] = t;
             float f, gcut;
             cutoff_fg(M.cutoff, M.rc, M.rci, M.pirc, r, f, gcut);
+            sm.fg[pos] = make_float2(f, gcut);
             float e[kFastU], d[kFastU];
 #pragma unroll
             for (int u = 0; u < kFastU; ++u) {
@@ -570,8 +572,8 @@ __device__ __forceinline__ void fast_radial(const SymmArgs& A, const DevModel& M
                 }
                 for (int J = s0 + lane; J < s1; J += kWave) {
                     const float r = sm.r[J];
-                    float fc, gc;
-                    cutoff_fg(M.cutoff, M.rc, M.rci, M.pirc, r, fc, gc);
+                    const float2 fg = sm.fg[J];
+                    const float fc = fg.x, gc = fg.y;
                     float se = 0.0f, sx = 0.0f;
 #pragma unroll
                     for (int m = 0; m < kRadChunk; ++m) {
@@ -591,8 +593,8 @@ __device__ __forceinline__ void fast_radial(const SymmArgs& A, const DevModel& M
         if (MODE == 1) {
             for (int J = s0 + lane; J < s1; J += kWave) {
                 const float4 gj = sm.g[J];
-                float fc, gc;
-                cutoff_fg(M.cutoff, M.rc, M.rci, M.pirc, sm.r[J], fc, gc);
+                const float2 fg = sm.fg[J];
+                const float fc = fg.x, gc = fg.y;
                 float amp = 0.0f;
                 for (int m = 0; m < rb.n; ++m) {
                     float v, d;
@@ -611,8 +613,9 @@ __device__ __forceinline__ void fast_radial(const SymmArgs& A, const DevModel& M
                 float acc = 0.0f;
                 for (int J = s0 + lane; J < s1; J += kWave) {
                     const float r = sm.r[J];
-                    float fc, gc, v, d;
-                    cutoff_fg(M.cutoff, M.rc, M.rci, M.pirc, r, fc, gc);
+                    const float2 fg = sm.fg[J];
+                    const float fc = fg.x, gc = fg.y;
+                    float v, d;
                     radial_fd(rb.name, A.rad_eta[rb.par + m], A.rad_rs[rb.par + m], r, fc, gc, v, d);
                     acc += MODE == 0 ? v : d * tv[J].w;
                 }
@@ -634,8 +637,8 @@ __device__ __forceinline__ void fast_radial(const SymmArgs& A, const DevModel& M
             for (int J = s0 + lane; J < s1; J += kWave) {
                 const float r = sm.r[J];
                 const float w = MODE == 2 ? tv[J].w : 0.0f;
-                float fc, gc;
-                cutoff_fg(M.cutoff, M.rc, M.rci, M.pirc, r, fc, gc);
+                const float2 fg = sm.fg[J];
+                const float fc = fg.x, gc = fg.y;
 #pragma unroll
                 for (int m = 0; m < kRadChunk; ++m) {
                     const float x = eta[m] * (r - rs[m]), e = __expf(-x * x);
@@ -664,7 +667,8 @@ __global__ __launch_bounds__(64) void k_fast(SymmArgs A) {
     sm.ef = sm.g + maxn;
     sm.efd = sm.ef + maxn;
     float4* tv = sm.efd + (MODE == 0 ? 0 : maxn);                // [maxn] (mode 2)
-    sm.r = reinterpret_cast<float*>(tv + (MODE == 2 ? maxn : 0));
+    sm.fg = reinterpret_cast<float2*>(tv + (MODE == 2 ? maxn : 0));
+    sm.r = reinterpret_cast<float*>(sm.fg + maxn);
     sm.canon = reinterpret_cast<int*>(sm.r + maxn);
     sm.seg = sm.canon + maxn;
     sm.extra = reinterpret_cast<float*>(sm.seg + 16);
@@ -803,8 +807,11 @@ __global__ __launch_bounds__(256) void k_moment(SymmArgs A) {
     float* Am = dD + S * U4;                                    // [NP][U4][ZP]
     float* Ub = Am + NP * U4 * ZP;                              // [S][NM][U4] (mode 1)
     float* part = Ub + (MODE == 1 ? S * NM * U4 : 0);           // [NG][NM][U4] x (1 or 2)
+    float* wq = part + (NG > 1 ? NG * NM * U4 * (MODE == 2 ? 2 : 1) : 0);   // [NM] monomial weights
+    sm.fg = reinterpret_cast<float2*>(pw);   // cutoffs live in the power rows until the radial part is done
 
     n = fast_load(A, Mo, i, a, o, n, sm, MODE != 0, MODE == 1);
+    if (lane < NM) wq[lane] = c_mono.w[lane];
     if (MODE == 1) {
         for (int m = lane; m < ldg; m += kWave) rowbuf[m] = A.dEdG[(size_t)i * ldg + m];
         for (int t = lane; t < 3 * n; t += kWave) sP[t] = 0.0f;
@@ -823,6 +830,7 @@ __global__ __launch_bounds__(256) void k_moment(SymmArgs A) {
         wave_sync();
     }
     fast_radial<MODE>(A, Mo, a, sm, tv, rowbuf, sP, rowbuf);
+    wave_sync();
     // powers of the unit vector (and, for the tangent, their derivatives along
     // the projected direction (t - r^ (r^.t)) / r)
     for (int J = lane; J < n; J += kWave) {
@@ -971,7 +979,7 @@ __global__ __launch_bounds__(256) void k_moment(SymmArgs A) {
             const float* dTc = dT + (c * U4 + u) * NM;
             const int q0 = mono_start(m), q1 = mono_start(m + 1);
             for (int q = q0; q < q1; ++q) {
-                const float w = c_mono.w[q];
+                const float w = wq[q];
                 s += MODE == 0 ? w * Tb[q] * Tc[q] : w * (dTb[q] * Tc[q] + Tb[q] * dTc[q]);
             }
             if (b == c) s = 0.5f * (s - (MODE == 0 ? D[b * U4 + u] : dD[b * U4 + u]));
@@ -998,10 +1006,12 @@ __global__ __launch_bounds__(256) void k_moment(SymmArgs A) {
     // adjoint: dE/dT_b[alpha][u] and dE/dD_b[u]
     for (int it = lane; it < S * NM * U4; it += kWave) {
         const int b = it / (NM * U4), al = (it / U4) % NM, u = it % U4;
-        const int m = c_mono.m[al];
+        int m = 0;   // degree of monomial al
+#pragma unroll
+        for (int mm = 1; mm <= Z; ++mm) m += al >= mono_start(mm) ? 1 : 0;
         float s = 0.0f;
         for (int c = 0; c < S; ++c) s += Am[(pair_of(b, c) * U4 + u) * ZP + m] * T[(c * U4 + u) * NM + al];
-        Ub[it] = c_mono.w[al] * s;
+        Ub[it] = wq[al] * s;
     }
     if (lane < S * U4) {
         const int b = lane / U4, u = lane % U4;
@@ -1068,6 +1078,7 @@ static size_t moment_smem(const SymmArgs& A, int mode, int Z, int S, int NP) {
     f += (size_t)S * U4 * NM * (mode == 2 ? 2 : 1) + 2 * (size_t)S * U4 + (size_t)NP * U4 * ZP;
     f += (mode == 1 ? (size_t)S * NM * U4 : 0);
     f += (NG > 1 ? (size_t)NG * NM * U4 * (mode == 2 ? 2 : 1) : 0);
+    f += NM;
     return b + 4 * f;
 }
 
@@ -1102,7 +1113,7 @@ static size_t plain_smem(const SymmArgs& A, int mode) {
 
 static size_t fast_smem(const SymmArgs& A, int mode) {
     size_t b = 16 * (size_t)A.maxn * (2 + (mode == 0 ? 0 : 1) + (mode == 2 ? 1 : 0));   // g, ef, efd, tv
-    b += 4 * 2 * (size_t)A.maxn + 4 * 16;                                              // r, canon, seg
+    b += 8 * (size_t)A.maxn + 4 * 2 * (size_t)A.maxn + 4 * 16;                         // fg, r, canon, seg
     b += 4 * (size_t)A.ldg;
     if (mode == 1) b += 12 * (size_t)A.maxn;
     return b;
