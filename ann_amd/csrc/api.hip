@@ -267,6 +267,26 @@ static int upload_model(ann_ctx* ctx, const ann_model_desc* d) {
     }
     ctx->nparams = ptot;
     ctx->S = S;
+    {
+        // operand tables of the matrix-core network: for hidden layer l, W_l as
+        // the right operand of the forward product (K = inputs, columns =
+        // neurons) and W_l^T for the backward product (K = neurons, columns =
+        // inputs); per (k block of 32, column tile of 16, BF16 term) one
+        // 16-byte vector per lane
+        int acc = 0;
+        for (int a = 0; a < S; ++a) {
+            NetDesc& D = M.net[a];
+            for (int l = 0; l < D.nlayer - 1; ++l) {
+                const int ni = l == 0 ? D.nin : D.nodes[l - 1], no = D.nodes[l];
+                D.wsf[l] = acc;
+                acc += (ni + 31) / 32 * ((no + 15) / 16) * 3 * kWave;
+                D.wsb[l] = acc;
+                acc += (no + 31) / 32 * ((ni + 15) / 16) * 3 * kWave;
+            }
+        }
+        ctx->wsplit_n = acc;
+        ctx->wsplit_valid = false;
+    }
     // device copies
     ANN_HIP(hipMalloc(&ctx->dm, sizeof(DevModel)));
     ANN_HIP(hipMemcpy(ctx->dm, &M, sizeof(DevModel), hipMemcpyHostToDevice));
@@ -281,6 +301,7 @@ static int upload_model(ann_ctx* ctx, const ann_model_desc* d) {
     ANN_HIP(up(ctx->da.ang_zeta, azeta));
     ANN_HIP(up(ctx->da.ang_lam, alam));
     ANN_HIP(up(ctx->da.W, W));
+    ANN_HIP(hipMalloc(&ctx->d_wsplit, sizeof(uint4) * std::max(ctx->wsplit_n, 1)));
     ANN_HIP(up(ctx->da.inpw, iw));
     ANN_HIP(up(ctx->da.inpb, ib));
     ANN_HIP(hipMalloc(&ctx->da.params, sizeof(double) * ptot));
@@ -295,6 +316,8 @@ static int upload_model(ann_ctx* ctx, const ann_model_desc* d) {
     ANN_HIP(hipHostGetDevicePointer((void**)&ctx->d_rb, ctx->h_rb, 0));
     ANN_HIP(hipMalloc(&ctx->d_adam_t, sizeof(int)));
     ANN_HIP(hipMemset(ctx->d_adam_t, 0, sizeof(int)));
+    ANN_HIP(split_weights(ctx, nullptr));
+    ANN_HIP(hipStreamSynchronize(ctx->stream));
     return ANN_OK;
 }
 
@@ -325,7 +348,7 @@ extern "C" int ann_destroy(ann_ctx* ctx) {
                     ctx->d_Eatom, ctx->d_P, ctx->d_u, ctx->d_v, ctx->d_F, ctx->d_E, ctx->d_gE, ctx->d_atomvir,
                     ctx->d_scr, ctx->d_gpart, ctx->d_rank, ctx->d_spcount, ctx->d_perm, ctx->d_tiles, ctx->d_spend, ctx->d_grad_tiles, ctx->d_gsplit,
                     ctx->d_chunks, ctx->d_chunk_off, ctx->d_part, ctx->d_pos_ref, ctx->d_keatom, ctx->d_md_flag,
-                    ctx->d_bin_struc, ctx->d_binrec, ctx->d_adam_t};
+                    ctx->d_bin_struc, ctx->d_binrec, ctx->d_adam_t, ctx->d_wsplit};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (ctx->h_rb) hipHostFree(ctx->h_rb);
@@ -360,6 +383,7 @@ extern "C" int ann_set_params(ann_ctx* ctx, const double* p, int n) {
     std::vector<float> W(p, p + n);
     ANN_HIP(hipMemcpyAsync(ctx->da.params, p, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
     ANN_HIP(hipMemcpyAsync(ctx->da.W, W.data(), sizeof(float) * n, hipMemcpyHostToDevice, ctx->stream));
+    ANN_HIP(split_weights(ctx, nullptr));
     ANN_HIP(hipStreamSynchronize(ctx->stream));
     ANN_HIP(hipMemsetAsync(ctx->d_adam_t, 0, sizeof(int), ctx->stream));
     ANN_HIP(hipMemsetAsync(ctx->da.adam_m, 0, sizeof(double) * n, ctx->stream));

@@ -61,6 +61,8 @@ struct NetDesc {
     int poff;                                  // start of this species in the packed params
     int nparam;
     int inoff;                                 // into inpw / inpb
+    int wsf[kMaxLayers], wsb[kMaxLayers];      // hidden layer l: BF16x3 operand tables of W_l and W_l^T
+                                               // (16-byte units into the split-weight array)
     float c, outw, outb;
     double evac;
 };
@@ -182,6 +184,9 @@ struct ann_ctx {
     int64_t binrec_cap = 0;
     int* d_cnt = nullptr;             // [N] neighbour counts
     int* d_flag = nullptr;            // error flags
+    uint4* d_wsplit = nullptr;        // hidden-layer weights as MFMA operand tables (three BF16 terms)
+    int wsplit_n = 0;                 // 16-byte entries
+    bool wsplit_valid = false;        // the tables match the FP32 working copy of the weights
     int* h_rb = nullptr;              // pinned host words written by kernels for the host to read after a sync
     int* d_rb = nullptr;              // the same words as seen from the device
     int64_t keys_cap = 0;
@@ -255,6 +260,9 @@ struct Launch {
     ~Launch();
 };
 hipError_t neighbors_build(ann_ctx* ctx, double rc);
+// rebuild the BF16x3 weight tables of the matrix-core network from the FP32
+// weights; tick, if set, is incremented by one thread (the Adam step counter)
+hipError_t split_weights(ann_ctx* ctx, int* tick);
 // exclusive scan of n ints (out[n-1] = sum of in[0..n-2]); with rb set, also
 // copies nflags words of flags, then out[n-1], to the pinned words rb
 hipError_t exclusive_scan(const int* in, int* out, int n, hipStream_t st, const int* flags = nullptr,

@@ -9,8 +9,8 @@
 //   backward: g_{L-1} = outw W_L^T . sigma'_{L-1}, g_{l-1} = (g_l W_l) . sigma'_{l-1},
 //             dE/dG = (g_0 W_0) . inpw.
 // Activations move between the accumulator layout and the operand layout
-// through LDS; weights are read as FP32 from the working copy and split on
-// the fly.
+// through LDS; the weights come as operand tables split once per parameter
+// update (k_wsplit).
 #include "net_dev.h"
 
 namespace ann {
@@ -36,6 +36,7 @@ struct MfmaArgs {
     const float* G;
     float* Eatom;
     float* dEdG;
+    const uint4* wtab;     // BF16x3 operand tables of the hidden-layer weights
 };
 
 // acc[ct] += X (16 x K, rows = atoms, from LDS) * B (K x 16 columns of tile ct)
@@ -55,11 +56,22 @@ __device__ __forceinline__ void split3(float x, __bf16& a, __bf16& b, __bf16& c)
     c = (__bf16)(r1 - (float)b);
 }
 
-template <int NT, typename FB>
-__device__ __forceinline__ void mm_bf16x3(int K, int ncol, const float (*X)[kMaxK + 1], FB fb, f32x4 (&acc)[NT]) {
+__device__ __forceinline__ bf16x8 as_bf16x8(uint4 u) {
+    bf16x8 v;
+    __builtin_memcpy(&v, &u, sizeof(v));
+    return v;
+}
+
+// The right operands come pre-split from a table (k_wsplit): for k block kk
+// and column tile ct, the three BF16 terms of lane l are the 16-byte entries
+// ((kk * nt + ct) * 3 + p) * 64 + l, p = 0, 1, 2.
+template <int NT>
+__device__ __forceinline__ void mm_bf16x3(int K, int ncol, const float (*X)[kMaxK + 1], const uint4* __restrict__ tab,
+                                          f32x4 (&acc)[NT]) {
     const int lane = threadIdx.x % kWave;
     const int row = lane % 16, kb = 8 * (lane / 16);
-    for (int k0 = 0; k0 < K; k0 += 32) {
+    const int nt = (ncol + 15) / 16;
+    for (int k0 = 0, kk = 0; k0 < K; k0 += 32, ++kk) {
         bf16x8 a0, a1, a2;
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
@@ -70,15 +82,9 @@ __device__ __forceinline__ void mm_bf16x3(int K, int ncol, const float (*X)[kMax
         }
 #pragma unroll
         for (int ct = 0; ct < NT; ++ct) {
-            if (ct * 16 >= ncol) break;
-            bf16x8 b0, b1, b2;
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const int k = k0 + kb + j;
-                __bf16 y0, y1, y2;
-                split3(k < K ? fb(k, ct * 16 + row) : 0.0f, y0, y1, y2);
-                b0[j] = y0; b1[j] = y1; b2[j] = y2;
-            }
+            if (ct >= nt) break;
+            const uint4* e = tab + (size_t)((kk * nt + ct) * 3) * kWave + lane;
+            const bf16x8 b0 = as_bf16x8(e[0]), b1 = as_bf16x8(e[kWave]), b2 = as_bf16x8(e[2 * kWave]);
             f32x4 c = acc[ct];
             c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2, b0, c, 0, 0, 0);
             c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, c, 0, 0, 0);
@@ -89,6 +95,57 @@ __device__ __forceinline__ void mm_bf16x3(int K, int ncol, const float (*X)[kMax
             acc[ct] = c;
         }
     }
+}
+
+// Operand tables of every hidden layer (blockIdx.y = (species, layer,
+// direction)): direction 0 holds W_l as the right operand of the forward
+// product (K = inputs, columns = neurons), direction 1 holds W_l^T for the
+// backward product (K = neurons, columns = inputs).  One thread per 16-byte
+// entry; tick (the Adam step counter) is advanced by one thread.
+__global__ __launch_bounds__(256) void k_wsplit(const DevModel* __restrict__ dm, const float* __restrict__ W,
+                                               uint4* __restrict__ tab, int* tick) {
+    const int seg = blockIdx.y;
+    const int a = seg / (2 * kMaxLayers), l = (seg / 2) % kMaxLayers, dir = seg % 2;
+    if (tick && seg == 0 && blockIdx.x == 0 && threadIdx.x == 0) *tick += 1;
+    if (a >= dm->S) return;
+    const NetDesc& D = dm->net[a];
+    if (l >= D.nlayer - 1) return;
+    const int ni = l == 0 ? D.nin : D.nodes[l - 1], no = D.nodes[l];
+    const int K = dir == 0 ? ni : no, ncol = dir == 0 ? no : ni;
+    const int nt = (ncol + 15) / 16;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (K + 31) / 32 * nt * 3 * kWave) return;
+    const int lane = e % kWave, p = (e / kWave) % 3, ct = (e / (3 * kWave)) % nt, kk = e / (3 * kWave * nt);
+    const int col = ct * 16 + lane % 16, kb = kk * 32 + 8 * (lane / 16);
+    const float* Wl = W + D.woff[l];
+    bf16x8 v;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int k = kb + j;
+        const float x = (k < K && col < ncol) ? (dir == 0 ? Wl[k * no + col] : Wl[col * no + k]) : 0.0f;
+        __bf16 x0, x1, x2;
+        split3(x, x0, x1, x2);
+        v[j] = p == 0 ? x0 : (p == 1 ? x1 : x2);
+    }
+    uint4 u;
+    __builtin_memcpy(&u, &v, sizeof(u));
+    tab[(dir == 0 ? D.wsf[l] : D.wsb[l]) + e] = u;
+}
+
+hipError_t split_weights(ann_ctx* ctx, int* tick) {
+    const DevModel& M = ctx->hm;
+    int most = 1;
+    for (int a = 0; a < M.S; ++a)
+        for (int l = 0; l < M.net[a].nlayer - 1; ++l) {
+            const int ni = l == 0 ? M.net[a].nin : M.net[a].nodes[l - 1], no = M.net[a].nodes[l];
+            most = std::max(most, (ni + 31) / 32 * ((no + 15) / 16) * 3 * kWave);
+            most = std::max(most, (no + 31) / 32 * ((ni + 15) / 16) * 3 * kWave);
+        }
+    dim3 grid((most + 255) / 256, std::max(M.S, 1) * 2 * kMaxLayers);
+    k_wsplit<<<grid, 256, 0, ctx->stream>>>(ctx->dm, ctx->da.W, ctx->d_wsplit, tick);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) ctx->wsplit_valid = true;
+    return e;
 }
 
 // one wavefront per 16-atom single-species tile
@@ -106,24 +163,37 @@ __global__ __launch_bounds__(64) void k_net_mfma(MfmaArgs A) {
     const float* iw = A.inpw + D.inoff;
     const float* ib = A.inpb + D.inoff;
     if (lane < kTile) atoms[lane] = lane < nrow ? A.perm[tl.y + lane] : -1;
-    // scaled input s = inpw (G + inpb) into LDS
-    for (int q = lane; q < kTile * kMaxK; q += kWave) {
-        const int r = q / kMaxK, k = q % kMaxK;
-        const int ai = r < nrow ? A.perm[tl.y + r] : -1;
-        act[r][k] = (ai >= 0 && k < D.nin) ? iw[k] * (A.G[(size_t)ai * A.ldg + k] + ib[k]) : 0.0f;
+    __syncthreads();
+    // scaled input s = inpw (G + inpb) into LDS: lanes over the inputs, rows unrolled
+    {
+        float w2[kMaxK / kWave], b2[kMaxK / kWave];
+#pragma unroll
+        for (int h = 0; h < kMaxK / kWave; ++h) {
+            const int k = lane + h * kWave;
+            w2[h] = k < D.nin ? iw[k] : 0.0f;
+            b2[h] = k < D.nin ? ib[k] : 0.0f;
+        }
+#pragma unroll
+        for (int r = 0; r < kTile; ++r) {
+            const int ai = atoms[r];
+#pragma unroll
+            for (int h = 0; h < kMaxK / kWave; ++h) {
+                const int k = lane + h * kWave;
+                act[r][k] = (ai >= 0 && k < D.nin) ? w2[h] * (A.G[(size_t)ai * A.ldg + k] + b2[h]) : 0.0f;
+            }
+        }
     }
     __syncthreads();
     const int col = lane % 16, rb = 4 * (lane / 16);
     int ni = D.nin;
     for (int l = 0; l < L - 1; ++l) {
         const int no = D.nodes[l];
-        const float* Wl = A.W + D.woff[l];
         const float* bl = A.W + D.boff[l];
         float out[kMaxWidth / 16][4];
         f32x4 accs[kMaxWidth / 16];
 #pragma unroll
         for (int ct = 0; ct < kMaxWidth / 16; ++ct) accs[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
-        mm_bf16x3<kMaxWidth / 16>(ni, no, act, [&](int k, int n) { return n < no ? Wl[k * no + n] : 0.0f; }, accs);
+        mm_bf16x3<kMaxWidth / 16>(ni, no, act, A.wtab + D.wsf[l], accs);
 #pragma unroll
         for (int ct = 0; ct < kMaxWidth / 16; ++ct) {
             if (ct * 16 >= no) break;
@@ -179,12 +249,11 @@ __global__ __launch_bounds__(64) void k_net_mfma(MfmaArgs A) {
     for (int l = L - 2; l >= 0; --l) {
         const int no = D.nodes[l];
         const int nprev = l == 0 ? D.nin : D.nodes[l - 1];
-        const float* Wl = A.W + D.woff[l];
         float out[kMaxK / 16][4];
         f32x4 accs[kMaxK / 16];
 #pragma unroll
         for (int ct = 0; ct < kMaxK / 16; ++ct) accs[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
-        mm_bf16x3<kMaxK / 16>(no, nprev, act, [&](int k, int m) { return m < nprev ? Wl[m * no + k] : 0.0f; }, accs);
+        mm_bf16x3<kMaxK / 16>(no, nprev, act, A.wtab + D.wsb[l], accs);
 #pragma unroll
         for (int ct = 0; ct < kMaxK / 16; ++ct) {
             if (ct * 16 >= nprev) break;
@@ -243,7 +312,9 @@ hipError_t net_fwd_mfma(ann_ctx* ctx, const float* G, float* Eatom, float* dEdG)
     A.G = G;
     A.Eatom = Eatom;
     A.dEdG = dEdG;
+    A.wtab = ctx->d_wsplit;
     if (A.ntiles == 0) return hipSuccess;
+    if (!ctx->wsplit_valid && (e = split_weights(ctx, nullptr)) != hipSuccess) return e;
     Launch L(ctx, "net_mfma");
     k_net_mfma<<<A.ntiles, 64, 0, ctx->stream>>>(A);
     return hipGetLastError();

@@ -739,7 +739,8 @@ __global__ void k_loss_total(int B, const double* lossE, const double* lossF, do
 }
 
 // step t = *tdev + 1 (the counter lives on the device so that a captured
-// training step advances it on every replay; k_adam_tick increments it)
+// training step advances it on every replay; the weight-split launch after the
+// update increments it)
 __global__ void k_adam(int n, double* __restrict__ p, double* __restrict__ m1, double* __restrict__ m2,
                        const double* __restrict__ g, double gamma, const int* __restrict__ tdev, double eps,
                        float* __restrict__ W) {
@@ -758,7 +759,6 @@ __global__ void k_adam(int n, double* __restrict__ p, double* __restrict__ m1, d
     W[i] = (float)np;
 }
 
-__global__ void k_adam_tick(int* __restrict__ tdev) { *tdev += 1; }
 
 hipError_t train_step_impl(ann_ctx* ctx, int form, const double* Eref, const double* Fref, double wE, double wF,
                            double gamma, double* loss, double* grad) {
@@ -796,8 +796,10 @@ hipError_t train_step_impl(ann_ctx* ctx, int form, const double* Eref, const dou
         k_adam<<<(ctx->nparams + 255) / 256, 256, 0, ctx->stream>>>(ctx->nparams, ctx->da.params, ctx->da.adam_m,
                                                                     ctx->da.adam_v, g, gamma, ctx->d_adam_t, 1e-16,
                                                                     ctx->da.W);
-        k_adam_tick<<<1, 1, 0, ctx->stream>>>(ctx->d_adam_t);
         if ((e = hipGetLastError()) != hipSuccess) return e;
+        // new FP32 weights: refresh the matrix-core operand tables; the same
+        // launch advances the Adam step counter
+        if ((e = split_weights(ctx, ctx->d_adam_t)) != hipSuccess) return e;
     }
     return hipSuccess;
 }
