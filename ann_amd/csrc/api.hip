@@ -312,8 +312,9 @@ static int upload_model(ann_ctx* ctx, const ann_model_desc* d) {
     ANN_HIP(hipMemset(ctx->da.adam_v, 0, sizeof(double) * ptot));
     ANN_HIP(hipMalloc(&ctx->d_flag, sizeof(int) * 4));
     ANN_HIP(hipMemset(ctx->d_flag, 0, sizeof(int) * 4));
-    ANN_HIP(hipHostMalloc((void**)&ctx->h_rb, sizeof(int) * kReadbackWords, hipHostMallocDefault));
+    ANN_HIP(hipHostMalloc((void**)&ctx->h_rb, sizeof(int) * kReadbackWords, hipHostMallocCoherent));
     ANN_HIP(hipHostGetDevicePointer((void**)&ctx->d_rb, ctx->h_rb, 0));
+    for (int q = 0; q < kReadbackWords; ++q) ctx->h_rb[q] = 0;
     ANN_HIP(hipMalloc(&ctx->d_adam_t, sizeof(int)));
     ANN_HIP(hipMemset(ctx->d_adam_t, 0, sizeof(int)));
     ANN_HIP(split_weights(ctx, nullptr));
@@ -482,6 +483,7 @@ extern "C" int ann_neighbors(ann_ctx* ctx, double rc) {
     const double r = rc > 0 ? rc : (double)ctx->hm.rc;
     const double rmodel = rc > 0 ? rc : 0;
     (void)rmodel;
+    ctx->have_nlist = false;      // the build rewrites the list arrays in place
     hipError_t e = neighbors_build(ctx, rc > 0 ? rc : ctx->hm.rc);
     if (e == hipErrorInvalidValue) return ANN_ERR_CAPACITY;
     if (e != hipSuccess) return fail(ANN_ERR_HIP, std::string("neighbour build: ") + hipGetErrorString(e));

@@ -188,6 +188,7 @@ struct ann_ctx {
     int wsplit_n = 0;                 // 16-byte entries
     bool wsplit_valid = false;        // the tables match the FP32 working copy of the weights
     int* h_rb = nullptr;              // pinned host words written by kernels for the host to read after a sync
+    int rb_seq = 0;                   // completion marker of the last neighbour-build readback
     int* d_rb = nullptr;              // the same words as seen from the device
     int64_t keys_cap = 0;
     // work buffers
@@ -264,9 +265,10 @@ hipError_t neighbors_build(ann_ctx* ctx, double rc);
 // weights; tick, if set, is incremented by one thread (the Adam step counter)
 hipError_t split_weights(ann_ctx* ctx, int* tick);
 // exclusive scan of n ints (out[n-1] = sum of in[0..n-2]); with rb set, also
-// copies nflags words of flags, then out[n-1], to the pinned words rb
+// copies nflags words of flags, then out[n-1], then seq (a completion marker)
+// to the pinned words rb
 hipError_t exclusive_scan(const int* in, int* out, int n, hipStream_t st, const int* flags = nullptr,
-                          int nflags = 0, int* rb = nullptr);
+                          int nflags = 0, int* rb = nullptr, int seq = 0);
 // copy n device ints to the pinned words rb (stream ordered; read after a sync)
 hipError_t readback(const int* src, int n, int* rb, hipStream_t st);
 hipError_t symm_fwd(ann_ctx* ctx, int form, float* G);

@@ -16,6 +16,8 @@
 // oracle, compacts accepted keys (j << 30 | packed n) through a ballot into
 // LDS and sorts them with a bitonic network.  Integer work; the roofline is
 // HBM/latency (positions in, 16 B + 13 B per pair out).
+#include <atomic>
+
 #include "common.h"
 
 #include <hipcub/hipcub.hpp>
@@ -69,11 +71,11 @@ __global__ void k_struc_of(const int* __restrict__ off, int nstruc, int* __restr
     for (int i = off[s] + threadIdx.x; i < off[s + 1]; i += blockDim.x) sof[i] = s;
 }
 
-__global__ void k_bin(int N, const double* __restrict__ pos, const int* __restrict__ sof,
-                      const CellInfo* __restrict__ ci, const double* __restrict__ rinv,
-                      int* __restrict__ bin_of, int* __restrict__ shift, int* __restrict__ bin_cnt) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N) return;
+// bin of atom i from its fractional coordinates; writes the atom's integer
+// shift (the lattice vector that wraps it into the cell)
+__device__ __forceinline__ int bin_atom(int i, const double* __restrict__ pos, const int* __restrict__ sof,
+                                        const CellInfo* __restrict__ ci, const double* __restrict__ rinv,
+                                        int* __restrict__ shift) {
     const int s = sof[i];
     const CellInfo c = ci[s];
     const double* Ri = rinv + 9 * s;
@@ -85,7 +87,20 @@ __global__ void k_bin(int N, const double* __restrict__ pos, const int* __restri
         int bb = (int)floor((f - fl) * c.nb[a]);
         b[a] = bb < 0 ? 0 : (bb >= c.nb[a] ? c.nb[a] - 1 : bb);
     }
-    const int g = c.boff + (b[0] * c.nb[1] + b[1]) * c.nb[2] + b[2];
+    return c.boff + (b[0] * c.nb[1] + b[1]) * c.nb[2] + b[2];
+}
+
+__device__ __forceinline__ double4 bin_record(int i, const double* __restrict__ pos, const int* __restrict__ shift) {
+    const uint64_t js = ((uint64_t)(uint32_t)i << 32) | pack_img(shift[3 * i], shift[3 * i + 1], shift[3 * i + 2]);
+    return make_double4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], __longlong_as_double((long long)js));
+}
+
+__global__ void k_bin(int N, const double* __restrict__ pos, const int* __restrict__ sof,
+                      const CellInfo* __restrict__ ci, const double* __restrict__ rinv,
+                      int* __restrict__ bin_of, int* __restrict__ shift, int* __restrict__ bin_cnt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    const int g = bin_atom(i, pos, sof, ci, rinv, shift);
     bin_of[i] = g;
     atomicAdd(&bin_cnt[g], 1);
 }
@@ -102,8 +117,7 @@ __global__ void k_fill_bins(int N, const int* __restrict__ bin_of, const int* __
     const int p = atomicAdd(&bin_fill[g], 1);
     const int slot = bin_start[g] + p;
     bin_atoms[slot] = i;
-    const uint64_t js = ((uint64_t)(uint32_t)i << 32) | pack_img(shift[3 * i], shift[3 * i + 1], shift[3 * i + 2]);
-    rec[slot] = make_double4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], __longlong_as_double((long long)js));
+    rec[slot] = bin_record(i, pos, shift);
 }
 
 __device__ __forceinline__ int floordiv(int a, int b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
@@ -478,9 +492,9 @@ __global__ void k_compact(int N, const double* __restrict__ pos, const int* __re
                           const double* __restrict__ cells, const int* __restrict__ species,
                           const uint64_t* __restrict__ keys, const int* __restrict__ off,
                           int* __restrict__ idx, uint32_t* __restrict__ img, int* __restrict__ rev,
-                          float4* __restrict__ geo, uint8_t* __restrict__ nsp) {
+                          float4* __restrict__ geo, uint8_t* __restrict__ nsp, int cap) {
     const int i = xcd_block(blockIdx.x, gridDim.x);
-    if (i >= N) return;
+    if (i >= N || off[N] > cap) return;     // a list that does not fit is compacted again after growing
     const int o = off[i], n = off[i + 1] - o;
     const double* R = cells + 9 * sof[i];
     for (int t = threadIdx.x; t < n; t += blockDim.x) {
@@ -508,9 +522,9 @@ __global__ void k_compact(int N, const double* __restrict__ pos, const int* __re
 // so a slot without a partner (an asymmetric list, impossible for the
 // symmetric FP64 test) stays in range; the missing upper match is flagged.
 __global__ void k_rev(int N, const int* __restrict__ idx, const uint32_t* __restrict__ img,
-                      const int* __restrict__ off, int* __restrict__ rev, int* __restrict__ flag) {
+                      const int* __restrict__ off, int* __restrict__ rev, int* __restrict__ flag, int cap) {
     const int i = xcd_block(blockIdx.x, gridDim.x);
-    if (i >= N) return;
+    if (i >= N || off[N] > cap) return;
     const int o = off[i], n = off[i + 1] - o;
     for (int t = threadIdx.x; t < n; t += blockDim.x) {
         const int j = idx[o + t];
@@ -573,7 +587,8 @@ __device__ __forceinline__ void store_host(int* rb, int q, int v) {
 }
 
 __global__ __launch_bounds__(kScanThreads) void k_scan_small(const int* __restrict__ in, int* __restrict__ out, int n,
-                                                             const int* __restrict__ flags, int nflags, int* rb) {
+                                                             const int* __restrict__ flags, int nflags, int* rb,
+                                                             int seq) {
     __shared__ int wsum[kScanThreads / kWave];
     __shared__ int s_carry;
     const int t = threadIdx.x, lane = t % kWave, w = t / kWave;
@@ -621,23 +636,31 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_small(const int* __restri
         for (int q = 0; q < nflags; ++q) store_host(rb, q, flags[q]);
         store_host(rb, nflags, n > 0 ? s_carry - in[n - 1] : 0);
         __threadfence_system();
+        store_host(rb, nflags + 1, seq);            // the words above are complete
+        __threadfence_system();
     }
 }
 
-__global__ void k_readback(const int* __restrict__ src, int n, int* rb) {
+__global__ void k_readback(const int* __restrict__ src, int n, int* rb, int seq) {
     const int t = threadIdx.x;
     if (t < n) store_host(rb, t, src[t]);
     __threadfence_system();
+    __syncthreads();
+    if (seq && t == 0) {
+        store_host(rb, n, seq);
+        __threadfence_system();
+    }
 }
 
 hipError_t readback(const int* src, int n, int* rb, hipStream_t st) {
-    k_readback<<<1, 64, 0, st>>>(src, n, rb);
+    k_readback<<<1, 64, 0, st>>>(src, n, rb, 0);
     return hipGetLastError();
 }
 
-hipError_t exclusive_scan(const int* in, int* out, int n, hipStream_t st, const int* flags, int nflags, int* rb) {
+hipError_t exclusive_scan(const int* in, int* out, int n, hipStream_t st, const int* flags, int nflags, int* rb,
+                          int seq) {
     if (n <= kScanMax) {
-        k_scan_small<<<1, kScanThreads, 0, st>>>(in, out, n, flags, nflags, rb);
+        k_scan_small<<<1, kScanThreads, 0, st>>>(in, out, n, flags, nflags, rb, seq);
         return hipGetLastError();
     }
     size_t bytes = 0;
@@ -654,7 +677,8 @@ hipError_t exclusive_scan(const int* in, int* out, int n, hipStream_t st, const 
     if ((e = hipcub::DeviceScan::ExclusiveSum(tmp, bytes, in, out, n, st)) != hipSuccess) return e;
     if (rb) {
         if (nflags > 0 && (e = readback(flags, nflags, rb, st)) != hipSuccess) return e;
-        e = readback(out + n - 1, 1, rb + nflags, st);
+        k_readback<<<1, 64, 0, st>>>(out + n - 1, 1, rb + nflags, seq);   // total, then the marker
+        e = hipGetLastError();
     }
     return e;
 }
@@ -725,6 +749,38 @@ static hipError_t grid_setup(ann_ctx* ctx, double rc) {
     return hipSuccess;
 }
 
+// Wait until the pinned word slot holds seq (written last by the readback of a
+// kernel on the stream), polling instead of synchronising the stream so that
+// work queued behind that kernel keeps the device busy meanwhile.
+static hipError_t wait_marker(ann_ctx* ctx, int slot, int seq) {
+    volatile const int* w = ctx->h_rb + slot;
+    for (long it = 0;; ++it) {
+        if (*w == seq) break;
+        if ((it & 255) == 255) {
+            const hipError_t e = hipStreamQuery(ctx->stream);
+            if (e == hipSuccess) {                   // stream drained: the marker must be there
+                if (*w == seq) break;
+                return hipErrorUnknown;
+            }
+            if (e != hipErrorNotReady) return e;
+        }
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    return hipSuccess;
+}
+
+static void launch_compact(ann_ctx* ctx, int cap) {
+    const int N = ctx->natoms;
+    hipStream_t st = ctx->stream;
+    NList& nl = ctx->nl;
+    Launch L(ctx, "nb_compact");
+    k_compact<<<N, 128, 0, st>>>(N, ctx->d_pos, ctx->d_struc_of, ctx->d_cells, ctx->d_species, ctx->d_keys, nl.off,
+                                 nl.idx, nl.img, nl.rev, nl.geo, nl.nsp, cap);
+    // a missing reverse slot (impossible for the symmetric FP64 test) is
+    // reported by the next synchronising call (ann_nlist_size, next build)
+    k_rev<<<N, 128, 0, st>>>(N, nl.idx, nl.img, nl.off, nl.rev, ctx->d_flag + 2, cap);
+}
+
 hipError_t neighbors_build(ann_ctx* ctx, double rc) {
     const int N = ctx->natoms;
     hipStream_t st = ctx->stream;
@@ -737,20 +793,20 @@ hipError_t neighbors_build(ann_ctx* ctx, double rc) {
     HIPCHK(grow(ctx->d_binrec, rcap, (int64_t)N));
     ctx->binrec_cap = rcap;
     const CellInfo* dci = reinterpret_cast<const CellInfo*>(ctx->d_cellinfo);
-    int* bin_fill = ctx->d_bin_cnt + (nbins + 1);
     {
         // bin counters, fill cursors, overflow marks and flag[0..1] (flag[2]: sticky reverse-slot error)
+        int* bin_fill = ctx->d_bin_cnt + (nbins + 1);
         const int nc = 3 * ctx->bin_cap;
         k_clear2<<<(nc + 2 + 255) / 256, 256, 0, st>>>(ctx->d_bin_cnt, nc, ctx->d_flag, 2);
+        {
+            Launch L(ctx, "nb_bin");
+            k_bin<<<(N + 255) / 256, 256, 0, st>>>(N, ctx->d_pos, ctx->d_struc_of, dci, ctx->d_cellmat,
+                                                  ctx->d_bin_of, ctx->d_shift, ctx->d_bin_cnt);
+        }
+        HIPCHK(exclusive_scan(ctx->d_bin_cnt, ctx->d_bin_start, nbins + 1, st));
+        k_fill_bins<<<(N + 255) / 256, 256, 0, st>>>(N, ctx->d_bin_of, ctx->d_bin_start, bin_fill,
+                                                    ctx->d_bin_atoms, ctx->d_pos, ctx->d_shift, ctx->d_binrec);
     }
-    {
-        Launch L(ctx, "nb_bin");
-        k_bin<<<(N + 255) / 256, 256, 0, st>>>(N, ctx->d_pos, ctx->d_struc_of, dci, ctx->d_cellmat,
-                                              ctx->d_bin_of, ctx->d_shift, ctx->d_bin_cnt);
-    }
-    HIPCHK(exclusive_scan(ctx->d_bin_cnt, ctx->d_bin_start, nbins + 1, st));
-    k_fill_bins<<<(N + 255) / 256, 256, 0, st>>>(N, ctx->d_bin_of, ctx->d_bin_start, bin_fill,
-                                                ctx->d_bin_atoms, ctx->d_pos, ctx->d_shift, ctx->d_binrec);
     {
         Launch L(ctx, "nb_search");
         k_search_bins<<<ctx->grid_nsb, kStageThreads, 0, st>>>(dci, ctx->d_cells, ctx->d_bin_struc, ctx->d_bin_start,
@@ -758,9 +814,17 @@ hipError_t neighbors_build(ann_ctx* ctx, double rc) {
                                                       ctx->d_bin_ovf);
     }
     HIPCHK(hipGetLastError());
-    // offsets, then flags and pair count to pinned host words: the build's one synchronisation
-    HIPCHK(exclusive_scan(ctx->d_cnt, ctx->nl.off, N + 1, st, ctx->d_flag, 4, ctx->d_rb));
-    HIPCHK(hipStreamSynchronize(st));
+    NList& nl = ctx->nl;
+    // Offsets, then flags, pair count and a completion marker to pinned host
+    // words -- the build's one wait.  While the host waits, the compaction runs
+    // speculatively into the current list arrays; it does nothing if the list
+    // does not fit, and is repeated after growing them.
+    int seq = ++ctx->rb_seq;
+    HIPCHK(exclusive_scan(ctx->d_cnt, ctx->nl.off, N + 1, st, ctx->d_flag, 4, ctx->d_rb, seq));
+    const int spec_cap = (int)std::min<int64_t>(nl.cap, INT32_MAX);
+    if (spec_cap > 0) launch_compact(ctx, spec_cap);
+    HIPCHK(hipGetLastError());
+    HIPCHK(wait_marker(ctx, 5, seq));
     volatile const int* rb = ctx->h_rb;
     int flags[4] = {rb[0], rb[1], rb[2], rb[3]};
     int total = rb[4];
@@ -774,8 +838,11 @@ hipError_t neighbors_build(ann_ctx* ctx, double rc) {
                                        ctx->d_cnt, ctx->d_flag, ctx->d_bin_ovf);
         }
         HIPCHK(hipGetLastError());
-        HIPCHK(exclusive_scan(ctx->d_cnt, ctx->nl.off, N + 1, st, ctx->d_flag, 4, ctx->d_rb));
-        HIPCHK(hipStreamSynchronize(st));
+        seq = ++ctx->rb_seq;
+        HIPCHK(exclusive_scan(ctx->d_cnt, ctx->nl.off, N + 1, st, ctx->d_flag, 4, ctx->d_rb, seq));
+        if (spec_cap > 0) launch_compact(ctx, spec_cap);
+        HIPCHK(hipGetLastError());
+        HIPCHK(wait_marker(ctx, 5, seq));
         for (int q = 0; q < 4; ++q) flags[q] = rb[q];
         total = rb[4];
     }
@@ -784,7 +851,10 @@ hipError_t neighbors_build(ann_ctx* ctx, double rc) {
         set_error("an atom has " + std::to_string(flags[1]) + " neighbours; capacity is " + std::to_string(kMaxNeigh));
         return hipErrorInvalidValue;
     }
-    NList& nl = ctx->nl;
+    nl.npairs = total;
+    nl.maxn = flags[1];
+    if (spec_cap > 0 && total <= spec_cap) return hipSuccess;     // the speculative compaction stands
+    if (spec_cap > 0) HIPCHK(hipStreamSynchronize(st));            // its kernels still hold the old arrays
     int64_t c1 = nl.cap, c2 = nl.cap, c3 = nl.cap, c4 = nl.cap, c5 = nl.cap;
     HIPCHK(grow(nl.idx, c1, total));
     HIPCHK(grow(nl.img, c2, total));
@@ -792,16 +862,7 @@ hipError_t neighbors_build(ann_ctx* ctx, double rc) {
     HIPCHK(grow(nl.geo, c4, total));
     HIPCHK(grow(nl.nsp, c5, total));
     nl.cap = std::min(std::min(c1, c2), std::min(std::min(c3, c4), c5));
-    nl.npairs = total;
-    nl.maxn = flags[1];
-    {
-        Launch L(ctx, "nb_compact");
-        k_compact<<<N, 128, 0, st>>>(N, ctx->d_pos, ctx->d_struc_of, ctx->d_cells, ctx->d_species, ctx->d_keys,
-                                     nl.off, nl.idx, nl.img, nl.rev, nl.geo, nl.nsp);
-        // a missing reverse slot (impossible for the symmetric FP64 test) is
-        // reported by the next synchronising call (ann_nlist_size, next build)
-        k_rev<<<N, 128, 0, st>>>(N, nl.idx, nl.img, nl.off, nl.rev, ctx->d_flag + 2);
-    }
+    launch_compact(ctx, (int)std::min<int64_t>(nl.cap, INT32_MAX));
     return hipGetLastError();
 }
 
