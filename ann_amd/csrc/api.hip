@@ -349,7 +349,7 @@ extern "C" int ann_destroy(ann_ctx* ctx) {
                     ctx->d_Eatom, ctx->d_P, ctx->d_u, ctx->d_v, ctx->d_F, ctx->d_E, ctx->d_gE, ctx->d_atomvir,
                     ctx->d_scr, ctx->d_gpart, ctx->d_rank, ctx->d_spcount, ctx->d_perm, ctx->d_tiles, ctx->d_spend, ctx->d_grad_tiles, ctx->d_gsplit,
                     ctx->d_chunks, ctx->d_chunk_off, ctx->d_part, ctx->d_pos_ref, ctx->d_keatom, ctx->d_md_flag,
-                    ctx->d_bin_struc, ctx->d_binrec, ctx->d_adam_t, ctx->d_wsplit};
+                    ctx->d_bin_struc, ctx->d_binrec, ctx->d_adam_t, ctx->d_wsplit, ctx->d_T};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (ctx->h_rb) hipHostFree(ctx->h_rb);
@@ -611,6 +611,11 @@ static hipError_t ensure_work(ann_ctx* ctx) {
         if ((e = dalloc(ctx->d_u, 3 * (size_t)N)) != hipSuccess) return e;
         if ((e = dalloc(ctx->d_F, 3 * (size_t)N)) != hipSuccess) return e;
         if ((e = dalloc(ctx->d_atomvir, 9 * (size_t)N)) != hipSuccess) return e;
+        if (ctx->hm.moment_z > 0) {   // moments of the moment form: [N][S][4][monomials]
+            const int z = ctx->hm.moment_z, nm = (z + 1) * (z + 2) * (z + 3) / 6;
+            if ((e = dalloc(ctx->d_T, (size_t)N * ctx->S * kMaxUnionEta * nm)) != hipSuccess) return e;
+        }
+        ctx->T_gen = ~0ull;
         ctx->work_atoms = N;
     }
     if (P > ctx->work_pairs || !ctx->d_P) {

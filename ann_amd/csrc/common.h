@@ -218,6 +218,8 @@ struct ann_ctx {
     hipEvent_t gev[2] = {nullptr, nullptr};
     hipGraphExec_t gexec = nullptr;
     uint64_t gen = 0;                 // bumped whenever positions, the list or the batch change
+    float* d_T = nullptr;             // moment form: per-atom moments T of the last G evaluation
+    uint64_t T_gen = ~0ull;           // gen at which d_T was written (~0: never)
     uint64_t gkey_gen = 0;
     int gkey_form = -1;
     const void* gkey_ptr[4] = {nullptr, nullptr, nullptr, nullptr};

@@ -53,6 +53,8 @@ struct SymmArgs {
     float* P;
     const float* u;
     float* v;
+    float* Tout;              // moment form, G: store each atom's moments T here (or null)
+    const float* Tin;         // moment form, adjoint: moments of the same geometry (or null: recompute)
 };
 
 // Sum N per-lane values over the 64 lanes with a transposing butterfly: at
@@ -799,8 +801,11 @@ __global__ __launch_bounds__(256) void k_moment(SymmArgs A) {
     sm.extra = reinterpret_cast<float*>(sm.seg + 16);
     float* rowbuf = sm.extra;                                   // [ldg]
     float* sP = rowbuf + ldg;                                   // [maxn][3] (mode 1)
+    // adjoint with stored moments: no power rows (the cutoff pairs still use their space)
+    const bool haveT = MODE == 1 && A.Tin != nullptr;
+    const int TS = S * U4 * NM;                                 // moments per atom
     float* pw = sP + (MODE == 1 ? 3 * maxn : 0);                // [maxn][PWS]
-    float* T = pw + (size_t)maxn * PWS;                         // [S][U4][NM]
+    float* T = pw + (size_t)maxn * (haveT ? 2 : PWS);           // [S][U4][NM]
     float* dT = T + S * U4 * NM;                                // [S][U4][NM] (mode 2)
     float* D = dT + (MODE == 2 ? S * U4 * NM : 0);              // [S][U4]
     float* dD = D + S * U4;                                     // [S][U4] (mode 2: dD, mode 1: dE/dD)
@@ -831,9 +836,12 @@ __global__ __launch_bounds__(256) void k_moment(SymmArgs A) {
     }
     fast_radial<MODE>(A, Mo, a, sm, tv, rowbuf, sP, rowbuf);
     wave_sync();
+    if (haveT) {
+        for (int q = lane; q < TS; q += kWave) T[q] = A.Tin[(size_t)i * TS + q];
+    }
     // powers of the unit vector (and, for the tangent, their derivatives along
     // the projected direction (t - r^ (r^.t)) / r)
-    for (int J = lane; J < n; J += kWave) {
+    for (int J = lane; J < (haveT ? 0 : n); J += kWave) {
         const float4 gq = sm.g[J];
         float* row = pw + (size_t)J * PWS;
         const float v3[3] = {gq.x, gq.y, gq.z};
@@ -856,7 +864,7 @@ __global__ __launch_bounds__(256) void k_moment(SymmArgs A) {
     }
     wave_sync();
     // moments T_b[u][alpha]: lane = (alpha, group); groups stride the neighbours
-    {
+    if (!haveT) {
         const int al = lane % NM, grp = lane / NM;
         const bool act = grp < NG;
         const int ea = c_mono.a[al], eb = c_mono.b[al], ec = c_mono.c[al];
@@ -953,6 +961,9 @@ __global__ __launch_bounds__(256) void k_moment(SymmArgs A) {
         }
     }
     wave_sync();
+    if (MODE == 0 && A.Tout) {
+        for (int q = lane; q < TS; q += kWave) A.Tout[(size_t)i * TS + q] = T[q];
+    }
     // block contractions (modes 0, 2) or their adjoint seeds (mode 1)
     if (MODE == 1) {
         for (int p = 0; p < NP; ++p) {
@@ -1070,7 +1081,7 @@ __global__ __launch_bounds__(256) void k_moment(SymmArgs A) {
 
 static size_t moment_smem(const SymmArgs& A, int mode, int Z, int S, int NP) {
     const int NM = nmono(Z), ZP = Z + 1, U4 = kMaxUnionEta;
-    const int PWS = (3 * ZP * (mode == 2 ? 2 : 1)) | 1;
+    const int PWS = (mode == 1 && A.Tin) ? 2 : (3 * ZP * (mode == 2 ? 2 : 1)) | 1;
     const int NG = kWave / NM;
     size_t b = 16 * (size_t)A.maxn * (2 + (mode == 0 ? 0 : 1) + (mode == 2 ? 1 : 0));
     b += 4 * 2 * (size_t)A.maxn + 4 * 16;
@@ -1130,6 +1141,8 @@ static hipError_t run(ann_ctx* ctx, int form, SymmArgs A) {
                                {"symm_fwd_mom", "symm_bwd_mom", "symm_tan_mom"}};
     Launch L(ctx, names[moment ? 2 : (fast ? 1 : 0)][MODE]);
     if (moment) {
+        if (MODE == 0) A.Tout = ctx->d_T;
+        if (MODE == 1 && ctx->d_T && ctx->T_gen == ctx->gen) A.Tin = ctx->d_T;
         const size_t sw = (moment_smem(A, MODE, M.moment_z, M.S, M.NP) + 15) / 16 * 16;
         int apb = ctx->tune_apb[MODE];
         while (apb > 1 && sw * apb > 65536) apb /= 2;          // dynamic LDS of one workgroup
@@ -1142,6 +1155,7 @@ static hipError_t run(ann_ctx* ctx, int form, SymmArgs A) {
             case 3: k_moment<MODE, 3><<<grid, block, sm, ctx->stream>>>(A); break;
             default: k_moment<MODE, 4><<<grid, block, sm, ctx->stream>>>(A); break;
         }
+        if (MODE == 0 && A.Tout) ctx->T_gen = ctx->gen;
         return hipGetLastError();
     }
     if (fast) {

@@ -47,18 +47,13 @@ def cases():
     return _cases()
 
 
-@pytest.mark.parametrize("name", CASES)
-def test_neighbor_list_bit_exact(cases, oracle_mod, name):
-    model, batch = cases[name]
-    print(f"case {name}", flush=True)
-    eng = _engine(model, batch)
+def _check_nlist(eng, batch, orc, rc=None):
     nl = eng.nlist()
-    orc = oracle_mod.Oracle(model)
     off = batch.atom_off
     base = 0
     for k in range(batch.nstruc):
         pos, cell, sp = batch.struc(k)
-        ref = orc.nlist(pos, cell, sp)
+        ref = orc.nlist(pos, cell, sp, rc)
         a0, a1 = off[k], off[k + 1]
         s0, s1 = nl["off"][a0], nl["off"][a1]
         assert np.array_equal(nl["off"][a0:a1 + 1] - s0, ref.off)
@@ -70,6 +65,29 @@ def test_neighbor_list_bit_exact(cases, oracle_mod, name):
         assert np.all(np.abs(geo[:, 3] - ref.d) <= 2 * np.spacing(ref.d.astype(np.float32)))
         base += s1 - s0
     assert base == len(nl["idx"])
+
+
+@pytest.mark.parametrize("name", CASES)
+def test_neighbor_list_bit_exact(cases, oracle_mod, name):
+    model, batch = cases[name]
+    print(f"case {name}", flush=True)
+    eng = _engine(model, batch)
+    _check_nlist(eng, batch, oracle_mod.Oracle(model))
+
+
+def test_rebuilds_with_growing_and_shrinking_lists(cases, oracle_mod):
+    """A short cutoff, then the model cutoff (the list outgrows its arrays: the
+    compaction that ran behind the scan is void and is repeated after growing),
+    then the short cutoff again (the compaction behind the scan stands)."""
+    model, batch = cases["water4"]
+    from ann_amd.gpu import Engine
+    eng = Engine(model, "cuda:0")
+    eng.set_batch(batch)
+    orc = oracle_mod.Oracle(model)
+    for rc in (3.0, model.rc, 3.0, model.rc):
+        print(f"rc {rc}", flush=True)
+        eng.neighbors(rc)
+        _check_nlist(eng, batch, orc, rc)
 
 
 @pytest.mark.parametrize("form", FORMS)
