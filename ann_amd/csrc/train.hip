@@ -49,6 +49,8 @@ struct Bwd2Args {
     const double* gE;
     float* scr;
     int64_t base[kMaxSpecies];   // scratch start of each species
+    int kp, wp, nlh;             // matrix-core kernel: LDS row strides of the activations and of sigma', sigma'';
+                                 // most hidden layers of any species
 };
 
 // 16 lanes per atom, 4 atoms per wavefront; lane q owns neurons q, q+16, ...
@@ -208,11 +210,18 @@ __global__ __launch_bounds__(64) void k_net_bwd2(Bwd2Args A) {
 // and receives D rows 4 (l / 16) .. +3 of column l % 16.
 constexpr int kB2Tile = 16;
 constexpr int kB2K = 128;      // widest input supported (same bound as the forward MFMA network)
+constexpr int kB2Chunk = 8;    // k steps (of 4) per weight chunk held in registers
 
 __global__ __launch_bounds__(64) void k_bwd2_mfma(Bwd2Args A, const int* __restrict__ perm,
                                                   const int2* __restrict__ tiles, const int* __restrict__ spend) {
-    __shared__ float sx[kB2Tile][kB2K + 1], sxd[kB2Tile][kB2K + 1];
-    __shared__ float sdz[kMaxLayers - 1][kB2Tile][kMaxWidth + 1], sqz[kMaxLayers - 1][kB2Tile][kMaxWidth + 1];
+    // dynamic LDS sized by the model: activations / gradients [16][kp] (x and its
+    // tangent) and sigma', sigma'' zd of every hidden layer [L-1][16][wp]
+    extern __shared__ float b2smem[];
+    const int KP = A.kp, WP = A.wp;
+    float* sx = b2smem;
+    float* sxd = sx + kB2Tile * KP;
+    float* sdz = sxd + kB2Tile * KP;
+    float* sqz = sdz + A.nlh * kB2Tile * WP;
     __shared__ int64_t srow[kB2Tile];
     __shared__ float sgE[kB2Tile];
     const int lane = threadIdx.x;
@@ -236,24 +245,24 @@ __global__ __launch_bounds__(64) void k_bwd2_mfma(Bwd2Args A, const int* __restr
     }
     // layer-0 input x = inpw (G + inpb), xd = inpw v
 #pragma unroll 8
-    for (int q = lane; q < kB2Tile * kB2K; q += kWave) {
-        const int r = q / kB2K, k = q % kB2K;
+    for (int q = lane; q < kB2Tile * KP; q += kWave) {
+        const int r = q / KP, k = q % KP;
         const int ai = r < nrow ? perm[tl.y + r] : -1;
         float x = 0.0f, xd = 0.0f;
         if (ai >= 0 && k < D.nin) {
             x = iw[k] * (A.G[(size_t)ai * A.ldg + k] + ib[k]);
             xd = A.v ? iw[k] * A.v[(size_t)ai * A.ldg + k] : 0.0f;
         }
-        sx[r][k] = x;
-        sxd[r][k] = xd;
+        sx[r * KP + k] = x;
+        sxd[r * KP + k] = xd;
     }
     __syncthreads();
     for (int q = lane; q < kB2Tile * D.nin; q += kWave) {
         const int r = q / D.nin, k = q % D.nin;
         if (srow[r] >= 0) {
             float* row = A.scr + srow[r];
-            row[k] = sx[r][k];
-            row[D.nin + k] = sxd[r][k];
+            row[k] = sx[r * KP + k];
+            row[D.nin + k] = sxd[r * KP + k];
         }
     }
     const int c16 = lane % 16, k4 = lane / 16, rb = 4 * k4;
@@ -269,30 +278,45 @@ __global__ __launch_bounds__(64) void k_bwd2_mfma(Bwd2Args A, const int* __restr
         f32x4 AZ[NT], AD[NT];
 #pragma unroll
         for (int ct = 0; ct < NT; ++ct) { AZ[ct] = f32x4{0.f, 0.f, 0.f, 0.f}; AD[ct] = f32x4{0.f, 0.f, 0.f, 0.f}; }
-        // this lane's weight operands of the whole layer, loaded up front so that
-        // one memory latency covers the layer (a wavefront per tile leaves too
-        // few wavefronts per SIMD to hide a latency per k step)
-        float wr[kB2K / 4][NT];
+        // this lane's weight operands in chunks of kB2Chunk k steps, the next
+        // chunk's loads issued before the current chunk's products (bounded
+        // registers, so two wavefronts fit on a SIMD)
+        auto wload = [&](float (&w)[kB2Chunk][NT], int c0) {
 #pragma unroll
-        for (int st = 0; st < kB2K / 4; ++st) {
-            const int k = 4 * st + k4;
+            for (int st = 0; st < kB2Chunk; ++st) {
+                const int k = 4 * (c0 + st) + k4;
 #pragma unroll
-            for (int ct = 0; ct < NT; ++ct) {
-                const int n = ct * 16 + c16;
-                wr[st][ct] = (4 * st < ni && k < ni && n < no) ? Wl[k * no + n] : 0.0f;
+                for (int ct = 0; ct < NT; ++ct) {
+                    const int n = ct * 16 + c16;
+                    w[st][ct] = (k < ni && n < no) ? Wl[k * no + n] : 0.0f;
+                }
             }
-        }
+        };
+        auto wmul = [&](const float (&w)[kB2Chunk][NT], int c0) {
 #pragma unroll
-        for (int st = 0; st < kB2K / 4; ++st) {
-            if (4 * st >= ni) break;
-            const int k = 4 * st + k4;
-            const bool ok = k < ni;
-            const float xa = ok ? sx[c16][k] : 0.0f, xda = ok ? sxd[c16][k] : 0.0f;
+            for (int st = 0; st < kB2Chunk; ++st) {
+                if (4 * (c0 + st) >= ni) break;
+                const int k = 4 * (c0 + st) + k4;
+                const bool ok = k < ni;
+                const float xa = ok ? sx[c16 * KP + k] : 0.0f, xda = ok ? sxd[c16 * KP + k] : 0.0f;
 #pragma unroll
-            for (int ct = 0; ct < NT; ++ct) {
-                if (ct * 16 >= no) break;
-                AZ[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa, wr[st][ct], AZ[ct], 0, 0, 0);
-                AD[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(xda, wr[st][ct], AD[ct], 0, 0, 0);
+                for (int ct = 0; ct < NT; ++ct) {
+                    if (ct * 16 >= no) break;
+                    AZ[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa, w[st][ct], AZ[ct], 0, 0, 0);
+                    AD[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(xda, w[st][ct], AD[ct], 0, 0, 0);
+                }
+            }
+        };
+        {
+            float wa[kB2Chunk][NT], wb[kB2Chunk][NT];
+            const int nst = (ni + 3) / 4;
+            wload(wa, 0);
+            for (int c0 = 0; c0 < nst; c0 += 2 * kB2Chunk) {
+                if (c0 + kB2Chunk < nst) wload(wb, c0 + kB2Chunk);
+                wmul(wa, c0);
+                if (c0 + kB2Chunk >= nst) break;
+                if (c0 + 2 * kB2Chunk < nst) wload(wa, c0 + 2 * kB2Chunk);
+                wmul(wb, c0 + kB2Chunk);
             }
         }
 #pragma unroll
@@ -307,8 +331,8 @@ __global__ __launch_bounds__(64) void k_bwd2_mfma(Bwd2Args A, const int* __restr
                 oa[ct][i] = av;
                 od[ct][i] = dv * ad[i];
                 if (n < no) {
-                    sdz[l][rb + i][n] = dv;
-                    sqz[l][rb + i][n] = d2 * ad[i];
+                    sdz[(l * kB2Tile + rb + i) * WP + n] = dv;
+                    sqz[(l * kB2Tile + rb + i) * WP + n] = d2 * ad[i];
                 }
             }
         }
@@ -321,8 +345,8 @@ __global__ __launch_bounds__(64) void k_bwd2_mfma(Bwd2Args A, const int* __restr
             for (int i = 0; i < 4; ++i) {
                 const int r = rb + i;
                 if (n < no) {
-                    sx[r][n] = oa[ct][i];
-                    sxd[r][n] = od[ct][i];
+                    sx[r * KP + n] = oa[ct][i];
+                    sxd[r * KP + n] = od[ct][i];
                     if (srow[r] >= 0) {
                         float* row = A.scr + srow[r] + loff[l + 1];
                         row[n] = oa[ct][i];
@@ -349,10 +373,10 @@ __global__ __launch_bounds__(64) void k_bwd2_mfma(Bwd2Args A, const int* __restr
         __syncthreads();
         for (int q = lane; q < kB2Tile * nlast; q += kWave) {
             const int r = q / nlast, m = q % nlast;
-            const float d = sdz[L - 2][r][m], qq = sqz[L - 2][r][m];
+            const float d = sdz[((L - 2) * kB2Tile + r) * WP + m], qq = sqz[((L - 2) * kB2Tile + r) * WP + m];
             const float ab = WL[m] * sgE[r] * D.outw, adb = WL[m] * zdbL;
-            sx[r][m] = ab * d + adb * qq;
-            sxd[r][m] = adb * d;
+            sx[r * KP + m] = ab * d + adb * qq;
+            sxd[r * KP + m] = adb * d;
         }
         __syncthreads();
     }
@@ -363,8 +387,8 @@ __global__ __launch_bounds__(64) void k_bwd2_mfma(Bwd2Args A, const int* __restr
             const int r = q / no, n = q % no;
             if (srow[r] >= 0) {
                 float* row = A.scr + srow[r] + loff[l] + 2 * nprev;
-                row[n] = sx[r][n];
-                row[no + n] = sxd[r][n];
+                row[n] = sx[r * KP + n];
+                row[no + n] = sxd[r * KP + n];
             }
         }
         if (l == 0) break;
@@ -373,27 +397,43 @@ __global__ __launch_bounds__(64) void k_bwd2_mfma(Bwd2Args A, const int* __restr
         f32x4 A1[NT], A2[NT];
 #pragma unroll
         for (int mt = 0; mt < NT; ++mt) { A1[mt] = f32x4{0.f, 0.f, 0.f, 0.f}; A2[mt] = f32x4{0.f, 0.f, 0.f, 0.f}; }
-        float wr[kMaxWidth / 4][NT];       // no <= kMaxWidth: weights up front, as in the forward
+        // W_l^T operands in chunks, double buffered as in the forward
+        auto wload = [&](float (&w)[kB2Chunk][NT], int c0) {
 #pragma unroll
-        for (int st = 0; st < kMaxWidth / 4; ++st) {
-            const int k = 4 * st + k4;
+            for (int st = 0; st < kB2Chunk; ++st) {
+                const int k = 4 * (c0 + st) + k4;
 #pragma unroll
-            for (int mt = 0; mt < NT; ++mt) {
-                const int m = mt * 16 + c16;
-                wr[st][mt] = (4 * st < no && k < no && m < nprev) ? Wl[m * no + k] : 0.0f;
+                for (int mt = 0; mt < NT; ++mt) {
+                    const int m = mt * 16 + c16;
+                    w[st][mt] = (k < no && m < nprev) ? Wl[m * no + k] : 0.0f;
+                }
             }
-        }
+        };
+        auto wmul = [&](const float (&w)[kB2Chunk][NT], int c0) {
 #pragma unroll
-        for (int st = 0; st < kMaxWidth / 4; ++st) {
-            if (4 * st >= no) break;
-            const int k = 4 * st + k4;
-            const bool ok = k < no;
-            const float za = ok ? sx[c16][k] : 0.0f, zda = ok ? sxd[c16][k] : 0.0f;
+            for (int st = 0; st < kB2Chunk; ++st) {
+                if (4 * (c0 + st) >= no) break;
+                const int k = 4 * (c0 + st) + k4;
+                const bool ok = k < no;
+                const float za = ok ? sx[c16 * KP + k] : 0.0f, zda = ok ? sxd[c16 * KP + k] : 0.0f;
 #pragma unroll
-            for (int mt = 0; mt < NT; ++mt) {
-                if (mt * 16 >= nprev) break;
-                A1[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(za, wr[st][mt], A1[mt], 0, 0, 0);
-                A2[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(zda, wr[st][mt], A2[mt], 0, 0, 0);
+                for (int mt = 0; mt < NT; ++mt) {
+                    if (mt * 16 >= nprev) break;
+                    A1[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(za, w[st][mt], A1[mt], 0, 0, 0);
+                    A2[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(zda, w[st][mt], A2[mt], 0, 0, 0);
+                }
+            }
+        };
+        {
+            float wa[kB2Chunk][NT], wb[kB2Chunk][NT];
+            const int nst = (no + 3) / 4;
+            wload(wa, 0);
+            for (int c0 = 0; c0 < nst; c0 += 2 * kB2Chunk) {
+                if (c0 + kB2Chunk < nst) wload(wb, c0 + kB2Chunk);
+                wmul(wa, c0);
+                if (c0 + kB2Chunk >= nst) break;
+                if (c0 + 2 * kB2Chunk < nst) wload(wa, c0 + 2 * kB2Chunk);
+                wmul(wb, c0 + kB2Chunk);
             }
         }
 #pragma unroll
@@ -404,7 +444,8 @@ __global__ __launch_bounds__(64) void k_bwd2_mfma(Bwd2Args A, const int* __restr
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int r = rb + i;
-                const float d = m < nprev ? sdz[l - 1][r][m] : 0.0f, qq = m < nprev ? sqz[l - 1][r][m] : 0.0f;
+                const float d = m < nprev ? sdz[((l - 1) * kB2Tile + r) * WP + m] : 0.0f,
+                            qq = m < nprev ? sqz[((l - 1) * kB2Tile + r) * WP + m] : 0.0f;
                 nb[mt][i] = a1[i] * d + a2[i] * qq;
                 ndb[mt][i] = a2[i] * d;
             }
@@ -416,7 +457,7 @@ __global__ __launch_bounds__(64) void k_bwd2_mfma(Bwd2Args A, const int* __restr
             const int m = mt * 16 + c16;
 #pragma unroll
             for (int i = 0; i < 4; ++i)
-                if (m < nprev) { sx[rb + i][m] = nb[mt][i]; sxd[rb + i][m] = ndb[mt][i]; }
+                if (m < nprev) { sx[(rb + i) * KP + m] = nb[mt][i]; sxd[(rb + i) * KP + m] = ndb[mt][i]; }
         }
         __syncthreads();
     }
@@ -641,9 +682,20 @@ hipError_t net_bwd2(ann_ctx* ctx, const float* G, const float* v, const double* 
     if (mfma_net_supported(ctx->hm)) {
         if ((e = species_perm(ctx)) != hipSuccess) return e;
         Launch L(ctx, "net_bwd2");
+        int kmax = 1, wmax = 1, nlh = 1;
+        for (int s = 0; s < ctx->S; ++s) {
+            const NetDesc& D = ctx->hm.net[s];
+            kmax = std::max(kmax, D.nin);
+            nlh = std::max(nlh, D.nlayer - 1);
+            for (int l = 0; l < D.nlayer; ++l) { kmax = std::max(kmax, D.nodes[l]); wmax = std::max(wmax, D.nodes[l]); }
+        }
+        A.kp = (kmax + 1) | 1;
+        A.wp = (wmax + 1) | 1;
+        A.nlh = nlh;
+        const size_t smem = sizeof(float) * (2 * (size_t)kB2Tile * A.kp + 2 * (size_t)nlh * kB2Tile * A.wp);
         if (ctx->ntiles > 0)
-            k_bwd2_mfma<<<ctx->ntiles, 64, 0, ctx->stream>>>(A, ctx->d_perm, reinterpret_cast<const int2*>(ctx->d_tiles),
-                                                            ctx->d_spend_end);
+            k_bwd2_mfma<<<ctx->ntiles, 64, smem, ctx->stream>>>(A, ctx->d_perm, reinterpret_cast<const int2*>(ctx->d_tiles),
+                                                               ctx->d_spend_end);
     } else {
         Launch L(ctx, "net_bwd2");
         k_net_bwd2<<<(A.N + 3) / 4, 64, 0, ctx->stream>>>(A);
