@@ -99,31 +99,41 @@ __global__ __launch_bounds__(256) void k_chunk_sums(const int* __restrict__ chun
                                                     const double* __restrict__ keatom, int qmask,
                                                     double* __restrict__ part, double* __restrict__ E,
                                                     double* __restrict__ V, double* __restrict__ KE) {
-    __shared__ double red[256];
-    const int c = blockIdx.x, t = threadIdx.x;
+    __shared__ double red[256 / kWave][kSumQ];
+    const int c = blockIdx.x, t = threadIdx.x, lane = t % kWave, w = t / kWave;
     const int s = chunks[3 * c], a0 = chunks[3 * c + 1], a1 = chunks[3 * c + 2];
+    // every quantity in one pass over the chunk (thread-strided, fixed order)
+    double acc[kSumQ];
+#pragma unroll
+    for (int q = 0; q < kSumQ; ++q) acc[q] = 0.0;
+    for (int a = a0 + t; a < a1; a += blockDim.x) {
+        if (qmask & 1) acc[0] += (double)Eatom[a] + (with_evac ? dm->net[species[a]].evac : 0.0);
+        if (qmask & 2) {
+#pragma unroll
+            for (int q = 1; q < 10; ++q) acc[q] += avir[9 * (size_t)a + (q - 1)];
+        }
+        if (qmask & 4) acc[10] += keatom[a];
+    }
+    // wavefront butterflies, then the four wavefront partials in order
+#pragma unroll
     for (int q = 0; q < kSumQ; ++q) {
+        double v = acc[q];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        if (lane == 0) red[w][q] = v;
+    }
+    __syncthreads();
+    if (t < kSumQ) {
+        const int q = t;
         const int bit = q == 0 ? 1 : (q < 10 ? 2 : 4);
-        if (!(qmask & bit)) continue;
-        double acc = 0.0;
-        for (int a = a0 + t; a < a1; a += blockDim.x) {
-            if (q == 0) acc += (double)Eatom[a] + (with_evac ? dm->net[species[a]].evac : 0.0);
-            else if (q < 10) acc += avir[9 * (size_t)a + (q - 1)];
-            else acc += keatom[a];
+        if (qmask & bit) {
+            double v = red[0][q];
+            for (int k = 1; k < 256 / kWave; ++k) v += red[k][q];
+            if (part) part[(size_t)kSumQ * c + q] = v;
+            else if (q == 0) E[s] = v;
+            else if (q < 10) V[9 * (size_t)s + (q - 1)] = v;
+            else KE[s] = v;
         }
-        red[t] = acc;
-        __syncthreads();
-        for (int w = 128; w > 0; w >>= 1) {
-            if (t < w) red[t] += red[t + w];
-            __syncthreads();
-        }
-        if (t == 0) {
-            if (part) part[(size_t)kSumQ * c + q] = red[0];
-            else if (q == 0) E[s] = red[0];
-            else if (q < 10) V[9 * (size_t)s + (q - 1)] = red[0];
-            else KE[s] = red[0];
-        }
-        __syncthreads();
     }
 }
 

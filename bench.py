@@ -46,11 +46,12 @@ def work_model(model, batch, nl):
     moment form (M = (Z+1)(Z+2)(Z+3)/6 monomials, U widths of the centre species,
     NP species pairs, nk (lambda, zeta) combinations, S species):
       symm_fwd_mom = P(10 + 6 nR) + sum_i [n_i (M(2 + 2U) + 2U) + NP U 3M + sum_f 2(zeta_f + 1)]
-      symm_bwd_mom = P(20 + 6 nR) + sum_i [n_i (M(2 + 2U) + 2U + M(4U + 16) + 20)
-                                            + NP U (Z+1) 2 nk + S M U 2S]
+      symm_bwd_mom = P(20 + 6 nR) + sum_i [n_i (M(4U + 16) + 20) + NP U (Z+1) 2 nk + S M U 2S]
+                     (the moments themselves come from the G kernel)
     networks: 4 MAC per atom and parameter-matrix entry (forward + input backward);
-    HBM kernels in bytes: nb_search 24 B/atom + 8 B/slot, nb_compact 24 B/atom +
-    45 B/slot, force_gather 96 B/atom + 44 B/slot, struc_sums 76 B/atom."""
+    HBM kernels in bytes: nb_search 24 B/atom + 8 B/slot, nb_compact (CSR fill and
+    reverse slots) 28 B/atom + 57 B/slot, force_gather 96 B/atom + 44 B/slot, struc_sums
+    76 B/atom."""
     off = nl["off"]
     nsp = batch.species[nl["idx"]]
     S = model.nspecies
@@ -81,7 +82,8 @@ def work_model(model, batch, nl):
         nk = max(len(set(zip(bl[2].tolist(), bl[3].tolist()))) for bl in blocks)
         zsum = sum(float((bl[2] + 1).sum()) for bl in blocks)
         fwd_m += n_i * 10 + 6 * pair + n_i * (M * (2 + 2 * U) + 2 * U) + NP * U * 3 * M + 2 * zsum
-        bwd_m += n_i * 20 + 6 * pair + n_i * (M * (2 + 2 * U) + 2 * U + M * (4 * U + 16) + 20) \
+        # the adjoint reads the moments the G kernel stored (no moment sums of its own)
+        bwd_m += n_i * 20 + 6 * pair + n_i * (M * (4 * U + 16) + 20) \
             + NP * U * (Z + 1) * 2 * nk + S * M * U * 2 * S
     mac = 0
     for i in range(batch.ntotal):
@@ -93,7 +95,7 @@ def work_model(model, batch, nl):
     N = batch.ntotal
     return {"symm_fwd": fwd, "symm_bwd": bwd, "symm_fwd_mom": fwd_m, "symm_bwd_mom": bwd_m, "net": 4.0 * mac,
             "pairs": P, "triplets": T,
-            "bytes": {"nb_search": 24.0 * N + 8.0 * P, "nb_compact": 24.0 * N + 45.0 * P,
+            "bytes": {"nb_search": 24.0 * N + 8.0 * P, "nb_compact": 28.0 * N + 57.0 * P,
                       "force_gather": 96.0 * N + 44.0 * P, "struc_sums": 76.0 * N}}
 
 
