@@ -355,6 +355,7 @@ extern "C" int ann_destroy(ann_ctx* ctx) {
     if (ctx->h_rb) hipHostFree(ctx->h_rb);
     for (hipEvent_t e : ctx->evpool) hipEventDestroy(e);
     if (ctx->gexec) hipGraphExecDestroy(ctx->gexec);
+    if (ctx->ef_exec) hipGraphExecDestroy(ctx->ef_exec);
     for (hipEvent_t e : ctx->gev)
         if (e) hipEventDestroy(e);
     if (ctx->gstream) hipStreamDestroy(ctx->gstream);
@@ -645,15 +646,85 @@ extern "C" int ann_force_reduce(ann_ctx* ctx, const float* P, const float* Eatom
     return ANN_OK;
 }
 
+static hipError_t ef_body(ann_ctx* ctx, int form, int with_evac, double* E, double* F, double* V) {
+    hipError_t e;
+    if ((e = symm_fwd(ctx, form, ctx->d_G)) != hipSuccess) return e;
+    if ((e = net_fwd(ctx, form, ctx->d_G, ctx->d_Eatom, F || V ? ctx->d_dEdG : nullptr)) != hipSuccess) return e;
+    if (F || V) {
+        if ((e = symm_bwd(ctx, form, ctx->d_dEdG, ctx->d_P)) != hipSuccess) return e;
+        return force_reduce(ctx, ctx->d_P, ctx->d_Eatom, with_evac, F, E, V);
+    }
+    if (E) return struc_sums(ctx, ctx->d_Eatom, with_evac, nullptr, nullptr, E, nullptr, nullptr);
+    return hipSuccess;
+}
+
 extern "C" int ann_energy_forces(ann_ctx* ctx, int form, int with_evac, double* E, double* F, double* V) {
     int rc = need_nlist(ctx); if (rc) return rc;
     if ((rc = check_form(form))) return rc;
     RET_HIP(ensure_work(ctx), "allocation");
-    RET_HIP(symm_fwd(ctx, form, ctx->d_G), "symm_fwd");
-    RET_HIP(net_fwd(ctx, form, ctx->d_G, ctx->d_Eatom, F || V ? ctx->d_dEdG : nullptr), "net_fwd");
-    if (F || V) RET_HIP(symm_bwd(ctx, form, ctx->d_dEdG, ctx->d_P), "symm_bwd");
-    if (F || V) RET_HIP(force_reduce(ctx, ctx->d_P, ctx->d_Eatom, with_evac, F, E, V), "force_reduce");
-    else if (E) RET_HIP(struc_sums(ctx, ctx->d_Eatom, with_evac, nullptr, nullptr, E, nullptr, nullptr), "energy");
+    if (!(ctx->ef_graph_on && ctx->ef_warm)) {
+        RET_HIP(ef_body(ctx, form, with_evac, E, F, V), "energy_forces");
+        ctx->ef_warm = true;
+        return ANN_OK;
+    }
+    // everything the captured launches bake in: arguments, buffers, grid and LDS sizes
+    const NList& nl = ctx->nl;
+    std::vector<uint64_t> key = {
+        (uint64_t)form, (uint64_t)with_evac, (uint64_t)(uintptr_t)E, (uint64_t)(uintptr_t)F, (uint64_t)(uintptr_t)V,
+        (uint64_t)(uintptr_t)nl.off, (uint64_t)(uintptr_t)nl.idx, (uint64_t)(uintptr_t)nl.img,
+        (uint64_t)(uintptr_t)nl.rev, (uint64_t)(uintptr_t)nl.geo, (uint64_t)(uintptr_t)nl.nsp, (uint64_t)nl.maxn,
+        (uint64_t)ctx->natoms, (uint64_t)ctx->nstruc, (uint64_t)(uintptr_t)ctx->d_pos,
+        (uint64_t)(uintptr_t)ctx->d_species, (uint64_t)(uintptr_t)ctx->d_G, (uint64_t)(uintptr_t)ctx->d_P,
+        (uint64_t)(uintptr_t)ctx->d_dEdG, (uint64_t)(uintptr_t)ctx->d_Eatom, (uint64_t)(uintptr_t)ctx->d_atomvir,
+        (uint64_t)(uintptr_t)ctx->d_perm,
+        (uint64_t)(uintptr_t)ctx->d_T, (uint64_t)(uintptr_t)ctx->d_chunks, (uint64_t)ctx->nchunks,
+        (uint64_t)ctx->tune_apb[0], (uint64_t)ctx->tune_apb[1], (uint64_t)ctx->tune_apb[2],
+        (uint64_t)ctx->tune_angular, (uint64_t)ctx->wsplit_valid};
+    if (!ctx->ef_exec || key != ctx->ef_key) {
+        if (ctx->ef_exec) { hipGraphExecDestroy(ctx->ef_exec); ctx->ef_exec = nullptr; }
+        // host-side set-up that cannot be captured: the species permutation
+        if (form != ANN_FORM_PLAIN) RET_HIP(species_perm(ctx), "species permutation");
+        hipStream_t user = ctx->stream;
+        RET_HIP(hipEventRecord(ctx->gev[0], user), "graph event");
+        RET_HIP(hipStreamWaitEvent(ctx->gstream, ctx->gev[0], 0), "graph wait");
+        ctx->stream = ctx->gstream;
+        ctx->capturing = true;
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamBeginCapture(ctx->gstream, hipStreamCaptureModeRelaxed);
+        hipError_t e2 = e == hipSuccess ? ef_body(ctx, form, with_evac, E, F, V) : e;
+        hipError_t e3 = hipStreamEndCapture(ctx->gstream, &graph);
+        ctx->capturing = false;
+        ctx->stream = user;
+        if (e != hipSuccess || e2 != hipSuccess || e3 != hipSuccess)
+            return fail(ANN_ERR_HIP, std::string("energy/forces capture: ") +
+                                         hipGetErrorString(e != hipSuccess ? e : (e2 != hipSuccess ? e2 : e3)));
+        e = hipGraphInstantiate(&ctx->ef_exec, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (e != hipSuccess) { ctx->ef_exec = nullptr; return fail(ANN_ERR_HIP, std::string("graph: ") + hipGetErrorString(e)); }
+        ctx->ef_key = key;
+    } else {
+        RET_HIP(hipEventRecord(ctx->gev[0], ctx->stream), "graph event");
+        RET_HIP(hipStreamWaitEvent(ctx->gstream, ctx->gev[0], 0), "graph wait");
+    }
+    RET_HIP(hipGraphLaunch(ctx->ef_exec, ctx->gstream), "graph launch");
+    RET_HIP(hipEventRecord(ctx->gev[1], ctx->gstream), "graph event");
+    RET_HIP(hipStreamWaitEvent(ctx->stream, ctx->gev[1], 0), "graph wait");
+    return ANN_OK;
+}
+
+extern "C" int ann_ef_graph(ann_ctx* ctx, int enable) {
+    CHECK_CTX(ctx);
+    if (enable && !ctx->gstream) {
+        ANN_HIP(hipStreamCreateWithFlags(&ctx->gstream, hipStreamNonBlocking));
+        ANN_HIP(hipEventCreateWithFlags(&ctx->gev[0], hipEventDisableTiming));
+        ANN_HIP(hipEventCreateWithFlags(&ctx->gev[1], hipEventDisableTiming));
+    }
+    if (!enable && ctx->ef_exec) {
+        ANN_HIP(hipStreamSynchronize(ctx->gstream));
+        ANN_HIP(hipGraphExecDestroy(ctx->ef_exec));
+        ctx->ef_exec = nullptr;
+    }
+    ctx->ef_graph_on = enable != 0;
     return ANN_OK;
 }
 

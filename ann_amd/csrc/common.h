@@ -220,6 +220,9 @@ struct ann_ctx {
     uint64_t gen = 0;                 // bumped whenever positions, the list or the batch change
     float* d_T = nullptr;             // moment form: per-atom moments T of the last G evaluation
     uint64_t T_gen = ~0ull;           // gen at which d_T was written (~0: never)
+    bool ef_graph_on = false, ef_warm = false;   // E+F graph: enabled, one eager call done
+    hipGraphExec_t ef_exec = nullptr;
+    std::vector<uint64_t> ef_key;               // what the captured E+F kernels depend on
     uint64_t gkey_gen = 0;
     int gkey_form = -1;
     const void* gkey_ptr[4] = {nullptr, nullptr, nullptr, nullptr};
@@ -263,6 +266,7 @@ struct Launch {
     ~Launch();
 };
 hipError_t neighbors_build(ann_ctx* ctx, double rc);
+hipError_t species_perm(ann_ctx* ctx);   // species-sorted atom order and 16-atom tiles of the batch
 // rebuild the BF16x3 weight tables of the matrix-core network from the FP32
 // weights; tick, if set, is incremented by one thread (the Adam step counter)
 hipError_t split_weights(ann_ctx* ctx, int* tick);

@@ -54,7 +54,7 @@ EXPORTS = ["ann_last_error", "ann_create", "ann_destroy", "ann_set_stream", "ann
            "ann_set_scaling", "ann_set_batch", "ann_neighbors", "ann_nlist_size", "ann_nlist_copy",
            "ann_symm_fwd", "ann_net_fwd", "ann_symm_bwd", "ann_force_reduce", "ann_energy_forces",
            "ann_symm_tangent", "ann_net_bwd2", "ann_train_step", "ann_get_params", "ann_profile",
-           "ann_profile_get", "ann_refresh_geometry", "ann_wrap_positions", "ann_md_step", "ann_train_graph",
+           "ann_profile_get", "ann_refresh_geometry", "ann_wrap_positions", "ann_md_step", "ann_train_graph", "ann_ef_graph",
            "ann_autotune", "ann_get_tuning", "ann_set_tuning"]
 
 
@@ -227,6 +227,11 @@ class Engine:
     def train_graph(self, enable: bool = True) -> None:
         """Capture the training step as a HIP graph after its first call and replay it."""
         _check(lib().ann_train_graph(self.h, int(enable)))
+
+    def ef_graph(self, enable: bool = True) -> None:
+        """Capture energy_forces as a HIP graph after its first call and replay it
+        while its buffers, the list's arrays and the tuning stay the same."""
+        _check(lib().ann_ef_graph(self.h, int(enable)))
 
     # ------------------------------------------------------------ molecular dynamics
     def refresh_geometry(self) -> None:

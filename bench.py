@@ -200,6 +200,11 @@ def run_uneven(steps, warmup):
         dt = time_ef(eng, form, k, max(1, warmup // 2), out)
         res[name] = {"atoms_per_s": batch.ntotal / dt, "ms_per_step": dt * 1e3, "steps": k}
     res["fast_over_plain"] = res["fast"]["atoms_per_s"] / res["plain"]["atoms_per_s"]
+    # the same FAST step with the E+F kernels replayed from a captured HIP graph
+    eng.ef_graph(True)
+    dt = time_ef(eng, FAST, steps, max(2, warmup // 2), out)
+    eng.ef_graph(False)
+    res["fast_graph"] = {"atoms_per_s": batch.ntotal / dt, "ms_per_step": dt * 1e3, "steps": steps}
     return res
 
 

@@ -188,6 +188,13 @@ int ann_train_step(ann_ctx* ctx, int form, const double* d_Eref, const double* d
  * context stream).  A new batch, list or positions re-captures.  No
  * counterpart in the reference (its trainer is host-driven). */
 int ann_train_graph(ann_ctx* ctx, int enable);
+/* enable (1) or disable (0) HIP-graph capture of ann_energy_forces: after one
+ * eager call the E+F kernels (basis, network, adjoint, force gather, sums) are
+ * captured on a private stream and later calls replay them while the form,
+ * the output buffers, the list's arrays and widest row, and the tuning are
+ * unchanged; a neighbour build that keeps them keeps the graph (the kernels
+ * read positions and list by pointer).  No counterpart in the reference. */
+int ann_ef_graph(ann_ctx* ctx, int enable);
 /* copy the FP64 master parameters (device) to host */
 int ann_get_params(ann_ctx* ctx, double* params, int n);
 

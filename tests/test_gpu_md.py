@@ -145,3 +145,31 @@ def test_chunked_sums_of_a_replica():
     assert abs(Eb - 64 * Es) <= 1e-6 * big.ntotal
     assert rf <= 1e-5
     assert np.abs(Vb.cpu().numpy() - 64 * Vs.cpu().numpy()).max() <= 1e-4 * np.abs(64 * Vs.cpu().numpy()).max()
+
+
+def test_ef_graph_replays_match_eager():
+    """energy_forces replayed from a captured HIP graph, with positions moved and
+    the list rebuilt before every call, gives bitwise the eager results."""
+    import torch
+    from ann_amd import configs
+    from ann_amd.gpu import Engine, FAST
+    model, batch = configs.model_water(), configs.batch_water(4, seed=21)
+    e1, e2 = Engine(model, "cuda:0"), Engine(model, "cuda:0")
+    e1.set_batch(batch)
+    e2.set_batch(batch)
+    e2.ef_graph(True)
+    dev = e1.device
+    outs = [tuple(torch.empty(sh, dtype=torch.float64, device=dev)
+                  for sh in ((batch.nstruc,), (batch.ntotal, 3), (batch.nstruc, 3, 3))) for _ in range(2)]
+    rng = np.random.default_rng(5)
+    for step in range(6):
+        print(f"step {step}", flush=True)
+        d = torch.tensor(rng.normal(scale=0.01, size=(batch.ntotal, 3)), device=dev)
+        for e, o in ((e1, outs[0]), (e2, outs[1])):
+            e.d_pos += d
+            e.neighbors()
+            e.energy_forces(FAST, out=o)
+        torch.cuda.synchronize()
+        for a, b in zip(outs[0], outs[1]):
+            assert torch.equal(a, b), step
+    e2.ef_graph(False)
