@@ -551,7 +551,62 @@ __host__ __device__ inline bool fast_shape_ok(int ng, int nk) {
 }
 
 // radial part, lane over the neighbours of one species segment, 16 functions at a time
-constexpr int kRadChunk = 16;
+// GAUSSIAN radial functions of one (centre, neighbour species) block, CH per
+// pass with widths and centres in registers.  Adjoint: per neighbour,
+// d/dr [e fc] = e (gc - 2 eta x fc) weighted by dE/dG, split into two running
+// sums; G / tangent: lanes over the neighbours, one transposing reduction.
+template <int MODE, int CH>
+__device__ __forceinline__ void radial_gauss(const SymmArgs& A, const RadBlock& rb, const FastSmem& sm, int s0, int s1,
+                                             const float4* tv, const float* sd, float* sP, float* out) {
+    const int lane = threadIdx.x % kWave;
+    for (int m0 = 0; m0 < rb.n; m0 += CH) {
+        float eta[CH], rs[CH], gs[CH];
+#pragma unroll
+        for (int m = 0; m < CH; ++m) {
+            const bool in = m0 + m < rb.n;
+            eta[m] = in ? A.rad_eta[rb.par + m0 + m] : 0.0f;
+            rs[m] = in ? A.rad_rs[rb.par + m0 + m] : 0.0f;
+            gs[m] = (MODE == 1 && in) ? sd[rb.goff + m0 + m] : 0.0f;
+        }
+        if (MODE == 1) {
+            for (int J = s0 + lane; J < s1; J += kWave) {
+                const float r = sm.r[J];
+                const float2 fg = sm.fg[J];
+                float se = 0.0f, sx = 0.0f;
+#pragma unroll
+                for (int m = 0; m < CH; ++m) {
+                    const float x = eta[m] * (r - rs[m]), ge = gs[m] * __expf(-x * x);
+                    se += ge;
+                    sx += ge * eta[m] * x;
+                }
+                const float amp = se * fg.y - 2.0f * fg.x * sx;
+                const float4 gj = sm.g[J];
+                sP[3 * J] += amp * gj.x;
+                sP[3 * J + 1] += amp * gj.y;
+                sP[3 * J + 2] += amp * gj.z;
+            }
+            continue;
+        }
+        float acc[CH];
+#pragma unroll
+        for (int m = 0; m < CH; ++m) acc[m] = 0.0f;
+        for (int J = s0 + lane; J < s1; J += kWave) {
+            const float r = sm.r[J];
+            const float w = MODE == 2 ? tv[J].w : 0.0f;
+            const float2 fg = sm.fg[J];
+            const float fc = fg.x, gc = fg.y;
+#pragma unroll
+            for (int m = 0; m < CH; ++m) {
+                const float x = eta[m] * (r - rs[m]), e = __expf(-x * x);
+                acc[m] += MODE == 0 ? e * fc : e * (gc - 2.0f * eta[m] * x * fc) * w;
+            }
+        }
+        int idx;
+        const float sum = wave_sum_multi<CH>(acc, idx);
+        if ((lane & (kWave / CH - 1)) == 0 && m0 + idx < rb.n) out[rb.goff + m0 + idx] = sum;
+    }
+}
+
 template <int MODE>
 __device__ __forceinline__ void fast_radial(const SymmArgs& A, const DevModel& M, int a, const FastSmem& sm, const float4* tv,
                             const float* sd, float* sP, float* out) {
@@ -560,36 +615,11 @@ __device__ __forceinline__ void fast_radial(const SymmArgs& A, const DevModel& M
         const RadBlock rb = M.rad[a * S + b];
         if (rb.n == 0) continue;
         const int s0 = sm.seg[b], s1 = sm.seg[b + 1];
-        if (MODE == 1 && rb.name == RAD_GAUSSIAN) {
-            // 16 functions per pass: widths, centres and dE/dG held in registers,
-            // d/dr [e fc] = e (gc - 2 eta x fc) split into two running sums
-            for (int m0 = 0; m0 < rb.n; m0 += kRadChunk) {
-                float eta[kRadChunk], rs[kRadChunk], gs[kRadChunk];
-#pragma unroll
-                for (int m = 0; m < kRadChunk; ++m) {
-                    const bool in = m0 + m < rb.n;
-                    eta[m] = in ? A.rad_eta[rb.par + m0 + m] : 0.0f;
-                    rs[m] = in ? A.rad_rs[rb.par + m0 + m] : 0.0f;
-                    gs[m] = in ? sd[rb.goff + m0 + m] : 0.0f;
-                }
-                for (int J = s0 + lane; J < s1; J += kWave) {
-                    const float r = sm.r[J];
-                    const float2 fg = sm.fg[J];
-                    const float fc = fg.x, gc = fg.y;
-                    float se = 0.0f, sx = 0.0f;
-#pragma unroll
-                    for (int m = 0; m < kRadChunk; ++m) {
-                        const float x = eta[m] * (r - rs[m]), ge = gs[m] * __expf(-x * x);
-                        se += ge;
-                        sx += ge * eta[m] * x;
-                    }
-                    const float amp = se * gc - 2.0f * fc * sx;
-                    const float4 gj = sm.g[J];
-                    sP[3 * J] += amp * gj.x;
-                    sP[3 * J + 1] += amp * gj.y;
-                    sP[3 * J + 2] += amp * gj.z;
-                }
-            }
+        if (rb.name == RAD_GAUSSIAN) {
+            // eight per pass for blocks of at most eight functions, else sixteen
+            // (the tangent keeps sixteen: a second copy raises its register count)
+            if (MODE != 2 && rb.n <= 8) radial_gauss<MODE, 8>(A, rb, sm, s0, s1, tv, sd, sP, out);
+            else radial_gauss<MODE, 16>(A, rb, sm, s0, s1, tv, sd, sP, out);
             continue;
         }
         if (MODE == 1) {
@@ -609,47 +639,19 @@ __device__ __forceinline__ void fast_radial(const SymmArgs& A, const DevModel& M
             }
             continue;
         }
-        if (rb.name != RAD_GAUSSIAN) {
-            // other families: one function at a time, plain wave sums
-            for (int m = 0; m < rb.n; ++m) {
-                float acc = 0.0f;
-                for (int J = s0 + lane; J < s1; J += kWave) {
-                    const float r = sm.r[J];
-                    const float2 fg = sm.fg[J];
-                    const float fc = fg.x, gc = fg.y;
-                    float v, d;
-                    radial_fd(rb.name, A.rad_eta[rb.par + m], A.rad_rs[rb.par + m], r, fc, gc, v, d);
-                    acc += MODE == 0 ? v : d * tv[J].w;
-                }
-                acc = wave_sum(acc);
-                if (lane == 0) out[rb.goff + m] = acc;
-            }
-            continue;
-        }
-        // GAUSSIAN: 16 functions per pass in registers, one transposing reduction
-        for (int m0 = 0; m0 < rb.n; m0 += kRadChunk) {
-            float acc[kRadChunk], eta[kRadChunk], rs[kRadChunk];
-#pragma unroll
-            for (int m = 0; m < kRadChunk; ++m) {
-                acc[m] = 0.0f;
-                const bool in = m0 + m < rb.n;
-                eta[m] = in ? A.rad_eta[rb.par + m0 + m] : 0.0f;
-                rs[m] = in ? A.rad_rs[rb.par + m0 + m] : 0.0f;
-            }
+        // other families: one function at a time, plain wave sums
+        for (int m = 0; m < rb.n; ++m) {
+            float acc = 0.0f;
             for (int J = s0 + lane; J < s1; J += kWave) {
                 const float r = sm.r[J];
-                const float w = MODE == 2 ? tv[J].w : 0.0f;
                 const float2 fg = sm.fg[J];
                 const float fc = fg.x, gc = fg.y;
-#pragma unroll
-                for (int m = 0; m < kRadChunk; ++m) {
-                    const float x = eta[m] * (r - rs[m]), e = __expf(-x * x);
-                    acc[m] += MODE == 0 ? e * fc : e * (gc - 2.0f * eta[m] * x * fc) * w;
-                }
+                float v, d;
+                radial_fd(rb.name, A.rad_eta[rb.par + m], A.rad_rs[rb.par + m], r, fc, gc, v, d);
+                acc += MODE == 0 ? v : d * tv[J].w;
             }
-            int idx;
-            const float s = wave_sum_multi<kRadChunk>(acc, idx);
-            if ((lane & (kWave / kRadChunk - 1)) == 0 && m0 + idx < rb.n) out[rb.goff + m0 + idx] = s;
+            acc = wave_sum(acc);
+            if (lane == 0) out[rb.goff + m] = acc;
         }
     }
 }
