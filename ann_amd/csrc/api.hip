@@ -27,15 +27,18 @@ static int pool_event(ann_ctx* ctx) {
     return ctx->evused++;
 }
 
+static bool timed(const ann_ctx* ctx, const char* name) {
+    return ctx->prof && !ctx->capturing && (ctx->prof_only.empty() || ctx->prof_only == name);
+}
 Launch::Launch(ann_ctx* c, const char* n) : ctx(c), name(n) {
-    if (!ctx->prof || ctx->capturing) return;
+    if (!timed(ctx, name)) return;
     const int a = pool_event(ctx), b = pool_event(ctx);
     if (a < 0 || b < 0) return;
     hipEventRecord(ctx->evpool[a], ctx->stream);
     ctx->timers[name].ev.push_back({a, b});
 }
 Launch::~Launch() {
-    if (!ctx->prof || ctx->capturing) return;
+    if (!timed(ctx, name)) return;
     auto& v = ctx->timers[name].ev;
     if (!v.empty()) hipEventRecord(ctx->evpool[v.back().second], ctx->stream);
 }
@@ -955,6 +958,14 @@ extern "C" int ann_profile(ann_ctx* ctx, int enable) {
     if (rc) return rc;
     ctx->prof = enable != 0;
     ctx->timers.clear();
+    return ANN_OK;
+}
+
+extern "C" int ann_profile_only(ann_ctx* ctx, const char* name) {
+    CHECK_CTX(ctx);
+    int rc = drain_timers(ctx);
+    if (rc) return rc;
+    ctx->prof_only = name ? name : "";
     return ANN_OK;
 }
 

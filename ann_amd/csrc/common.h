@@ -239,6 +239,7 @@ struct ann_ctx {
     int md_cap = 0;
     // profiling
     bool prof = false;
+    std::string prof_only;            // time only this kernel family (empty: all)
     std::map<std::string, ann::Timer> timers;
     std::vector<hipEvent_t> evpool;   // created once, reused between drains
     int evused = 0;

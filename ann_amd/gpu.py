@@ -54,7 +54,7 @@ EXPORTS = ["ann_last_error", "ann_create", "ann_destroy", "ann_set_stream", "ann
            "ann_set_scaling", "ann_set_batch", "ann_neighbors", "ann_nlist_size", "ann_nlist_copy",
            "ann_symm_fwd", "ann_net_fwd", "ann_symm_bwd", "ann_force_reduce", "ann_energy_forces",
            "ann_symm_tangent", "ann_net_bwd2", "ann_train_step", "ann_get_params", "ann_profile",
-           "ann_profile_get", "ann_refresh_geometry", "ann_wrap_positions", "ann_md_step", "ann_train_graph", "ann_ef_graph",
+           "ann_profile_get", "ann_profile_only", "ann_refresh_geometry", "ann_wrap_positions", "ann_md_step", "ann_train_graph", "ann_ef_graph",
            "ann_autotune", "ann_get_tuning", "ann_set_tuning"]
 
 
@@ -270,6 +270,10 @@ class Engine:
     # ------------------------------------------------------------ timing
     def profile(self, enable: bool = True) -> None:
         _check(lib().ann_profile(self.h, int(enable)))
+
+    def profile_only(self, name: str = "") -> None:
+        """Time only the kernel family `name` ("" for every family)."""
+        _check(lib().ann_profile_only(self.h, name.encode()))
 
     def profile_get(self, name: str):
         ms, n = C.c_double(), C.c_int()

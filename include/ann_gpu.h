@@ -216,6 +216,9 @@ int ann_profile(ann_ctx* ctx, int enable);
 /* accumulated time (ms) and launch count of the kernel family `name`;
  * synchronises the stream */
 int ann_profile_get(ann_ctx* ctx, const char* name, double* total_ms, int* count);
+/* time only the kernel family `name` (null or empty: every family), so that a
+ * timed region carries two event records per step instead of two per kernel */
+int ann_profile_only(ann_ctx* ctx, const char* name);
 
 #ifdef __cplusplus
 }
