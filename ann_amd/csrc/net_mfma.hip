@@ -67,7 +67,7 @@ __device__ __forceinline__ bf16x8 as_bf16x8(uint4 u) {
 // ((kk * nt + ct) * 3 + p) * 64 + l, p = 0, 1, 2.
 template <int NT>
 __device__ __forceinline__ void mm_bf16x3(int K, int ncol, const float (*X)[kMaxK + 1], const uint4* __restrict__ tab,
-                                          f32x4 (&acc)[NT]) {
+                                          f32x4 (&acc)[NT], int wv) {
     const int lane = threadIdx.x % kWave;
     const int row = lane % 16, kb = 8 * (lane / 16);
     const int nt = (ncol + 15) / 16;
@@ -83,6 +83,7 @@ __device__ __forceinline__ void mm_bf16x3(int K, int ncol, const float (*X)[kMax
 #pragma unroll
         for (int ct = 0; ct < NT; ++ct) {
             if (ct >= nt) break;
+            if ((ct & 1) != wv) continue;           // column tiles alternate between the two wavefronts
             const uint4* e = tab + (size_t)((kk * nt + ct) * 3) * kWave + lane;
             const bf16x8 b0 = as_bf16x8(e[0]), b1 = as_bf16x8(e[kWave]), b2 = as_bf16x8(e[2 * kWave]);
             f32x4 c = acc[ct];
@@ -148,12 +149,14 @@ hipError_t split_weights(ann_ctx* ctx, int* tick) {
     return e;
 }
 
-// one wavefront per 16-atom single-species tile
-__global__ __launch_bounds__(64) void k_net_mfma(MfmaArgs A) {
+// two wavefronts per 16-atom single-species tile: the column tiles of every
+// product alternate between them (the launch has fewer tiles than SIMDs)
+__global__ __launch_bounds__(128) void k_net_mfma(MfmaArgs A) {
     __shared__ float act[kTile][kMaxK + 1];     // activations / gradients, operand side
     __shared__ float sdz[kMaxLayers - 1][kTile][kMaxWidth + 1];
     __shared__ int atoms[kTile];
-    const int t = blockIdx.x, lane = threadIdx.x;
+    const int t = blockIdx.x, tid = threadIdx.x, lane = tid % kWave, wv = tid / kWave;
+    constexpr int kThreads = 2 * kWave;
     if (t >= A.ntiles) return;
     const int2 tl = A.tiles[t];
     const int a = tl.x;
@@ -162,7 +165,7 @@ __global__ __launch_bounds__(64) void k_net_mfma(MfmaArgs A) {
     const int L = D.nlayer;
     const float* iw = A.inpw + D.inoff;
     const float* ib = A.inpb + D.inoff;
-    if (lane < kTile) atoms[lane] = lane < nrow ? A.perm[tl.y + lane] : -1;
+    if (tid < kTile) atoms[tid] = tid < nrow ? A.perm[tl.y + tid] : -1;
     __syncthreads();
     // scaled input s = inpw (G + inpb) into LDS: lanes over the inputs, rows unrolled
     {
@@ -174,7 +177,7 @@ __global__ __launch_bounds__(64) void k_net_mfma(MfmaArgs A) {
             b2[h] = k < D.nin ? ib[k] : 0.0f;
         }
 #pragma unroll
-        for (int r = 0; r < kTile; ++r) {
+        for (int r = wv; r < kTile; r += 2) {
             const int ai = atoms[r];
 #pragma unroll
             for (int h = 0; h < kMaxK / kWave; ++h) {
@@ -193,10 +196,11 @@ __global__ __launch_bounds__(64) void k_net_mfma(MfmaArgs A) {
         f32x4 accs[kMaxWidth / 16];
 #pragma unroll
         for (int ct = 0; ct < kMaxWidth / 16; ++ct) accs[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
-        mm_bf16x3<kMaxWidth / 16>(ni, no, act, A.wtab + D.wsf[l], accs);
+        mm_bf16x3<kMaxWidth / 16>(ni, no, act, A.wtab + D.wsf[l], accs, wv);
 #pragma unroll
         for (int ct = 0; ct < kMaxWidth / 16; ++ct) {
             if (ct * 16 >= no) break;
+            if ((ct & 1) != wv) continue;
             const f32x4 acc = accs[ct];
             const int n = ct * 16 + col;
 #pragma unroll
@@ -211,12 +215,13 @@ __global__ __launch_bounds__(64) void k_net_mfma(MfmaArgs A) {
 #pragma unroll
         for (int ct = 0; ct < kMaxWidth / 16; ++ct) {
             if (ct * 16 >= no) break;
+            if ((ct & 1) != wv) continue;
             const int n = ct * 16 + col;
 #pragma unroll
             for (int i = 0; i < 4; ++i) act[rb + i][n] = n < no ? out[ct][i] : 0.0f;
         }
         // zero the padding up to the next multiple of 32
-        for (int q = lane; q < kTile * 32; q += kWave) {
+        for (int q = tid; q < kTile * 32; q += kThreads) {
             const int r = q / 32, k = no + q % 32;
             if (k < kMaxK) act[r][k] = 0.0f;
         }
@@ -226,14 +231,14 @@ __global__ __launch_bounds__(64) void k_net_mfma(MfmaArgs A) {
     // linear output layer (1 neuron) on the vector ALU
     const float* WL = A.W + D.woff[L - 1];
     const float bL = A.W[D.boff[L - 1]];
-    if (lane < nrow) {
+    if (tid < nrow) {
         float z = bL;
-        for (int m = 0; m < ni; ++m) z += WL[m] * act[lane][m];
-        A.Eatom[atoms[lane]] = D.outb + D.outw * z;
+        for (int m = 0; m < ni; ++m) z += WL[m] * act[tid][m];
+        A.Eatom[atoms[tid]] = D.outb + D.outw * z;
     }
     if (!A.dEdG) return;
     if (L == 1) {
-        for (int q = lane; q < kTile * A.ldg; q += kWave) {
+        for (int q = tid; q < kTile * A.ldg; q += kThreads) {
             const int r = q / A.ldg, m = q % A.ldg;
             if (r < nrow) A.dEdG[(size_t)atoms[r] * A.ldg + m] = m < D.nin ? D.outw * WL[m] * iw[m] : 0.0f;
         }
@@ -241,7 +246,7 @@ __global__ __launch_bounds__(64) void k_net_mfma(MfmaArgs A) {
     }
     __syncthreads();
     // g_{L-1} = outw W_L . sigma'_{L-1}  (operand layout, rows = atoms)
-    for (int q = lane; q < kTile * kMaxK; q += kWave) {
+    for (int q = tid; q < kTile * kMaxK; q += kThreads) {
         const int r = q / kMaxK, m = q % kMaxK;
         act[r][m] = m < ni ? D.outw * WL[m] * sdz[L - 2][r][m] : 0.0f;
     }
@@ -253,10 +258,11 @@ __global__ __launch_bounds__(64) void k_net_mfma(MfmaArgs A) {
         f32x4 accs[kMaxK / 16];
 #pragma unroll
         for (int ct = 0; ct < kMaxK / 16; ++ct) accs[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
-        mm_bf16x3<kMaxK / 16>(no, nprev, act, A.wtab + D.wsb[l], accs);
+        mm_bf16x3<kMaxK / 16>(no, nprev, act, A.wtab + D.wsb[l], accs, wv);
 #pragma unroll
         for (int ct = 0; ct < kMaxK / 16; ++ct) {
             if (ct * 16 >= nprev) break;
+            if ((ct & 1) != wv) continue;
             const f32x4 acc = accs[ct];
             const int m = ct * 16 + col;
 #pragma unroll
@@ -271,11 +277,12 @@ __global__ __launch_bounds__(64) void k_net_mfma(MfmaArgs A) {
 #pragma unroll
             for (int ct = 0; ct < kMaxK / 16; ++ct) {
                 if (ct * 16 >= nprev) break;
+                if ((ct & 1) != wv) continue;
                 const int m = ct * 16 + col;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) act[rb + i][m] = m < nprev ? out[ct][i] : 0.0f;
             }
-            for (int q = lane; q < kTile * 32; q += kWave) {
+            for (int q = tid; q < kTile * 32; q += kThreads) {
                 const int r = q / 32, k = nprev + q % 32;
                 if (k < kMaxK) act[r][k] = 0.0f;
             }
@@ -283,6 +290,7 @@ __global__ __launch_bounds__(64) void k_net_mfma(MfmaArgs A) {
 #pragma unroll
             for (int ct = 0; ct < kMaxK / 16; ++ct) {
                 if (ct * 16 >= A.ldg) break;
+                if ((ct & 1) != wv) continue;
                 const int m = ct * 16 + col;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
@@ -316,7 +324,7 @@ hipError_t net_fwd_mfma(ann_ctx* ctx, const float* G, float* Eatom, float* dEdG)
     if (A.ntiles == 0) return hipSuccess;
     if (!ctx->wsplit_valid && (e = split_weights(ctx, nullptr)) != hipSuccess) return e;
     Launch L(ctx, "net_mfma");
-    k_net_mfma<<<A.ntiles, 64, 0, ctx->stream>>>(A);
+    k_net_mfma<<<A.ntiles, 128, 0, ctx->stream>>>(A);
     return hipGetLastError();
 }
 
