@@ -212,7 +212,7 @@ constexpr int kB2Tile = 16;
 constexpr int kB2K = 128;      // widest input supported (same bound as the forward MFMA network)
 constexpr int kB2Chunk = 8;    // k steps (of 4) per weight chunk held in registers
 
-__global__ __launch_bounds__(64) void k_bwd2_mfma(Bwd2Args A, const int* __restrict__ perm,
+__global__ __launch_bounds__(128) void k_bwd2_mfma(Bwd2Args A, const int* __restrict__ perm,
                                                   const int2* __restrict__ tiles, const int* __restrict__ spend) {
     // dynamic LDS sized by the model: activations / gradients [16][kp] (x and its
     // tangent) and sigma', sigma'' zd of every hidden layer [L-1][16][wp]
@@ -224,7 +224,9 @@ __global__ __launch_bounds__(64) void k_bwd2_mfma(Bwd2Args A, const int* __restr
     float* sqz = sdz + A.nlh * kB2Tile * WP;
     __shared__ int64_t srow[kB2Tile];
     __shared__ float sgE[kB2Tile];
-    const int lane = threadIdx.x;
+    // two wavefronts per tile: column tiles ct = 2 h + wv of every product go to wavefront wv
+    const int tid = threadIdx.x, lane = tid % kWave, wv = tid / kWave;
+    constexpr int kThreads = 2 * kWave;
     const int2 tl = tiles[blockIdx.x];
     const int a = tl.x;
     const int nrow = min(kB2Tile, spend[a] - tl.y);
@@ -233,10 +235,10 @@ __global__ __launch_bounds__(64) void k_bwd2_mfma(Bwd2Args A, const int* __restr
     const int stride = bwd2_stride(D);
     const float* iw = A.inpw + D.inoff;
     const float* ib = A.inpb + D.inoff;
-    if (lane < kB2Tile) {
-        const int ai = lane < nrow ? perm[tl.y + lane] : -1;
-        srow[lane] = ai >= 0 ? A.base[a] + (int64_t)A.rank[ai] * stride : -1;
-        sgE[lane] = ai >= 0 ? (float)A.gE[A.sof[ai]] : 0.0f;
+    if (tid < kB2Tile) {
+        const int ai = tid < nrow ? perm[tl.y + tid] : -1;
+        srow[tid] = ai >= 0 ? A.base[a] + (int64_t)A.rank[ai] * stride : -1;
+        sgE[tid] = ai >= 0 ? (float)A.gE[A.sof[ai]] : 0.0f;
     }
     int loff[kMaxLayers];
     {
@@ -245,7 +247,7 @@ __global__ __launch_bounds__(64) void k_bwd2_mfma(Bwd2Args A, const int* __restr
     }
     // layer-0 input x = inpw (G + inpb), xd = inpw v
 #pragma unroll 8
-    for (int q = lane; q < kB2Tile * KP; q += kWave) {
+    for (int q = tid; q < kB2Tile * KP; q += kThreads) {
         const int r = q / KP, k = q % KP;
         const int ai = r < nrow ? perm[tl.y + r] : -1;
         float x = 0.0f, xd = 0.0f;
@@ -257,7 +259,8 @@ __global__ __launch_bounds__(64) void k_bwd2_mfma(Bwd2Args A, const int* __restr
         sxd[r * KP + k] = xd;
     }
     __syncthreads();
-    for (int q = lane; q < kB2Tile * D.nin; q += kWave) {
+    __syncthreads();
+    for (int q = tid; q < kB2Tile * D.nin; q += kThreads) {
         const int r = q / D.nin, k = q % D.nin;
         if (srow[r] >= 0) {
             float* row = A.scr + srow[r];
@@ -266,7 +269,7 @@ __global__ __launch_bounds__(64) void k_bwd2_mfma(Bwd2Args A, const int* __restr
         }
     }
     const int c16 = lane % 16, k4 = lane / 16, rb = 4 * k4;
-    constexpr int NT = kMaxWidth / 16;
+    constexpr int NT = kMaxWidth / 32;          // this wavefront's column tiles
     int ni = D.nin;
     for (int l = 0; l < L - 1; ++l) {
         const int no = D.nodes[l];
@@ -286,9 +289,9 @@ __global__ __launch_bounds__(64) void k_bwd2_mfma(Bwd2Args A, const int* __restr
             for (int st = 0; st < kB2Chunk; ++st) {
                 const int k = 4 * (c0 + st) + k4;
 #pragma unroll
-                for (int ct = 0; ct < NT; ++ct) {
-                    const int n = ct * 16 + c16;
-                    w[st][ct] = (k < ni && n < no) ? Wl[k * no + n] : 0.0f;
+                for (int h = 0; h < NT; ++h) {
+                    const int n = (2 * h + wv) * 16 + c16;
+                    w[st][h] = (k < ni && n < no) ? Wl[k * no + n] : 0.0f;
                 }
             }
         };
@@ -300,10 +303,10 @@ __global__ __launch_bounds__(64) void k_bwd2_mfma(Bwd2Args A, const int* __restr
                 const bool ok = k < ni;
                 const float xa = ok ? sx[c16 * KP + k] : 0.0f, xda = ok ? sxd[c16 * KP + k] : 0.0f;
 #pragma unroll
-                for (int ct = 0; ct < NT; ++ct) {
-                    if (ct * 16 >= no) break;
-                    AZ[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa, w[st][ct], AZ[ct], 0, 0, 0);
-                    AD[ct] = __builtin_amdgcn_mfma_f32_16x16x4f32(xda, w[st][ct], AD[ct], 0, 0, 0);
+                for (int h = 0; h < NT; ++h) {
+                    if ((2 * h + wv) * 16 >= no) break;
+                    AZ[h] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa, w[st][h], AZ[h], 0, 0, 0);
+                    AD[h] = __builtin_amdgcn_mfma_f32_16x16x4f32(xda, w[st][h], AD[h], 0, 0, 0);
                 }
             }
         };
@@ -320,16 +323,17 @@ __global__ __launch_bounds__(64) void k_bwd2_mfma(Bwd2Args A, const int* __restr
             }
         }
 #pragma unroll
-        for (int ct = 0; ct < NT; ++ct) {
+        for (int h = 0; h < NT; ++h) {
+            const int ct = 2 * h + wv;
             if (ct * 16 >= no) break;
             const int n = ct * 16 + c16;
-            const f32x4 az = AZ[ct], ad = AD[ct];
+            const f32x4 az = AZ[h], ad = AD[h];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 float av = 0.f, dv = 0.f, d2 = 0.f;
                 if (n < no) act_f<true>(D.neuron, D.c, az[i] + bl[n], av, dv, d2);
-                oa[ct][i] = av;
-                od[ct][i] = dv * ad[i];
+                oa[h][i] = av;
+                od[h][i] = dv * ad[i];
                 if (n < no) {
                     sdz[(l * kB2Tile + rb + i) * WP + n] = dv;
                     sqz[(l * kB2Tile + rb + i) * WP + n] = d2 * ad[i];
@@ -338,19 +342,20 @@ __global__ __launch_bounds__(64) void k_bwd2_mfma(Bwd2Args A, const int* __restr
         }
         __syncthreads();
 #pragma unroll
-        for (int ct = 0; ct < NT; ++ct) {
+        for (int h = 0; h < NT; ++h) {
+            const int ct = 2 * h + wv;
             if (ct * 16 >= no) break;
             const int n = ct * 16 + c16;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int r = rb + i;
                 if (n < no) {
-                    sx[r * KP + n] = oa[ct][i];
-                    sxd[r * KP + n] = od[ct][i];
+                    sx[r * KP + n] = oa[h][i];
+                    sxd[r * KP + n] = od[h][i];
                     if (srow[r] >= 0) {
                         float* row = A.scr + srow[r] + loff[l + 1];
-                        row[n] = oa[ct][i];
-                        row[no + n] = od[ct][i];
+                        row[n] = oa[h][i];
+                        row[no + n] = od[h][i];
                     }
                 }
             }
@@ -361,9 +366,9 @@ __global__ __launch_bounds__(64) void k_bwd2_mfma(Bwd2Args A, const int* __restr
     // output layer (linear, one neuron): zbar = gE outw, zdbar = outw
     const float zdbL = A.v ? D.outw : 0.0f;
     const int nlast = L > 1 ? D.nodes[L - 2] : D.nin;
-    if (lane < kB2Tile && srow[lane] >= 0) {
-        float* row = A.scr + srow[lane] + loff[L - 1] + 2 * nlast;
-        row[0] = sgE[lane] * D.outw;
+    if (tid < kB2Tile && srow[tid] >= 0) {
+        float* row = A.scr + srow[tid] + loff[L - 1] + 2 * nlast;
+        row[0] = sgE[tid] * D.outw;
         row[1] = zdbL;
     }
     if (L == 1) return;
@@ -371,7 +376,7 @@ __global__ __launch_bounds__(64) void k_bwd2_mfma(Bwd2Args A, const int* __restr
     {
         const float* WL = A.W + D.woff[L - 1];
         __syncthreads();
-        for (int q = lane; q < kB2Tile * nlast; q += kWave) {
+        for (int q = tid; q < kB2Tile * nlast; q += kThreads) {
             const int r = q / nlast, m = q % nlast;
             const float d = sdz[((L - 2) * kB2Tile + r) * WP + m], qq = sqz[((L - 2) * kB2Tile + r) * WP + m];
             const float ab = WL[m] * sgE[r] * D.outw, adb = WL[m] * zdbL;
@@ -383,7 +388,7 @@ __global__ __launch_bounds__(64) void k_bwd2_mfma(Bwd2Args A, const int* __restr
     for (int l = L - 2; l >= 0; --l) {
         const int no = D.nodes[l];
         const int nprev = l == 0 ? D.nin : D.nodes[l - 1];
-        for (int q = lane; q < kB2Tile * no; q += kWave) {
+        for (int q = tid; q < kB2Tile * no; q += kThreads) {
             const int r = q / no, n = q % no;
             if (srow[r] >= 0) {
                 float* row = A.scr + srow[r] + loff[l] + 2 * nprev;
@@ -403,9 +408,9 @@ __global__ __launch_bounds__(64) void k_bwd2_mfma(Bwd2Args A, const int* __restr
             for (int st = 0; st < kB2Chunk; ++st) {
                 const int k = 4 * (c0 + st) + k4;
 #pragma unroll
-                for (int mt = 0; mt < NT; ++mt) {
-                    const int m = mt * 16 + c16;
-                    w[st][mt] = (k < no && m < nprev) ? Wl[m * no + k] : 0.0f;
+                for (int h = 0; h < NT; ++h) {
+                    const int m = (2 * h + wv) * 16 + c16;
+                    w[st][h] = (k < no && m < nprev) ? Wl[m * no + k] : 0.0f;
                 }
             }
         };
@@ -417,10 +422,10 @@ __global__ __launch_bounds__(64) void k_bwd2_mfma(Bwd2Args A, const int* __restr
                 const bool ok = k < no;
                 const float za = ok ? sx[c16 * KP + k] : 0.0f, zda = ok ? sxd[c16 * KP + k] : 0.0f;
 #pragma unroll
-                for (int mt = 0; mt < NT; ++mt) {
-                    if (mt * 16 >= nprev) break;
-                    A1[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(za, w[st][mt], A1[mt], 0, 0, 0);
-                    A2[mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(zda, w[st][mt], A2[mt], 0, 0, 0);
+                for (int h = 0; h < NT; ++h) {
+                    if ((2 * h + wv) * 16 >= nprev) break;
+                    A1[h] = __builtin_amdgcn_mfma_f32_16x16x4f32(za, w[st][h], A1[h], 0, 0, 0);
+                    A2[h] = __builtin_amdgcn_mfma_f32_16x16x4f32(zda, w[st][h], A2[h], 0, 0, 0);
                 }
             }
         };
@@ -437,27 +442,29 @@ __global__ __launch_bounds__(64) void k_bwd2_mfma(Bwd2Args A, const int* __restr
             }
         }
 #pragma unroll
-        for (int mt = 0; mt < NT; ++mt) {
+        for (int h = 0; h < NT; ++h) {
+            const int mt = 2 * h + wv;
             if (mt * 16 >= nprev) break;
             const int m = mt * 16 + c16;
-            const f32x4 a1 = A1[mt], a2 = A2[mt];
+            const f32x4 a1 = A1[h], a2 = A2[h];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const int r = rb + i;
                 const float d = m < nprev ? sdz[((l - 1) * kB2Tile + r) * WP + m] : 0.0f,
                             qq = m < nprev ? sqz[((l - 1) * kB2Tile + r) * WP + m] : 0.0f;
-                nb[mt][i] = a1[i] * d + a2[i] * qq;
-                ndb[mt][i] = a2[i] * d;
+                nb[h][i] = a1[i] * d + a2[i] * qq;
+                ndb[h][i] = a2[i] * d;
             }
         }
         __syncthreads();
 #pragma unroll
-        for (int mt = 0; mt < NT; ++mt) {
+        for (int h = 0; h < NT; ++h) {
+            const int mt = 2 * h + wv;
             if (mt * 16 >= nprev) break;
             const int m = mt * 16 + c16;
 #pragma unroll
             for (int i = 0; i < 4; ++i)
-                if (m < nprev) { sx[(rb + i) * KP + m] = nb[mt][i]; sxd[(rb + i) * KP + m] = ndb[mt][i]; }
+                if (m < nprev) { sx[(rb + i) * KP + m] = nb[h][i]; sxd[(rb + i) * KP + m] = ndb[h][i]; }
         }
         __syncthreads();
     }
@@ -694,7 +701,7 @@ hipError_t net_bwd2(ann_ctx* ctx, const float* G, const float* v, const double* 
         A.nlh = nlh;
         const size_t smem = sizeof(float) * (2 * (size_t)kB2Tile * A.kp + 2 * (size_t)nlh * kB2Tile * A.wp);
         if (ctx->ntiles > 0)
-            k_bwd2_mfma<<<ctx->ntiles, 64, smem, ctx->stream>>>(A, ctx->d_perm, reinterpret_cast<const int2*>(ctx->d_tiles),
+            k_bwd2_mfma<<<ctx->ntiles, 128, smem, ctx->stream>>>(A, ctx->d_perm, reinterpret_cast<const int2*>(ctx->d_tiles),
                                                                ctx->d_spend_end);
     } else {
         Launch L(ctx, "net_bwd2");
