@@ -569,8 +569,17 @@ __global__ __launch_bounds__(64) void k_grad_mfma(GradArgs A, int nparams) {
 __global__ void k_grad_combine(int nparams, int nsplit, const double* __restrict__ part, double* __restrict__ grad) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= nparams) return;
+    // the split partials are independent loads: issue them in groups, add in split order
     double s = 0.0;
-    for (int k = 0; k < nsplit; ++k) s += part[(size_t)k * nparams + p];
+    int k = 0;
+    for (; k + 8 <= nsplit; k += 8) {
+        double v[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) v[q] = part[(size_t)(k + q) * nparams + p];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) s += v[q];
+    }
+    for (; k < nsplit; ++k) s += part[(size_t)k * nparams + p];
     grad[p] = s;
 }
 

@@ -221,3 +221,20 @@ def test_autotune_picks_valid_settings(cases, oracle_mod):
     E, F, _ = eng.energy_forces(1)
     Eo, Fo, _ = oracle_mod.Oracle(model).compute_batch(batch, nthreads=4)
     assert np.sqrt(((F.cpu().numpy() - Fo) ** 2).sum() / (Fo ** 2).sum()) <= 1e-5
+
+
+def test_profile_only_times_one_family(cases):
+    """Kernel timing restricted to one family records that family only."""
+    from ann_amd.gpu import FAST
+    model, batch = cases["water4"]
+    eng = _engine(model, batch)
+    eng.energy_forces(FAST)
+    eng.profile(True)
+    eng.profile_only("symm_fwd_mom")
+    eng.energy_forces(FAST)
+    assert eng.profile_get("symm_fwd_mom")[1] == 1
+    assert eng.profile_get("net_mfma")[1] == 0
+    eng.profile_only("")
+    eng.energy_forces(FAST)
+    assert eng.profile_get("net_mfma")[1] == 1
+    eng.profile(False)
