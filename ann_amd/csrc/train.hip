@@ -515,28 +515,35 @@ __global__ __launch_bounds__(64) void k_grad_mfma(GradArgs A, int nparams) {
             for (int q = 0; q < 4; ++q) dacc[u][v][q] = 0.0;
         }
     int steps = 0;
-    for (int r = r0; r < r1; r += 4) {
-        const int row = r + k4;
-        const bool ok = row < r1;
-        const float* rr = base + (int64_t)(ok ? row : r0) * stride;
-        float z[2], zd[2], x[2], xd[2];
+    // four k steps (16 atoms) per trip: every load of the trip is issued before its products
+    for (int r = r0; r < r1; r += 16) {
+        float z[4][2], zd[4][2], x[4][2], xd[4][2];
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            const bool okn = ok && nA[u] < no;
-            z[u] = okn ? rr[2 * ni + nA[u]] : 0.0f;
-            zd[u] = okn ? rr[2 * ni + no + nA[u]] : 0.0f;
-            const int m = mB[u];
-            x[u] = ok ? (m < ni ? rr[m] : (m == ni ? 1.0f : 0.0f)) : 0.0f;
-            xd[u] = ok && m < ni ? rr[ni + m] : 0.0f;
+        for (int t = 0; t < 4; ++t) {
+            const int row = r + 4 * t + k4;
+            const bool ok = row < r1;
+            const float* rr = base + (int64_t)(ok ? row : r0) * stride;
+#pragma unroll
+            for (int u = 0; u < 2; ++u) {
+                const bool okn = ok && nA[u] < no;
+                z[t][u] = okn ? rr[2 * ni + nA[u]] : 0.0f;
+                zd[t][u] = okn ? rr[2 * ni + no + nA[u]] : 0.0f;
+                const int m = mB[u];
+                x[t][u] = ok ? (m < ni ? rr[m] : (m == ni ? 1.0f : 0.0f)) : 0.0f;
+                xd[t][u] = ok && m < ni ? rr[ni + m] : 0.0f;
+            }
         }
 #pragma unroll
-        for (int u = 0; u < 2; ++u)
+        for (int t = 0; t < 4; ++t)
 #pragma unroll
-            for (int v = 0; v < 2; ++v) {
-                acc[u][v] = __builtin_amdgcn_mfma_f32_16x16x4f32(z[u], x[v], acc[u][v], 0, 0, 0);
-                acc[u][v] = __builtin_amdgcn_mfma_f32_16x16x4f32(zd[u], xd[v], acc[u][v], 0, 0, 0);
-            }
-        if (++steps == 16) {
+            for (int u = 0; u < 2; ++u)
+#pragma unroll
+                for (int v = 0; v < 2; ++v) {
+                    acc[u][v] = __builtin_amdgcn_mfma_f32_16x16x4f32(z[t][u], x[t][v], acc[u][v], 0, 0, 0);
+                    acc[u][v] = __builtin_amdgcn_mfma_f32_16x16x4f32(zd[t][u], xd[t][v], acc[u][v], 0, 0, 0);
+                }
+        steps += 4;
+        if (steps == 16) {
             steps = 0;
 #pragma unroll
             for (int u = 0; u < 2; ++u)
