@@ -292,9 +292,20 @@ __device__ __forceinline__ int fast_load(const SymmArgs& A, const DevModel& M, i
     int cnt[kMaxSpecies];
 #pragma unroll
     for (int b = 0; b < kMaxSpecies; ++b) cnt[b] = 0;
+    // the first two rows of 64 neighbours stay in registers for the second pass
+    float4 g0 = make_float4(0.f, 0.f, 0.f, 0.f), g1 = g0;
+    int p0 = -1, p1 = -1;
     for (int q = 0; q < n; q += kWave) {
         const int t = q + lane;
-        const int sp = (t < n && A.geo[o + t].w <= rc) ? (int)A.nsp[o + t] : -1;
+        float4 gq = make_float4(0.f, 0.f, 0.f, 0.f);
+        int raw = -1;
+        if (t < n) {
+            gq = A.geo[o + t];
+            raw = (int)A.nsp[o + t];
+        }
+        if (q == 0) { g0 = gq; p0 = raw; }
+        if (q == kWave) { g1 = gq; p1 = raw; }
+        const int sp = (t < n && gq.w <= rc) ? raw : -1;
 #pragma unroll
         for (int b = 0; b < kMaxSpecies; ++b)
             if (b < S) cnt[b] += __popcll(__ballot(sp == b));
@@ -316,9 +327,16 @@ __device__ __forceinline__ int fast_load(const SymmArgs& A, const DevModel& M, i
     const uint64_t lt = (1ull << lane) - 1ull;
     for (int q = 0; q < n; q += kWave) {
         const int t = q + lane;
-        const float4 gq = t < n ? A.geo[o + t] : make_float4(0.f, 0.f, 0.f, 0.f);
+        float4 gq;
+        int raw;
+        if (q == 0) { gq = g0; raw = p0; }
+        else if (q == kWave) { gq = g1; raw = p1; }
+        else {
+            gq = t < n ? A.geo[o + t] : make_float4(0.f, 0.f, 0.f, 0.f);
+            raw = t < n ? (int)A.nsp[o + t] : -1;
+        }
         const bool keep = t < n && gq.w <= rc;
-        const int sp = keep ? (int)A.nsp[o + t] : -1;
+        const int sp = keep ? raw : -1;
         if (zero_P && t < n && !keep) {
             A.P[3 * ((size_t)o + t)] = 0.0f;
             A.P[3 * ((size_t)o + t) + 1] = 0.0f;
