@@ -835,10 +835,31 @@ __global__ __launch_bounds__(256) void k_moment(SymmArgs A) {
     float* wq = part + (NG > 1 ? NG * NM * U4 * (MODE == 2 ? 2 : 1) : 0);   // [NM] monomial weights
     sm.fg = reinterpret_cast<float2*>(pw);   // cutoffs live in the power rows until the radial part is done
 
+    // adjoint: the dE/dG row and the stored moments are loaded before the neighbours so
+    // that their latency overlaps the neighbour load (registers, written to LDS below)
+    constexpr int kPreT = 8, kPreD = 2;                         // moments / dE/dG prefetched per lane
+    float pre_d[kPreD], pre_t[kPreT];
+    if (MODE == 1) {
+#pragma unroll
+        for (int h = 0; h < kPreD; ++h) {
+            const int m = lane + h * kWave;
+            pre_d[h] = m < ldg ? A.dEdG[(size_t)i * ldg + m] : 0.0f;
+        }
+#pragma unroll
+        for (int h = 0; h < kPreT; ++h) {
+            const int q = lane + h * kWave;
+            pre_t[h] = (haveT && q < TS) ? A.Tin[(size_t)i * TS + q] : 0.0f;
+        }
+    }
     n = fast_load(A, Mo, i, a, o, n, sm, MODE != 0, MODE == 1);
     if (lane < NM) wq[lane] = c_mono.w[lane];
     if (MODE == 1) {
-        for (int m = lane; m < ldg; m += kWave) rowbuf[m] = A.dEdG[(size_t)i * ldg + m];
+#pragma unroll
+        for (int h = 0; h < kPreD; ++h) {
+            const int m = lane + h * kWave;
+            if (m < ldg) rowbuf[m] = pre_d[h];
+        }
+        for (int m = kPreD * kWave + lane; m < ldg; m += kWave) rowbuf[m] = A.dEdG[(size_t)i * ldg + m];
         for (int t = lane; t < 3 * n; t += kWave) sP[t] = 0.0f;
     } else {
         for (int m = lane; m < ldg; m += kWave) rowbuf[m] = 0.0f;
@@ -857,7 +878,12 @@ __global__ __launch_bounds__(256) void k_moment(SymmArgs A) {
     fast_radial<MODE>(A, Mo, a, sm, tv, rowbuf, sP, rowbuf);
     wave_sync();
     if (haveT) {
-        for (int q = lane; q < TS; q += kWave) T[q] = A.Tin[(size_t)i * TS + q];
+#pragma unroll
+        for (int h = 0; h < kPreT; ++h) {
+            const int q = lane + h * kWave;
+            if (q < TS) T[q] = pre_t[h];
+        }
+        for (int q = kPreT * kWave + lane; q < TS; q += kWave) T[q] = A.Tin[(size_t)i * TS + q];
     }
     // powers of the unit vector (and, for the tangent, their derivatives along
     // the projected direction (t - r^ (r^.t)) / r)
