@@ -286,7 +286,7 @@ __device__ __forceinline__ float pow_m1(float w, int zi, float zeta) {
 // cutoff (a list built with rc + skin) are dropped here; their adjoint slots
 // are written as zero when zero_P is set.  Returns the number kept.
 __device__ __forceinline__ int fast_load(const SymmArgs& A, const DevModel& M, int i, int a, int o, int n, FastSmem& sm,
-                         bool need_deriv, bool zero_P) {
+                         bool need_deriv, bool zero_P, float4* tv = nullptr) {
     const int lane = threadIdx.x % kWave, S = M.S, U = M.nunion[a];
     const float rc = M.rc;
     int cnt[kMaxSpecies];
@@ -294,17 +294,18 @@ __device__ __forceinline__ int fast_load(const SymmArgs& A, const DevModel& M, i
     for (int b = 0; b < kMaxSpecies; ++b) cnt[b] = 0;
     // the first two rows of 64 neighbours stay in registers for the second pass
     float4 g0 = make_float4(0.f, 0.f, 0.f, 0.f), g1 = g0;
-    int p0 = -1, p1 = -1;
+    int p0 = -1, p1 = -1, j0 = 0, j1 = 0;
     for (int q = 0; q < n; q += kWave) {
         const int t = q + lane;
         float4 gq = make_float4(0.f, 0.f, 0.f, 0.f);
-        int raw = -1;
+        int raw = -1, jr = 0;
         if (t < n) {
             gq = A.geo[o + t];
             raw = (int)A.nsp[o + t];
+            if (tv) jr = A.nidx[o + t];
         }
-        if (q == 0) { g0 = gq; p0 = raw; }
-        if (q == kWave) { g1 = gq; p1 = raw; }
+        if (q == 0) { g0 = gq; p0 = raw; j0 = jr; }
+        if (q == kWave) { g1 = gq; p1 = raw; j1 = jr; }
         const int sp = (t < n && gq.w <= rc) ? raw : -1;
 #pragma unroll
         for (int b = 0; b < kMaxSpecies; ++b)
@@ -325,15 +326,23 @@ __device__ __forceinline__ int fast_load(const SymmArgs& A, const DevModel& M, i
 #pragma unroll
     for (int u = 0; u < kFastU; ++u) ie[u] = u < U ? M.uieta2[a][u] : 0.0f;
     const uint64_t lt = (1ull << lane) - 1ull;
+    // tangent: the centre's displacement (the same for every lane)
+    float uix = 0.f, uiy = 0.f, uiz = 0.f;
+    if (tv) {
+        uix = A.u[3 * i];
+        uiy = A.u[3 * i + 1];
+        uiz = A.u[3 * i + 2];
+    }
     for (int q = 0; q < n; q += kWave) {
         const int t = q + lane;
         float4 gq;
-        int raw;
-        if (q == 0) { gq = g0; raw = p0; }
-        else if (q == kWave) { gq = g1; raw = p1; }
+        int raw, jr;
+        if (q == 0) { gq = g0; raw = p0; jr = j0; }
+        else if (q == kWave) { gq = g1; raw = p1; jr = j1; }
         else {
             gq = t < n ? A.geo[o + t] : make_float4(0.f, 0.f, 0.f, 0.f);
             raw = t < n ? (int)A.nsp[o + t] : -1;
+            jr = (tv && t < n) ? A.nidx[o + t] : 0;
         }
         const bool keep = t < n && gq.w <= rc;
         const int sp = keep ? raw : -1;
@@ -354,6 +363,11 @@ __device__ __forceinline__ int fast_load(const SymmArgs& A, const DevModel& M, i
         if (keep) {
             const float r = gq.w, ir = __builtin_amdgcn_rcpf(r);   // 1 ulp
             sm.g[pos] = make_float4(gq.x * ir, gq.y * ir, gq.z * ir, ir);
+            if (tv) {
+                // tangent direction t = u_i - u_j and its projection on the unit vector
+                const float tx = uix - A.u[3 * jr], ty = uiy - A.u[3 * jr + 1], tz = uiz - A.u[3 * jr + 2];
+                tv[pos] = make_float4(tx, ty, tz, (gq.x * ir) * tx + (gq.y * ir) * ty + (gq.z * ir) * tz);
+            }
             sm.r[pos] = r;
             sm.canon[pos] = t;
             float f, gcut;
@@ -851,7 +865,7 @@ __global__ __launch_bounds__(256) void k_moment(SymmArgs A) {
             pre_t[h] = (haveT && q < TS) ? A.Tin[(size_t)i * TS + q] : 0.0f;
         }
     }
-    n = fast_load(A, Mo, i, a, o, n, sm, MODE != 0, MODE == 1);
+    n = fast_load(A, Mo, i, a, o, n, sm, MODE != 0, MODE == 1, MODE == 2 ? tv : nullptr);
     if (lane < NM) wq[lane] = c_mono.w[lane];
     if (MODE == 1) {
 #pragma unroll
@@ -865,16 +879,6 @@ __global__ __launch_bounds__(256) void k_moment(SymmArgs A) {
         for (int m = lane; m < ldg; m += kWave) rowbuf[m] = 0.0f;
     }
     wave_sync();
-    if (MODE == 2) {
-        for (int J = lane; J < n; J += kWave) {
-            const int j = A.nidx[o + sm.canon[J]];
-            const float tx = A.u[3 * i] - A.u[3 * j], ty = A.u[3 * i + 1] - A.u[3 * j + 1],
-                        tz = A.u[3 * i + 2] - A.u[3 * j + 2];
-            const float4 gq = sm.g[J];
-            tv[J] = make_float4(tx, ty, tz, gq.x * tx + gq.y * ty + gq.z * tz);
-        }
-        wave_sync();
-    }
     fast_radial<MODE>(A, Mo, a, sm, tv, rowbuf, sP, rowbuf);
     wave_sync();
     if (haveT) {
