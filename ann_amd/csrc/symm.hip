@@ -639,6 +639,34 @@ __device__ __forceinline__ void radial_gauss(const SymmArgs& A, const RadBlock& 
     }
 }
 
+// GAUSSIAN block of at most 16 functions over a short species segment (G and
+// tangent): lanes over (neighbour, function) pairs, W = 8 or 16 functions per
+// lane group and 64 / W neighbours per pass, so every lane evaluates one
+// exponential per pass and the sum over neighbours is log2(64 / W) butterflies.
+// With few neighbours per species this replaces W exponentials per lane on a
+// mostly idle wavefront and a W-value transposing reduction.
+template <int MODE>
+__device__ __forceinline__ void radial_gauss_pairs(const SymmArgs& A, const RadBlock& rb, const FastSmem& sm, int s0, int s1,
+                                                   int W, const float4* tv, float* out) {
+    const int lane = threadIdx.x % kWave;
+    const int m = lane & (W - 1), nj = kWave / W;
+    const bool in = m < rb.n;
+    const float eta = in ? A.rad_eta[rb.par + m] : 0.0f;
+    const float rs = in ? A.rad_rs[rb.par + m] : 0.0f;
+    float acc = 0.0f;
+    for (int J = s0 + lane / W; J < s1; J += nj) {
+        const float r = sm.r[J];
+        const float2 fg = sm.fg[J];
+        const float x = eta * (r - rs), e = __expf(-x * x);
+        acc += MODE == 0 ? e * fg.x : e * (fg.y - 2.0f * eta * x * fg.x) * tv[J].w;
+    }
+    for (int bit = W; bit < kWave; bit <<= 1) acc += __shfl_xor(acc, bit);
+    if (lane < W && in) out[rb.goff + m] = acc;
+}
+
+// passes of the pair form above which the lane-over-neighbour form is used
+constexpr int kPairPasses = 4;
+
 template <int MODE>
 __device__ __forceinline__ void fast_radial(const SymmArgs& A, const DevModel& M, int a, const FastSmem& sm, const float4* tv,
                             const float* sd, float* sP, float* out) {
@@ -647,6 +675,13 @@ __device__ __forceinline__ void fast_radial(const SymmArgs& A, const DevModel& M
         const RadBlock rb = M.rad[a * S + b];
         if (rb.n == 0) continue;
         const int s0 = sm.seg[b], s1 = sm.seg[b + 1];
+        if (rb.name == RAD_GAUSSIAN && MODE != 1 && rb.n <= 16) {
+            const int W = rb.n <= 8 ? 8 : 16;
+            if ((s1 - s0) * W <= kPairPasses * kWave) {
+                radial_gauss_pairs<MODE>(A, rb, sm, s0, s1, W, tv, out);
+                continue;
+            }
+        }
         if (rb.name == RAD_GAUSSIAN) {
             // eight per pass for blocks of at most eight functions, else sixteen
             // (the tangent keeps sixteen: a second copy raises its register count)
