@@ -639,12 +639,12 @@ __device__ __forceinline__ void radial_gauss(const SymmArgs& A, const RadBlock& 
     }
 }
 
-// GAUSSIAN block of at most 16 functions over a short species segment (G and
-// tangent): lanes over (neighbour, function) pairs, W = 8 or 16 functions per
-// lane group and 64 / W neighbours per pass, so every lane evaluates one
-// exponential per pass and the sum over neighbours is log2(64 / W) butterflies.
-// With few neighbours per species this replaces W exponentials per lane on a
-// mostly idle wavefront and a W-value transposing reduction.
+// GAUSSIAN block of at most 16 functions (G and tangent): lanes over
+// (neighbour, function) pairs, W = 8 or 16 functions per lane group and 64 / W
+// neighbours per pass, so every lane evaluates one exponential per pass and the
+// sum over neighbours is log2(64 / W) butterflies.  Against W exponentials per
+// lane on a partly idle wavefront and a W-value transposing reduction it wins
+// up to about 32 passes (water: 60 neighbours of one species, 15 passes).
 template <int MODE>
 __device__ __forceinline__ void radial_gauss_pairs(const SymmArgs& A, const RadBlock& rb, const FastSmem& sm, int s0, int s1,
                                                    int W, const float4* tv, float* out) {
@@ -665,7 +665,7 @@ __device__ __forceinline__ void radial_gauss_pairs(const SymmArgs& A, const RadB
 }
 
 // passes of the pair form above which the lane-over-neighbour form is used
-constexpr int kPairPasses = 4;
+constexpr int kPairPasses = 32;
 
 template <int MODE>
 __device__ __forceinline__ void fast_radial(const SymmArgs& A, const DevModel& M, int a, const FastSmem& sm, const float4* tv,
