@@ -667,6 +667,9 @@ __device__ __forceinline__ void radial_gauss_pairs(const SymmArgs& A, const RadB
 // passes of the pair form above which the lane-over-neighbour form is used
 constexpr int kPairPasses = 32;
 
+// the moment kernels index block widths by union width: both bounds must agree
+static_assert(kMaxUnionEta == kGridG, "union widths and block widths share one index");
+
 template <int MODE>
 __device__ __forceinline__ void fast_radial(const SymmArgs& A, const DevModel& M, int a, const FastSmem& sm, const float4* tv,
                             const float* sd, float* sP, float* out) {
@@ -1051,15 +1054,30 @@ __global__ __launch_bounds__(256) void k_moment(SymmArgs A) {
     }
     // block contractions (modes 0, 2) or their adjoint seeds (mode 1)
     if (MODE == 1) {
-        for (int p = 0; p < NP; ++p) {
+        // lanes over (pair, width, power) of every block at once; each lane loads
+        // all of its output indices and coefficients before using any, so the
+        // block descriptors cost a few rounds of memory latency, not one per term
+        for (int it = lane; it < NP * U4 * ZP; it += kWave) {
+            const int p = it / (U4 * ZP), u = (it / ZP) % U4, m = it % ZP;
             const AngBlock& B = Mo.ang[a * NP + p];
-            for (int it = lane; it < U4 * ZP; it += kWave) {
-                const int u = it / ZP, m = it % ZP;
-                float s = 0.0f;
-                if (B.n > 0 && u < B.ng)
-                    for (int k = 0; k < B.nk; ++k) s += rowbuf[B.goff + B.outm[u * B.nk + k]] * B.mcoef[k][m];
-                Am[p * U4 * ZP + it] = s;
+            const int nk = min(max(B.nk, 0), kGridK), goff = B.goff;
+            const bool use = B.n > 0 && u < B.ng;
+            // u * nk + k < kGridG * kGridK for every k < kGridK (u < kGridG, nk <= kGridK):
+            // entries past nk are read but not used
+            int om[kGridK];
+            float mc[kGridK];
+#pragma unroll
+            for (int k = 0; k < kGridK; ++k) {
+                om[k] = B.outm[u * nk + k];
+                mc[k] = B.mcoef[k][m];
             }
+            float s = 0.0f;
+            if (use) {
+#pragma unroll
+                for (int k = 0; k < kGridK; ++k)
+                    if (k < nk) s += rowbuf[goff + om[k]] * mc[k];
+            }
+            Am[it] = s;
         }
     }
     for (int it = lane; MODE != 1 && it < NP * U4 * ZP; it += kWave) {
@@ -1084,14 +1102,22 @@ __global__ __launch_bounds__(256) void k_moment(SymmArgs A) {
     }
     wave_sync();
     if (MODE != 1) {
-        for (int p = 0; p < NP; ++p) {
+        // lanes over (pair, width, combination) of every block at once, loads
+        // issued before the guards that use them
+        for (int it = lane; it < NP * kGridG * kGridK; it += kWave) {
+            const int p = it / (kGridG * kGridK), g = (it / kGridK) % kGridG, k = it % kGridK;
             const AngBlock& B = Mo.ang[a * NP + p];
-            for (int q = lane; q < B.ng * B.nk; q += kWave) {
-                const int g = q / B.nk, k = q % B.nk;
+            const int nk = min(max(B.nk, 0), kGridK), goff = B.goff;
+            const bool use = B.n > 0 && g < B.ng && k < nk;
+            const int om = B.outm[g * nk + k];   // < kGridG * kGridK; used only when use
+            float mc[ZP];
+#pragma unroll
+            for (int m = 0; m < ZP; ++m) mc[m] = B.mcoef[k][m];
+            if (use) {
                 float v = 0.0f;
 #pragma unroll
-                for (int m = 0; m < ZP; ++m) v += B.mcoef[k][m] * Am[(p * U4 + g) * ZP + m];
-                rowbuf[B.goff + B.outm[q]] = v;
+                for (int m = 0; m < ZP; ++m) v += mc[m] * Am[(p * U4 + g) * ZP + m];
+                rowbuf[goff + om] = v;
             }
         }
         wave_sync();
