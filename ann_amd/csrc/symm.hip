@@ -951,10 +951,16 @@ __global__ __launch_bounds__(256) void k_moment(SymmArgs A) {
         }
     }
     wave_sync();
-    // moments T_b[u][alpha]: lane = (alpha, group); groups stride the neighbours
+    // moments T_b[u][alpha]: lane = (alpha, group); groups stride the neighbours.
+    // G with zeta up to 4 (35 monomials, one group would leave 29 lanes idle):
+    // monomials 1, x, y, z are summed with the diagonal sums below and the other
+    // 31 run as two groups of 32 lanes, joined by one cross-half exchange.
+    constexpr bool SPLIT = MODE == 0 && NM == 35;
+    constexpr int NGL = SPLIT ? 2 : NG;                        // neighbour groups of the loop
     if (!haveT) {
-        const int al = lane % NM, grp = lane / NM;
-        const bool act = grp < NG;
+        const int al = SPLIT ? min(4 + lane % 32, NM - 1) : lane % NM;
+        const int grp = SPLIT ? lane / 32 : lane / NM;
+        const bool act = SPLIT ? lane % 32 < 31 : grp < NG;
         const int ea = c_mono.a[al], eb = c_mono.b[al], ec = c_mono.c[al];
         for (int b = 0; b < S; ++b) {
             const int s0 = sm.seg[b], s1 = sm.seg[b + 1];
@@ -982,16 +988,23 @@ __global__ __launch_bounds__(256) void k_moment(SymmArgs A) {
                 int J = s0 + grp;
                 // four neighbours per trip: their power rows sit at fixed offsets
                 // from one address, so the loads use immediate offsets and overlap
-                for (; J + 3 * NG < s1; J += 4 * NG) {
+                for (; J + 3 * NGL < s1; J += 4 * NGL) {
                     const float* row = pw + (size_t)J * PWS;
                     step(row, J);
-                    step(row + NG * PWS, J + NG);
-                    step(row + 2 * NG * PWS, J + 2 * NG);
-                    step(row + 3 * NG * PWS, J + 3 * NG);
+                    step(row + NGL * PWS, J + NGL);
+                    step(row + 2 * NGL * PWS, J + 2 * NGL);
+                    step(row + 3 * NGL * PWS, J + 3 * NGL);
                 }
-                for (; J < s1; J += NG) step(pw + (size_t)J * PWS, J);
+                for (; J < s1; J += NGL) step(pw + (size_t)J * PWS, J);
             }
-            if (NG == 1) {
+            if (SPLIT) {
+#pragma unroll
+                for (int u = 0; u < U4; ++u) acc[u] += __shfl_xor(acc[u], 32);
+                if (act && grp == 0) {
+#pragma unroll
+                    for (int u = 0; u < U4; ++u) T[(b * U4 + u) * NM + al] = acc[u];
+                }
+            } else if (NG == 1) {
                 if (act) {
 #pragma unroll
                     for (int u = 0; u < U4; ++u) {
@@ -1028,9 +1041,23 @@ __global__ __launch_bounds__(256) void k_moment(SymmArgs A) {
                 float d4[U4], t4[U4];
 #pragma unroll
                 for (int u = 0; u < U4; ++u) { d4[u] = 0.0f; t4[u] = 0.0f; }
+                float lo[SPLIT ? 4 * U4 : 1];                  // [u][1, x, y, z] (split G)
+#pragma unroll
+                for (int q = 0; q < (SPLIT ? 4 * U4 : 1); ++q) lo[q] = 0.0f;
                 for (int J = sm.seg[b] + lane; J < sm.seg[b + 1]; J += kWave) {
                     const float4 e4 = sm.ef[J];
                     d4[0] += e4.x * e4.x; d4[1] += e4.y * e4.y; d4[2] += e4.z * e4.z; d4[3] += e4.w * e4.w;
+                    if (SPLIT) {
+                        const float4 gq = sm.g[J];
+                        const float ev[U4] = {e4.x, e4.y, e4.z, e4.w};
+#pragma unroll
+                        for (int u = 0; u < U4; ++u) {
+                            lo[4 * u] += ev[u];
+                            lo[4 * u + 1] += ev[u] * gq.x;
+                            lo[4 * u + 2] += ev[u] * gq.y;
+                            lo[4 * u + 3] += ev[u] * gq.z;
+                        }
+                    }
                     if (MODE == 2) {
                         const float4 f4 = sm.efd[J];
                         const float w2 = 2.0f * tv[J].w;
@@ -1041,6 +1068,12 @@ __global__ __launch_bounds__(256) void k_moment(SymmArgs A) {
                 int idx;
                 const float sd = wave_sum_multi<U4>(d4, idx);
                 if ((lane & (kWave / U4 - 1)) == 0) D[b * U4 + idx] = sd;
+                if (SPLIT) {
+                    // monomial order starts 1, x, y, z (degree 0, then x, y, z)
+                    int qi;
+                    const float sl = wave_sum_multi<(SPLIT ? 4 * U4 : 1)>(lo, qi);
+                    if ((lane & (kWave / (4 * U4) - 1)) == 0) T[(b * U4 + qi / 4) * NM + qi % 4] = sl;
+                }
                 if (MODE == 2) {
                     const float st = wave_sum_multi<U4>(t4, idx);
                     if ((lane & (kWave / U4 - 1)) == 0) dD[b * U4 + idx] = st;
