@@ -55,7 +55,7 @@ EXPORTS = ["ann_last_error", "ann_create", "ann_destroy", "ann_set_stream", "ann
            "ann_symm_fwd", "ann_net_fwd", "ann_symm_bwd", "ann_force_reduce", "ann_energy_forces",
            "ann_symm_tangent", "ann_net_bwd2", "ann_train_step", "ann_get_params", "ann_profile",
            "ann_profile_get", "ann_profile_only", "ann_refresh_geometry", "ann_wrap_positions", "ann_md_step", "ann_train_graph", "ann_ef_graph",
-           "ann_autotune", "ann_get_tuning", "ann_set_tuning"]
+           "ann_autotune", "ann_get_tuning", "ann_set_tuning", "ann_input_stats"]
 
 
 def lib():
@@ -105,6 +105,7 @@ class Engine:
         S, nin, ldg, npar = C.c_int(), C.c_int(), C.c_int(), C.c_int()
         _check(lib().ann_info(self.h, C.byref(S), C.byref(nin), C.byref(ldg), C.byref(npar)))
         self.ldg, self.max_nin, self.nparams = ldg.value, nin.value, npar.value
+        self.nspecies = S.value
         self.batch: Optional[Batch] = None
 
     def __del__(self):
@@ -124,6 +125,21 @@ class Engine:
         p = np.zeros(self.nparams)
         _check(lib().ann_get_params(self.h, p.ctypes.data_as(P_D), self.nparams))
         return p
+
+    def input_stats(self, G: Optional[torch.Tensor] = None):
+        """Per-species input statistics of the current batch (ann_input_stats).
+
+        Returns (count (S,), stats (S, 6, max_ninput)) with stats[:, q] = min, max,
+        harmonic mean, mean, deviation, rms; G defaults to the FAST G of the batch.
+        """
+        if G is None:
+            G = self.symm(FAST)
+        G = G.contiguous()
+        torch.cuda.synchronize(self.device)
+        cnt = np.zeros(self.nspecies, dtype=np.int32)
+        st = np.zeros((self.nspecies, 6, self.max_nin))
+        _check(lib().ann_input_stats(self.h, _ptr(G), cnt.ctypes.data_as(P_I), st.ctypes.data_as(P_D), st.size))
+        return cnt, st
 
     def set_scaling(self, species: int, inpw, inpb, outw: float = 1.0, outb: float = 0.0) -> None:
         w = np.ascontiguousarray(inpw, dtype=np.float64)

@@ -197,6 +197,14 @@ int ann_train_graph(ann_ctx* ctx, int enable);
 int ann_ef_graph(ann_ctx* ctx, int enable);
 /* copy the FP64 master parameters (device) to host */
 int ann_get_params(ann_ctx* ctx, double* params, int n);
+/* input preconditioning statistics of the current batch (replaces the loops of
+ * NNPTEFR::train, src/nnp/nnptefr2.cpp:690-787, over the training structures'
+ * symmetry functions): d_G holds N x ldg FP32 rows as written by ann_symm_fwd.
+ * Writes count[S] atoms per species and, on the host, stats[S][6][max_ninput]:
+ * min, max, harmonic mean N / sum(1/G), mean, deviation sqrt(sum (mean - G)^2 /
+ * (N - 1)) and rms sqrt(sum G^2 / N) per input; n = S * 6 * max_ninput.  Sums
+ * are FP64 in a fixed order; synchronous. */
+int ann_input_stats(ann_ctx* ctx, const float* d_G, int* count, double* stats, int n);
 
 /* ---- tuning ----
  * ann_autotune times the variants on the current batch and keeps the fastest
