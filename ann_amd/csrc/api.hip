@@ -404,6 +404,26 @@ extern "C" int ann_get_params(ann_ctx* ctx, double* p, int n) {
     return ANN_OK;
 }
 
+extern "C" int ann_get_optimizer_state(ann_ctx* ctx, double* m, double* v, int* step, int n) {
+    CHECK_CTX(ctx);
+    if (!m || !v || !step || n != ctx->nparams) return fail(ANN_ERR_ARG, "optimizer state: count mismatch");
+    ANN_HIP(hipMemcpyAsync(m, ctx->da.adam_m, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+    ANN_HIP(hipMemcpyAsync(v, ctx->da.adam_v, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+    ANN_HIP(hipMemcpyAsync(step, ctx->d_adam_t, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    ANN_HIP(hipStreamSynchronize(ctx->stream));
+    return ANN_OK;
+}
+
+extern "C" int ann_set_optimizer_state(ann_ctx* ctx, const double* m, const double* v, int step, int n) {
+    CHECK_CTX(ctx);
+    if (!m || !v || n != ctx->nparams || step < 0) return fail(ANN_ERR_ARG, "optimizer state: bad arguments");
+    ANN_HIP(hipMemcpyAsync(ctx->da.adam_m, m, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+    ANN_HIP(hipMemcpyAsync(ctx->da.adam_v, v, sizeof(double) * n, hipMemcpyHostToDevice, ctx->stream));
+    ANN_HIP(hipMemcpyAsync(ctx->d_adam_t, &step, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    ANN_HIP(hipStreamSynchronize(ctx->stream));
+    return ANN_OK;
+}
+
 extern "C" int ann_set_scaling(ann_ctx* ctx, int a, const double* inpw, const double* inpb, double outw,
                                double outb) {
     CHECK_CTX(ctx);

@@ -55,7 +55,8 @@ EXPORTS = ["ann_last_error", "ann_create", "ann_destroy", "ann_set_stream", "ann
            "ann_symm_fwd", "ann_net_fwd", "ann_symm_bwd", "ann_force_reduce", "ann_energy_forces",
            "ann_symm_tangent", "ann_net_bwd2", "ann_train_step", "ann_get_params", "ann_profile",
            "ann_profile_get", "ann_profile_only", "ann_refresh_geometry", "ann_wrap_positions", "ann_md_step", "ann_train_graph", "ann_ef_graph",
-           "ann_autotune", "ann_get_tuning", "ann_set_tuning", "ann_input_stats"]
+           "ann_autotune", "ann_get_tuning", "ann_set_tuning", "ann_input_stats",
+           "ann_get_optimizer_state", "ann_set_optimizer_state"]
 
 
 def lib():
@@ -106,6 +107,7 @@ class Engine:
         _check(lib().ann_info(self.h, C.byref(S), C.byref(nin), C.byref(ldg), C.byref(npar)))
         self.ldg, self.max_nin, self.nparams = ldg.value, nin.value, npar.value
         self.nspecies = S.value
+        self._scaling = {}
         self.batch: Optional[Batch] = None
 
     def __del__(self):
@@ -141,9 +143,31 @@ class Engine:
         _check(lib().ann_input_stats(self.h, _ptr(G), cnt.ctypes.data_as(P_I), st.ctypes.data_as(P_D), st.size))
         return cnt, st
 
+    def get_optimizer_state(self):
+        """(first moments, second moments, steps taken) of Adam."""
+        m, v, t = np.zeros(self.nparams), np.zeros(self.nparams), C.c_int()
+        _check(lib().ann_get_optimizer_state(self.h, m.ctypes.data_as(P_D), v.ctypes.data_as(P_D), C.byref(t),
+                                             self.nparams))
+        return m, v, t.value
+
+    def set_optimizer_state(self, m, v, step: int) -> None:
+        m = np.ascontiguousarray(m, dtype=np.float64)
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        _check(lib().ann_set_optimizer_state(self.h, m.ctypes.data_as(P_D), v.ctypes.data_as(P_D), int(step),
+                                             len(m)))
+
+    def scaling(self, species: int):
+        """(inpw, inpb, outw, outb) last set for a species, else the model's own."""
+        if species in self._scaling:
+            return self._scaling[species]
+        n = self.model.nets[species]
+        return (np.asarray(n.inpw, dtype=np.float64), np.asarray(n.inpb, dtype=np.float64), float(n.outw),
+                float(n.outb))
+
     def set_scaling(self, species: int, inpw, inpb, outw: float = 1.0, outb: float = 0.0) -> None:
         w = np.ascontiguousarray(inpw, dtype=np.float64)
         b = np.ascontiguousarray(inpb, dtype=np.float64)
+        self._scaling[species] = (w.copy(), b.copy(), float(outw), float(outb))
         _check(lib().ann_set_scaling(self.h, species, w.ctypes.data_as(P_D), b.ctypes.data_as(P_D),
                                      C.c_double(outw), C.c_double(outb)))
 

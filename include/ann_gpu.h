@@ -197,6 +197,12 @@ int ann_train_graph(ann_ctx* ctx, int enable);
 int ann_ef_graph(ann_ctx* ctx, int enable);
 /* copy the FP64 master parameters (device) to host */
 int ann_get_params(ann_ctx* ctx, double* params, int n);
+/* Adam state for a restart (NNPTEFR restart file, src/nnp/nnptefr2.cpp:73-195,
+ * 501-566, holds the optimiser with the potential): first and second moments
+ * (n = nparams doubles each, host) and the number of steps taken.  Set after
+ * ann_set_params, which restarts Adam from zero moments. */
+int ann_get_optimizer_state(ann_ctx* ctx, double* m, double* v, int* step, int n);
+int ann_set_optimizer_state(ann_ctx* ctx, const double* m, const double* v, int step, int n);
 /* input preconditioning statistics of the current batch (replaces the loops of
  * NNPTEFR::train, src/nnp/nnptefr2.cpp:690-787, over the training structures'
  * symmetry functions): d_G holds N x ldg FP32 rows as written by ann_symm_fwd.
