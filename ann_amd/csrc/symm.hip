@@ -1124,9 +1124,16 @@ __global__ __launch_bounds__(256) void k_moment(SymmArgs A) {
             const float* Tc = T + (c * U4 + u) * NM;
             const float* dTb = dT + (b * U4 + u) * NM;
             const float* dTc = dT + (c * U4 + u) * NM;
+            // degree m has (m + 1)(m + 2) / 2 monomials; every lane runs the
+            // length of the highest degree with the surplus terms weighted 0, so
+            // all loads are issued before the first product instead of one
+            // dependent round per monomial
+            constexpr int QD = (Z + 1) * (Z + 2) / 2;
             const int q0 = mono_start(m), q1 = mono_start(m + 1);
-            for (int q = q0; q < q1; ++q) {
-                const float w = wq[q];
+#pragma unroll
+            for (int j = 0; j < QD; ++j) {
+                const int q = min(q0 + j, NM - 1);
+                const float w = q0 + j < q1 ? wq[q] : 0.0f;
                 s += MODE == 0 ? w * Tb[q] * Tc[q] : w * (dTb[q] * Tc[q] + Tb[q] * dTc[q]);
             }
             if (b == c) s = 0.5f * (s - (MODE == 0 ? D[b * U4 + u] : dD[b * U4 + u]));
