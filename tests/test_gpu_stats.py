@@ -38,10 +38,14 @@ def _numpy_stats(G, species, nin, S):
     return out
 
 
-def test_stats_match_numpy_and_oracle(oracle_mod):
+@pytest.mark.parametrize("name", ["lips3", "water2"])
+def test_stats_match_numpy_and_oracle(oracle_mod, name):
     from ann_amd import configs
-    model, batch = configs.model_lips(), configs.batch_lips(3, seed=41)
-    print("case lips3 seed 41", flush=True)
+    if name == "lips3":
+        model, batch = configs.model_lips(), configs.batch_lips(3, seed=41)
+    else:
+        model, batch = configs.model_water(), configs.batch_water(2, seed=45)
+    print(f"case {name}", flush=True)
     eng = _engine(model, batch)
     G = eng.symm().cpu().numpy().astype(np.float64)
     cnt, st = eng.input_stats()
