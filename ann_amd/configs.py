@@ -238,3 +238,23 @@ def small_random(seed: int, natoms: int = 20, nspecies: int = 3, a0: float = 20.
     pos = rng.uniform(-a0, a0, size=(natoms, 3))
     spc = rng.integers(0, nspecies, size=natoms).astype(np.int32)
     return single(pos, np.eye(3) * a0 if cell is None else cell, spc)
+
+
+def model_family(rad: str = "GAUSSIAN", ang: str = "GAUSS", neuron: str = "TANH", cutoff: str = "TANH",
+                 seed: int = 4, nr: int = 8, hidden=(16, 16)) -> Model:
+    """Two-species (H/O) potential on the water structures with one radial and
+    one angular family throughout: nr radial functions per neighbour species
+    (rs 0.8..5.5 A, eta = 1/spacing) and 8 angular per pair, widths {1.5, 3.0} A
+    x zeta {1, 2} x lambda +-1 (a grid: GAUSS runs the moment form)."""
+    rng = np.random.default_rng(seed)
+    rs = np.linspace(0.8, 5.5, nr)
+    radb = (RAD[rad], rs, np.full(nr, 1.0 / (rs[1] - rs[0])))
+    eta = np.repeat([1.5, 3.0], 4)
+    zeta = np.tile([1.0, 1.0, 2.0, 2.0], 2)
+    lam = np.tile([1, -1], 4)
+    angb = (ANG[ang], eta, zeta, lam)
+    radial, angular = _uniform_basis(2, radb, angb)
+    nin = 2 * nr + 3 * 8
+    nets = [init_network(nin, list(hidden), NEURON[neuron], rng) for _ in range(2)]
+    return Model(["H", "O"], np.array([1.008, 15.999]), np.array([-0.5, -2.0]), 6.0, CUT[cutoff],
+                 radial, angular, nets)

@@ -30,9 +30,11 @@ void hipcheck(hipError_t e, const char* what) {
 const char* kCut[] = {"NONE", "STEP", "COS", "TANH", "POLY3"};
 const char* kRad[] = {"NONE", "GAUSSIAN", "SECH", "LOGISTIC", "TANH", "LOGCOSH", "LOGCOSH2"};
 const char* kAng[] = {"NONE", "GAUSS", "GAUSS2", "SECH", "STUDENT3", "STUDENT4", "STUDENT5"};
-const std::map<std::string, int> kNeuron = {{"LINEAR", 1}, {"SIGMOID", 2}, {"TANH", 3}, {"ARCTAN", 5},
-                                            {"RELU", 6},   {"ELU", 7},     {"SWISH", 10}, {"SOFTPLUS", 16},
-                                            {"SQPLUS", 17}};
+// NN::Neuron names and numbers (src/ml/nn.hpp:83-107, src/ml/nn.cpp:66-140)
+const std::map<std::string, int> kNeuron = {
+    {"LINEAR", 1}, {"SIGMOID", 2}, {"TANH", 3},   {"ISRU", 4},    {"ARCTAN", 5},  {"RELU", 6},
+    {"ELU", 7},    {"TANHRE", 8},  {"SQRE", 9},   {"SWISH", 10},  {"GELU", 11},   {"MISH", 12},
+    {"PFLU", 13},  {"SERF", 14},   {"LOGISH", 15}, {"SOFTPLUS", 16}, {"SQPLUS", 17}, {"ATISH", 18}};
 
 int lookup(const char* const* tab, int n, const std::string& s, const char* what) {
     for (int i = 1; i < n; ++i)

@@ -81,6 +81,62 @@ using namespace ann;
 
 extern "C" const char* ann_last_error(void) { return g_err.c_str(); }
 
+// Inputs whose scaled value s = inpw (G + inpb) an FP32 G cannot resolve: with
+// |inpw inpb| = R, the ~1e-7 relative rounding of G becomes ~1e-7 R in s.
+// Those above ctx->pre_ratio are recomputed in FP64 and centred (precise.hip);
+// the network applies inpb_net (inpb with those entries zeroed) to rows whose
+// flagged entries hold G + inpb.  Uploads the FP32 and FP64 scaling, the items
+// and the flags.
+static int update_precise(ann_ctx* ctx) {
+    DevModel& M = ctx->hm;
+    const size_t tot = ctx->h_inpw.size();
+    std::vector<int4> items;
+    std::vector<uint8_t> flag(std::max<size_t>(tot, 1), 0);
+    std::vector<float> iw(tot), ib(tot), ibn(tot);
+    for (int a = 0; a < M.S; ++a) {
+        M.pre_off[a] = (int)items.size();
+        const NetDesc& D = M.net[a];
+        for (int k = 0; k < D.nin; ++k) {
+            const double w = ctx->h_inpw[D.inoff + k], b = ctx->h_inpb[D.inoff + k];
+            iw[D.inoff + k] = (float)w;
+            ib[D.inoff + k] = (float)b;
+            const bool f = ctx->pre_ratio < 0 || std::fabs(w * b) > ctx->pre_ratio;
+            ibn[D.inoff + k] = f ? 0.0f : (float)b;
+            if (!f) continue;
+            flag[D.inoff + k] = 1;
+            int4 it = make_int4(k, -1, 0, 0);
+            for (int c = 0; c < M.S && it.y < 0; ++c) {
+                const RadBlock& rb = M.rad[a * M.S + c];
+                if (k >= rb.goff && k < rb.goff + rb.n) it = make_int4(k, 0, c, k - rb.goff);
+            }
+            for (int p = 0; p < M.NP && it.y < 0; ++p) {
+                const AngBlock& ab = M.ang[a * M.NP + p];
+                if (k >= ab.goff && k < ab.goff + ab.n) it = make_int4(k, 1, p, k - ab.goff);
+            }
+            if (it.y < 0) return fail(ANN_ERR_ARG, "input outside every basis block");
+            items.push_back(it);
+        }
+    }
+    M.pre_off[M.S] = (int)items.size();
+    for (int a = M.S + 1; a <= kMaxSpecies; ++a) M.pre_off[a] = (int)items.size();
+    ctx->npre = (int)items.size();
+    hipStream_t st = ctx->stream;
+    if (tot > 0) {
+        ANN_HIP(hipMemcpyAsync(ctx->da.inpw, iw.data(), sizeof(float) * tot, hipMemcpyHostToDevice, st));
+        ANN_HIP(hipMemcpyAsync(ctx->da.inpb, ib.data(), sizeof(float) * tot, hipMemcpyHostToDevice, st));
+        ANN_HIP(hipMemcpyAsync(ctx->da.inpb_net, ibn.data(), sizeof(float) * tot, hipMemcpyHostToDevice, st));
+        ANN_HIP(hipMemcpyAsync(ctx->da.inpb64, ctx->h_inpb.data(), sizeof(double) * tot, hipMemcpyHostToDevice, st));
+        ANN_HIP(hipMemcpyAsync(ctx->da.preflag, flag.data(), tot, hipMemcpyHostToDevice, st));
+    }
+    if (!items.empty())
+        ANN_HIP(hipMemcpyAsync(ctx->da.pitems, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice, st));
+    ANN_HIP(hipMemcpyAsync(ctx->dm, &M, sizeof(DevModel), hipMemcpyHostToDevice, st));
+    ANN_HIP(hipStreamSynchronize(st));
+    ctx->scaling_gen++;
+    ctx->G_valid = false;          // cached centred rows depend on the scaling
+    return ANN_OK;
+}
+
 static int upload_model(ann_ctx* ctx, const ann_model_desc* d) {
     const int S = d->nspecies;
     if (S < 1 || S > kMaxSpecies) return fail(ANN_ERR_UNSUPPORTED, "number of species must be in [1, 8]");
@@ -96,6 +152,8 @@ static int upload_model(ann_ctx* ctx, const ann_model_desc* d) {
     M.rci = (float)(1.0 / d->rc);
     M.pirc = (float)(3.141592653589793 / d->rc);
     std::vector<float> rrs, reta, aeta, azeta, alam;
+    std::vector<double> rrs64, reta64, aeta64, azeta64;   // the same tables in FP64 (FP64-evaluated inputs)
+    ctx->rc64 = d->rc;
     // radial blocks
     std::vector<int> nin(S, 0);
     for (int a = 0; a < S; ++a) {
@@ -112,6 +170,8 @@ static int upload_model(ann_ctx* ctx, const ann_model_desc* d) {
             for (int m = 0; m < n; ++m) {
                 rrs.push_back((float)d->rad_rs[d->rad_off[t] + m]);
                 reta.push_back((float)d->rad_eta[d->rad_off[t] + m]);
+                rrs64.push_back(d->rad_rs[d->rad_off[t] + m]);
+                reta64.push_back(d->rad_eta[d->rad_off[t] + m]);
             }
             goff += n;
         }
@@ -137,6 +197,8 @@ static int upload_model(ann_ctx* ctx, const ann_model_desc* d) {
                 if (!(eta[m] > 0)) return fail(ANN_ERR_ARG, "angular width must be positive");
                 aeta.push_back((float)eta[m]);
                 azeta.push_back((float)zeta[m]);
+                aeta64.push_back(eta[m]);
+                azeta64.push_back(zeta[m]);
                 alam.push_back((float)lam[m]);
             }
             nin[a] += n;
@@ -263,6 +325,8 @@ static int upload_model(ann_ctx* ctx, const ann_model_desc* d) {
         for (int m = 0; m < nin[a]; ++m) {
             iw.push_back((float)d->inpw[d->nn_inp_off[a] + m]);
             ib.push_back((float)d->inpb[d->nn_inp_off[a] + m]);
+            ctx->h_inpw.push_back(d->inpw[d->nn_inp_off[a] + m]);
+            ctx->h_inpb.push_back(d->inpb[d->nn_inp_off[a] + m]);
         }
         D.outw = (float)d->outw[a];
         D.outb = (float)d->outb[a];
@@ -307,6 +371,22 @@ static int upload_model(ann_ctx* ctx, const ann_model_desc* d) {
     ANN_HIP(hipMalloc(&ctx->d_wsplit, sizeof(uint4) * std::max(ctx->wsplit_n, 1)));
     ANN_HIP(up(ctx->da.inpw, iw));
     ANN_HIP(up(ctx->da.inpb, ib));
+    ANN_HIP(up(ctx->da.inpb_net, ib));
+    {
+        auto up64 = [](double*& dst, const std::vector<double>& v) -> hipError_t {
+            hipError_t e = hipMalloc(&dst, sizeof(double) * std::max<size_t>(v.size(), 1));
+            if (e != hipSuccess || v.empty()) return e;
+            return hipMemcpy(dst, v.data(), sizeof(double) * v.size(), hipMemcpyHostToDevice);
+        };
+        ANN_HIP(up64(ctx->da.rad_rs64, rrs64));
+        ANN_HIP(up64(ctx->da.rad_eta64, reta64));
+        ANN_HIP(up64(ctx->da.ang_eta64, aeta64));
+        ANN_HIP(up64(ctx->da.ang_zeta64, azeta64));
+        ANN_HIP(up64(ctx->da.inpb64, ctx->h_inpb));
+        const size_t nin_tot = std::max<size_t>(ctx->h_inpw.size(), 1);
+        ANN_HIP(hipMalloc(&ctx->da.pitems, sizeof(int4) * nin_tot));
+        ANN_HIP(hipMalloc(&ctx->da.preflag, nin_tot));
+    }
     ANN_HIP(hipMalloc(&ctx->da.params, sizeof(double) * ptot));
     ANN_HIP(hipMemcpy(ctx->da.params, ctx->params.data(), sizeof(double) * ptot, hipMemcpyHostToDevice));
     ANN_HIP(hipMalloc(&ctx->da.adam_m, sizeof(double) * ptot));
@@ -322,7 +402,7 @@ static int upload_model(ann_ctx* ctx, const ann_model_desc* d) {
     ANN_HIP(hipMemset(ctx->d_adam_t, 0, sizeof(int)));
     ANN_HIP(split_weights(ctx, nullptr));
     ANN_HIP(hipStreamSynchronize(ctx->stream));
-    return ANN_OK;
+    return update_precise(ctx);
 }
 
 extern "C" int ann_create(const ann_model_desc* desc, ann_ctx** out) {
@@ -352,7 +432,9 @@ extern "C" int ann_destroy(ann_ctx* ctx) {
                     ctx->d_Eatom, ctx->d_P, ctx->d_u, ctx->d_v, ctx->d_F, ctx->d_E, ctx->d_gE, ctx->d_atomvir,
                     ctx->d_scr, ctx->d_gpart, ctx->d_rank, ctx->d_spcount, ctx->d_perm, ctx->d_tiles, ctx->d_spend, ctx->d_grad_tiles, ctx->d_gsplit,
                     ctx->d_chunks, ctx->d_chunk_off, ctx->d_part, ctx->d_pos_ref, ctx->d_keatom, ctx->d_md_flag,
-                    ctx->d_bin_struc, ctx->d_binrec, ctx->d_adam_t, ctx->d_wsplit, ctx->d_T};
+                    ctx->d_bin_struc, ctx->d_binrec, ctx->d_adam_t, ctx->d_wsplit, ctx->d_T,
+                    ctx->da.pitems, ctx->da.preflag, ctx->da.inpb64, ctx->da.inpb_net, ctx->da.rad_rs64,
+                    ctx->da.rad_eta64, ctx->da.ang_eta64, ctx->da.ang_zeta64};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (ctx->h_rb) hipHostFree(ctx->h_rb);
@@ -429,13 +511,27 @@ extern "C" int ann_set_scaling(ann_ctx* ctx, int a, const double* inpw, const do
     CHECK_CTX(ctx);
     if (a < 0 || a >= ctx->S || !inpw || !inpb) return fail(ANN_ERR_ARG, "invalid species or null scaling");
     NetDesc& D = ctx->hm.net[a];
-    std::vector<float> w(inpw, inpw + D.nin), b(inpb, inpb + D.nin);
+    for (int k = 0; k < D.nin; ++k) {
+        if (!std::isfinite(inpw[k]) || !std::isfinite(inpb[k])) return fail(ANN_ERR_ARG, "non-finite input scaling");
+        ctx->h_inpw[D.inoff + k] = inpw[k];
+        ctx->h_inpb[D.inoff + k] = inpb[k];
+    }
     D.outw = (float)outw;
     D.outb = (float)outb;
-    ANN_HIP(hipMemcpyAsync(ctx->da.inpw + D.inoff, w.data(), sizeof(float) * D.nin, hipMemcpyHostToDevice, ctx->stream));
-    ANN_HIP(hipMemcpyAsync(ctx->da.inpb + D.inoff, b.data(), sizeof(float) * D.nin, hipMemcpyHostToDevice, ctx->stream));
-    ANN_HIP(hipMemcpyAsync(ctx->dm, &ctx->hm, sizeof(DevModel), hipMemcpyHostToDevice, ctx->stream));
-    ANN_HIP(hipStreamSynchronize(ctx->stream));
+    return update_precise(ctx);
+}
+
+extern "C" int ann_set_precise_ratio(ann_ctx* ctx, double ratio) {
+    CHECK_CTX(ctx);
+    if (std::isnan(ratio)) return fail(ANN_ERR_ARG, "invalid ratio");
+    ctx->pre_ratio = ratio;
+    return update_precise(ctx);
+}
+
+extern "C" int ann_precise_count(ann_ctx* ctx, int* count) {
+    CHECK_CTX(ctx);
+    if (!count) return fail(ANN_ERR_ARG, "null argument");
+    *count = ctx->npre;
     return ANN_OK;
 }
 
@@ -492,6 +588,7 @@ extern "C" int ann_set_batch(ann_ctx* ctx, const ann_batch* b) {
     ANN_HIP(struc_index(ctx));
     ANN_HIP(hipStreamSynchronize(ctx->stream));
     ctx->have_batch = true;
+    ctx->batch_gen++;
     ctx->have_nlist = false;
     ctx->grid_valid = false;
     ctx->gen++;
@@ -504,12 +601,12 @@ extern "C" int ann_set_batch(ann_ctx* ctx, const ann_batch* b) {
 extern "C" int ann_neighbors(ann_ctx* ctx, double rc) {
     CHECK_CTX(ctx);
     if (!ctx->have_batch) return fail(ANN_ERR_STATE, "no batch set");
-    const double r = rc > 0 ? rc : (double)ctx->hm.rc;
-    const double rmodel = rc > 0 ? rc : 0;
-    (void)rmodel;
+    const double r = rc > 0 ? rc : ctx->rc64;
     ctx->have_nlist = false;      // the build rewrites the list arrays in place
-    hipError_t e = neighbors_build(ctx, rc > 0 ? rc : ctx->hm.rc);
-    if (e == hipErrorInvalidValue) return ANN_ERR_CAPACITY;
+    if (!(rc >= 0) || std::isinf(rc)) return fail(ANN_ERR_ARG, "invalid neighbour cutoff");
+    ctx->nb_status = ANN_OK;
+    hipError_t e = neighbors_build(ctx, rc > 0 ? rc : ctx->rc64);
+    if (ctx->nb_status != ANN_OK) return ctx->nb_status;     // message already set
     if (e != hipSuccess) return fail(ANN_ERR_HIP, std::string("neighbour build: ") + hipGetErrorString(e));
     ctx->have_nlist = true;
     ctx->G_valid = false;
@@ -546,7 +643,10 @@ extern "C" int ann_nlist_size(ann_ctx* ctx, int64_t* np, int* maxn) {
     int rev_err = 0;
     ANN_HIP(hipMemcpyAsync(&rev_err, ctx->d_flag + 2, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     ANN_HIP(hipStreamSynchronize(ctx->stream));
-    if (rev_err) return fail(ANN_ERR_HIP, "neighbour list is not symmetric (reverse slot missing)");
+    if (rev_err) {
+        ANN_HIP(hipMemsetAsync(ctx->d_flag + 2, 0, sizeof(int), ctx->stream));
+        return fail(ANN_ERR_HIP, "neighbour list is not symmetric (reverse slot missing)");
+    }
     if (np) *np = ctx->nl.npairs;
     if (maxn) *maxn = ctx->nl.maxn;
     return ANN_OK;
@@ -624,6 +724,30 @@ extern "C" int ann_symm_bwd(ann_ctx* ctx, int form, const float* dEdG, float* P)
     return ANN_OK;
 }
 
+static hipError_t ensure_work(ann_ctx* ctx);
+
+extern "C" int ann_symm_fwd64(ann_ctx* ctx, int form, double* G64) {
+    int rc = need_nlist(ctx); if (rc) return rc;
+    if ((rc = check_form(form))) return rc;
+    if (!G64) return fail(ANN_ERR_ARG, "null output");
+    RET_HIP(ensure_work(ctx), "allocation");
+    ctx->G_valid = false;
+    RET_HIP(symm_fwd(ctx, form, ctx->d_G), "symm_fwd");
+    RET_HIP(widen_rows(ctx, ctx->d_G, G64), "symm_fwd64");
+    return ANN_OK;
+}
+
+extern "C" int ann_net_fwd64(ann_ctx* ctx, int form, const double* G64, float* Eatom, float* dEdG) {
+    int rc = need_nlist(ctx); if (rc) return rc;
+    if ((rc = check_form(form))) return rc;
+    if (!G64 || !Eatom) return fail(ANN_ERR_ARG, "null argument");
+    RET_HIP(ensure_work(ctx), "allocation");
+    ctx->G_valid = false;
+    RET_HIP(centre_rows(ctx, G64, ctx->d_G), "net_fwd64");
+    RET_HIP(net_fwd(ctx, form, ctx->d_G, Eatom, dEdG, true), "net_fwd");
+    return ANN_OK;
+}
+
 static hipError_t ensure_work(ann_ctx* ctx) {
     const int64_t N = ctx->natoms, P = ctx->nl.npairs;
     if (N > ctx->work_atoms) {
@@ -672,7 +796,8 @@ extern "C" int ann_force_reduce(ann_ctx* ctx, const float* P, const float* Eatom
 static hipError_t ef_body(ann_ctx* ctx, int form, int with_evac, double* E, double* F, double* V) {
     hipError_t e;
     if ((e = symm_fwd(ctx, form, ctx->d_G)) != hipSuccess) return e;
-    if ((e = net_fwd(ctx, form, ctx->d_G, ctx->d_Eatom, F || V ? ctx->d_dEdG : nullptr)) != hipSuccess) return e;
+    if ((e = precise_inputs(ctx, ctx->d_G, nullptr)) != hipSuccess) return e;
+    if ((e = net_fwd(ctx, form, ctx->d_G, ctx->d_Eatom, F || V ? ctx->d_dEdG : nullptr, true)) != hipSuccess) return e;
     if (F || V) {
         if ((e = symm_bwd(ctx, form, ctx->d_dEdG, ctx->d_P)) != hipSuccess) return e;
         return force_reduce(ctx, ctx->d_P, ctx->d_Eatom, with_evac, F, E, V);
@@ -685,12 +810,16 @@ extern "C" int ann_energy_forces(ann_ctx* ctx, int form, int with_evac, double* 
     int rc = need_nlist(ctx); if (rc) return rc;
     if ((rc = check_form(form))) return rc;
     RET_HIP(ensure_work(ctx), "allocation");
+    ctx->G_valid = false;         // the call rewrites the context's G rows
     if (!(ctx->ef_graph_on && ctx->ef_warm)) {
         RET_HIP(ef_body(ctx, form, with_evac, E, F, V), "energy_forces");
         ctx->ef_warm = true;
         return ANN_OK;
     }
-    // everything the captured launches bake in: arguments, buffers, grid and LDS sizes
+    // everything the captured launches bake in: arguments, buffers, grid and LDS
+    // sizes, the species tiles of the network (set up on the host, so they are
+    // brought up to date before the key is taken) and the FP64-evaluated inputs
+    if (form != ANN_FORM_PLAIN) RET_HIP(species_perm(ctx), "species permutation");
     const NList& nl = ctx->nl;
     std::vector<uint64_t> key = {
         (uint64_t)form, (uint64_t)with_evac, (uint64_t)(uintptr_t)E, (uint64_t)(uintptr_t)F, (uint64_t)(uintptr_t)V,
@@ -702,11 +831,10 @@ extern "C" int ann_energy_forces(ann_ctx* ctx, int form, int with_evac, double* 
         (uint64_t)(uintptr_t)ctx->d_perm,
         (uint64_t)(uintptr_t)ctx->d_T, (uint64_t)(uintptr_t)ctx->d_chunks, (uint64_t)ctx->nchunks,
         (uint64_t)ctx->tune_apb[0], (uint64_t)ctx->tune_apb[1], (uint64_t)ctx->tune_apb[2],
-        (uint64_t)ctx->tune_angular, (uint64_t)ctx->wsplit_valid};
+        (uint64_t)ctx->tune_angular, (uint64_t)ctx->wsplit_valid, ctx->batch_gen, (uint64_t)ctx->ntiles,
+        (uint64_t)(uintptr_t)ctx->d_tiles, ctx->scaling_gen, (uint64_t)ctx->npre};
     if (!ctx->ef_exec || key != ctx->ef_key) {
         if (ctx->ef_exec) { hipGraphExecDestroy(ctx->ef_exec); ctx->ef_exec = nullptr; }
-        // host-side set-up that cannot be captured: the species permutation
-        if (form != ANN_FORM_PLAIN) RET_HIP(species_perm(ctx), "species permutation");
         hipStream_t user = ctx->stream;
         RET_HIP(hipEventRecord(ctx->gev[0], user), "graph event");
         RET_HIP(hipStreamWaitEvent(ctx->gstream, ctx->gev[0], 0), "graph wait");
@@ -778,7 +906,8 @@ extern "C" int ann_md_step(ann_ctx* ctx, int form, double dt, const double* mass
     ctx->G_valid = false;
     ctx->gen++;
     RET_HIP(symm_fwd(ctx, form, ctx->d_G), "symm_fwd");
-    RET_HIP(net_fwd(ctx, form, ctx->d_G, ctx->d_Eatom, ctx->d_dEdG), "net_fwd");
+    RET_HIP(precise_inputs(ctx, ctx->d_G, nullptr), "precise_inputs");
+    RET_HIP(net_fwd(ctx, form, ctx->d_G, ctx->d_Eatom, ctx->d_dEdG, true), "net_fwd");
     RET_HIP(symm_bwd(ctx, form, ctx->d_dEdG, ctx->d_P), "symm_bwd");
     RET_HIP(force_reduce(ctx, ctx->d_P, ctx->d_Eatom, 1, F, E, V), "force_reduce");
     RET_HIP(md_kick(ctx, dt, mass, F, vel), "md_kick");

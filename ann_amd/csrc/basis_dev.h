@@ -81,13 +81,22 @@ __device__ __forceinline__ void radial_fd(int name, float eta, float rs, float r
         case RAD_LOGCOSH2: {  // a e/(e - 1), a = -eta x, e = exp(2 sqrt2 a)
             const float k2 = 1.414213562373095f;
             const float a = -eta * x;
-            if (fabsf(a) < 1e-6f) {
+            if (a == 0.0f) {
+                // the reference's value at exactly a == 0 (src/nnp/basis_radial.cpp:250,
+                // :367), kept as is: it is not the a -> 0 limit of the branch below
                 v = 0.5f * fc / k2;
                 d = 0.5f * (-eta * gc + fc / k2);
             } else {
-                const float e = __expf(2.0f * k2 * a), den = 1.0f / (e - 1.0f);
+                // e - 1 from expm1 and e - 1 - y by its series where the difference
+                // cancels, so that small |a| keeps FP32 accuracy
+                const float y = 2.0f * k2 * a;
+                const float em1 = expm1f(y), e = em1 + 1.0f, den = 1.0f / em1;
+                const float c = fabsf(y) < 0.25f
+                                    ? y * y * (0.5f + y * (1.0f / 6.0f + y * (1.0f / 24.0f + y * (1.0f / 120.0f +
+                                                           y * (1.0f / 720.0f + y * (1.0f / 5040.0f))))))
+                                    : em1 - y;
                 v = a * e * den * fc;
-                d = e * (eta * (1.0f + 2.0f * k2 * a - e) * den * fc + a * gc) * den;
+                d = e * (-eta * c * den * fc + a * gc) * den;
             }
         } break;
         default: v = 0.0f; d = 0.0f; break;

@@ -80,6 +80,7 @@ struct DevModel {
     int all_grid;                               // every angular block is grid-shaped
     int fast_ng, fast_nk;                       // the common grid shape (0 if blocks differ)
     int moment_z;                               // largest zeta if the moment form applies, else 0
+    int pre_off[kMaxSpecies + 1];               // FP64-evaluated inputs of species a: items [pre_off[a], pre_off[a+1])
 };
 
 // device pointers of the variable-size tables
@@ -93,6 +94,15 @@ struct DevArrays {
     float* inpw = nullptr;
     float* inpb = nullptr;
     double* params = nullptr;    // FP64 master parameters
+    // inputs evaluated in FP64 (large |inpw inpb|): items (input, kind, block, function)
+    int4* pitems = nullptr;
+    uint8_t* preflag = nullptr;  // per input (inoff-indexed): 1 if evaluated in FP64
+    double* inpb64 = nullptr;    // input bias in FP64
+    float* inpb_net = nullptr;   // input bias the network applies to centred rows (0 where FP64-evaluated)
+    double* rad_rs64 = nullptr;  // FP64 basis parameters for those inputs
+    double* rad_eta64 = nullptr;
+    double* ang_eta64 = nullptr;
+    double* ang_zeta64 = nullptr;
     double* adam_m = nullptr;
     double* adam_v = nullptr;
 };
@@ -134,6 +144,13 @@ struct ann_ctx {
     int S = 0, ldg = 0, nparams = 0, max_nin = 0;
     ann::DevModel hm{};
     std::vector<double> params;
+    double rc64 = 0;                  // model cutoff in FP64
+    std::vector<double> h_inpw, h_inpb;   // input scaling (FP64, inoff-indexed)
+    double pre_ratio = 32.0;          // inputs with |inpw inpb| above this are evaluated in FP64 (< 0: all)
+    int npre = 0;                     // number of FP64-evaluated inputs over all species
+    uint64_t scaling_gen = 0;         // bumped when the input scaling or its FP64 set changes
+    uint64_t batch_gen = 0;           // bumped by every ann_set_batch
+    int nb_status = 0;                // ANN_* status of the last neighbour build (capacity, image limit, symmetry)
     // device model
     ann::DevModel* dm = nullptr;
     ann::DevArrays da;
@@ -279,10 +296,20 @@ hipError_t exclusive_scan(const int* in, int* out, int n, hipStream_t st, const 
 // copy n device ints to the pinned words rb (stream ordered; read after a sync)
 hipError_t readback(const int* src, int n, int* rb, hipStream_t st);
 hipError_t symm_fwd(ann_ctx* ctx, int form, float* G);
+// inputs flagged for FP64 evaluation: with Gc set, Gc[i][k] = G_k + inpb_k (FP64
+// sum rounded to FP32) for every flagged k (other entries untouched); with G64
+// set, G64[i][k] = G_k in FP64 for every flagged k
+hipError_t precise_inputs(ann_ctx* ctx, float* Gc, double* G64);
+// G64 = G (widened), then the flagged entries recomputed in FP64
+hipError_t widen_rows(ann_ctx* ctx, const float* G, double* G64);
+// Gc = G64 rounded to FP32, centred (G + inpb in FP64) where flagged
+hipError_t centre_rows(ann_ctx* ctx, const double* G64, float* Gc);
 hipError_t symm_bwd(ann_ctx* ctx, int form, const float* dEdG, float* P);
 hipError_t symm_tangent(ann_ctx* ctx, int form, const float* u, float* v);
-hipError_t net_fwd(ann_ctx* ctx, int form, const float* G, float* Eatom, float* dEdG);
-hipError_t net_bwd2(ann_ctx* ctx, const float* G, const float* v, const double* gE, double* grad);
+// centred: G rows hold G + inpb at the FP64-evaluated inputs (precise_inputs);
+// the network then applies inpb_net instead of inpb
+hipError_t net_fwd(ann_ctx* ctx, int form, const float* G, float* Eatom, float* dEdG, bool centred = false);
+hipError_t net_bwd2(ann_ctx* ctx, const float* G, const float* v, const double* gE, double* grad, bool centred = false);
 hipError_t force_reduce(ann_ctx* ctx, const float* P, const float* Eatom, int with_evac, double* F,
                         double* E, double* V);
 hipError_t struc_sums(ann_ctx* ctx, const float* Eatom, int with_evac, const double* avir, const double* keatom,

@@ -19,6 +19,7 @@
 #include <atomic>
 
 #include "common.h"
+#include "pairvec_dev.h"
 
 #include <hipcub/hipcub.hpp>
 
@@ -28,34 +29,6 @@
 namespace ann {
 
 constexpr int kMaxImages = 1024;   // bin images searched per atom
-
-// r_i - r_j - R n in FP64, evaluated exactly as the oracle does (no
-// contraction) so that inclusion decisions agree bit for bit.
-__device__ __forceinline__ void pair_vec(const double* __restrict__ pos, int i, int j,
-                                         const double* __restrict__ R, int n0, int n1, int n2,
-                                         double& x, double& y, double& z, double& d2) {
-#pragma clang fp contract(off)
-    const double m0 = n0, m1 = n1, m2 = n2;
-    x = __dsub_rn(__dsub_rn(pos[3 * i], pos[3 * j]),
-                  __dadd_rn(__dadd_rn(__dmul_rn(R[0], m0), __dmul_rn(R[1], m1)), __dmul_rn(R[2], m2)));
-    y = __dsub_rn(__dsub_rn(pos[3 * i + 1], pos[3 * j + 1]),
-                  __dadd_rn(__dadd_rn(__dmul_rn(R[3], m0), __dmul_rn(R[4], m1)), __dmul_rn(R[5], m2)));
-    z = __dsub_rn(__dsub_rn(pos[3 * i + 2], pos[3 * j + 2]),
-                  __dadd_rn(__dadd_rn(__dmul_rn(R[6], m0), __dmul_rn(R[7], m1)), __dmul_rn(R[8], m2)));
-    d2 = __dadd_rn(__dadd_rn(__dmul_rn(x, x), __dmul_rn(y, y)), __dmul_rn(z, z));
-}
-
-// the same expression on positions already in registers
-__device__ __forceinline__ void pair_vec_v(double pix, double piy, double piz, double pjx, double pjy, double pjz,
-                                           const double* __restrict__ R, int n0, int n1, int n2,
-                                           double& x, double& y, double& z, double& d2) {
-#pragma clang fp contract(off)
-    const double m0 = n0, m1 = n1, m2 = n2;
-    x = __dsub_rn(__dsub_rn(pix, pjx), __dadd_rn(__dadd_rn(__dmul_rn(R[0], m0), __dmul_rn(R[1], m1)), __dmul_rn(R[2], m2)));
-    y = __dsub_rn(__dsub_rn(piy, pjy), __dadd_rn(__dadd_rn(__dmul_rn(R[3], m0), __dmul_rn(R[4], m1)), __dmul_rn(R[5], m2)));
-    z = __dsub_rn(__dsub_rn(piz, pjz), __dadd_rn(__dadd_rn(__dmul_rn(R[6], m0), __dmul_rn(R[7], m1)), __dmul_rn(R[8], m2)));
-    d2 = __dadd_rn(__dadd_rn(__dmul_rn(x, x), __dmul_rn(y, y)), __dmul_rn(z, z));
-}
 
 // per structure grid: nb[3] bins, search half-width k[3] in bins, bin count
 // and offset; the staged search groups sb^3 bins into one block (super-bins:
@@ -828,7 +801,13 @@ hipError_t neighbors_build(ann_ctx* ctx, double rc) {
     volatile const int* rb = ctx->h_rb;
     int flags[4] = {rb[0], rb[1], rb[2], rb[3]};
     int total = rb[4];
-    if (flags[2]) { set_error("the previous neighbour list was not symmetric (reverse slot missing)"); return hipErrorInvalidValue; }
+    if (flags[2]) {
+        // reported once: the flag of the list that lacked a reverse slot is cleared
+        HIPCHK(hipMemsetAsync(ctx->d_flag + 2, 0, sizeof(int), st));
+        ctx->nb_status = ANN_ERR_HIP;
+        set_error("the previous neighbour list was not symmetric (reverse slot missing)");
+        return hipErrorInvalidValue;
+    }
     if (flags[0] & 8) {
         // bins with a search box too large to stage: one wavefront per centre atom
         {
@@ -846,8 +825,13 @@ hipError_t neighbors_build(ann_ctx* ctx, double rc) {
         for (int q = 0; q < 4; ++q) flags[q] = rb[q];
         total = rb[4];
     }
-    if (flags[0] & 2) { set_error("neighbour search box exceeds the image limit (cell far narrower than rc)"); return hipErrorInvalidValue; }
+    if (flags[0] & 2) {
+        ctx->nb_status = ANN_ERR_UNSUPPORTED;
+        set_error("neighbour search box exceeds the image limit (cell far narrower than rc)");
+        return hipErrorInvalidValue;
+    }
     if (flags[0] & 1) {
+        ctx->nb_status = ANN_ERR_CAPACITY;
         set_error("an atom has " + std::to_string(flags[1]) + " neighbours; capacity is " + std::to_string(kMaxNeigh));
         return hipErrorInvalidValue;
     }

@@ -159,8 +159,8 @@ int max_width(const DevModel& M) {
     return w;
 }
 
-hipError_t net_fwd_plain(ann_ctx* ctx, const float* G, float* Eatom, float* dEdG) {
-    NetArgs A{ctx->dm, ctx->da.W, ctx->da.inpw, ctx->da.inpb, ctx->natoms, ctx->ldg, ctx->d_species,
+hipError_t net_fwd_plain(ann_ctx* ctx, const float* G, float* Eatom, float* dEdG, const float* inpb) {
+    NetArgs A{ctx->dm, ctx->da.W, ctx->da.inpw, inpb, ctx->natoms, ctx->ldg, ctx->d_species,
               G, Eatom, dEdG};
     if (A.N == 0) return hipSuccess;
     Launch L(ctx, "net_plain");
@@ -168,12 +168,13 @@ hipError_t net_fwd_plain(ann_ctx* ctx, const float* G, float* Eatom, float* dEdG
     return hipGetLastError();
 }
 
-hipError_t net_fwd_mfma(ann_ctx* ctx, const float* G, float* Eatom, float* dEdG);
+hipError_t net_fwd_mfma(ann_ctx* ctx, const float* G, float* Eatom, float* dEdG, const float* inpb);
 bool mfma_net_supported(const DevModel& M);
 
-hipError_t net_fwd(ann_ctx* ctx, int form, const float* G, float* Eatom, float* dEdG) {
-    if (form != ANN_FORM_PLAIN && mfma_net_supported(ctx->hm)) return net_fwd_mfma(ctx, G, Eatom, dEdG);
-    return net_fwd_plain(ctx, G, Eatom, dEdG);
+hipError_t net_fwd(ann_ctx* ctx, int form, const float* G, float* Eatom, float* dEdG, bool centred) {
+    const float* inpb = centred ? ctx->da.inpb_net : ctx->da.inpb;
+    if (form != ANN_FORM_PLAIN && mfma_net_supported(ctx->hm)) return net_fwd_mfma(ctx, G, Eatom, dEdG, inpb);
+    return net_fwd_plain(ctx, G, Eatom, dEdG, inpb);
 }
 
 }  // namespace ann

@@ -304,14 +304,14 @@ __global__ __launch_bounds__(128) void k_net_mfma(MfmaArgs A) {
     }
 }
 
-hipError_t net_fwd_mfma(ann_ctx* ctx, const float* G, float* Eatom, float* dEdG) {
+hipError_t net_fwd_mfma(ann_ctx* ctx, const float* G, float* Eatom, float* dEdG, const float* inpb) {
     hipError_t e = species_perm(ctx);
     if (e != hipSuccess) return e;
     MfmaArgs A{};
     A.dm = ctx->dm;
     A.W = ctx->da.W;
     A.inpw = ctx->da.inpw;
-    A.inpb = ctx->da.inpb;
+    A.inpb = inpb;
     A.ldg = ctx->ldg;
     A.perm = ctx->d_perm;
     A.tiles = reinterpret_cast<const int2*>(ctx->d_tiles);

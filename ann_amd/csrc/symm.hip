@@ -1292,8 +1292,11 @@ static hipError_t run(ann_ctx* ctx, int form, SymmArgs A) {
                                {"symm_fwd_mom", "symm_bwd_mom", "symm_tan_mom"}};
     Launch L(ctx, names[moment ? 2 : (fast ? 1 : 0)][MODE]);
     if (moment) {
-        if (MODE == 0) A.Tout = ctx->d_T;
-        if (MODE == 1 && ctx->d_T && ctx->T_gen == ctx->gen) A.Tin = ctx->d_T;
+        // the moment store holds work_atoms rows: it is used only when the batch
+        // fits (a forward call on a grown batch before any fused call skips it)
+        const bool tfits = ctx->d_T && ctx->work_atoms >= A.N;
+        if (MODE == 0 && tfits) A.Tout = ctx->d_T;
+        if (MODE == 1 && tfits && ctx->T_gen == ctx->gen) A.Tin = ctx->d_T;
         const size_t sw = (moment_smem(A, MODE, M.moment_z, M.S, M.NP) + 15) / 16 * 16;
         int apb = ctx->tune_apb[MODE];
         while (apb > 1 && sw * apb > 65536) apb /= 2;          // dynamic LDS of one workgroup
@@ -1306,7 +1309,7 @@ static hipError_t run(ann_ctx* ctx, int form, SymmArgs A) {
             case 3: k_moment<MODE, 3><<<grid, block, sm, ctx->stream>>>(A); break;
             default: k_moment<MODE, 4><<<grid, block, sm, ctx->stream>>>(A); break;
         }
-        if (MODE == 0 && A.Tout) ctx->T_gen = ctx->gen;
+        if (MODE == 0) ctx->T_gen = A.Tout ? ctx->gen : ~0ull;
         return hipGetLastError();
     }
     if (fast) {

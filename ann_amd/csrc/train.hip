@@ -680,14 +680,14 @@ hipError_t species_perm(ann_ctx* ctx) {
     return hipSuccess;
 }
 
-hipError_t net_bwd2(ann_ctx* ctx, const float* G, const float* v, const double* gE, double* grad) {
+hipError_t net_bwd2(ann_ctx* ctx, const float* G, const float* v, const double* gE, double* grad, bool centred) {
     hipError_t e = species_ranks(ctx);
     if (e != hipSuccess) return e;
     Bwd2Args A{};
     A.dm = ctx->dm;
     A.W = ctx->da.W;
     A.inpw = ctx->da.inpw;
-    A.inpb = ctx->da.inpb;
+    A.inpb = centred ? ctx->da.inpb_net : ctx->da.inpb;
     A.N = ctx->natoms;
     A.ldg = ctx->ldg;
     A.species = ctx->d_species;
@@ -841,9 +841,10 @@ hipError_t train_step_impl(ann_ctx* ctx, int form, const double* Eref, const dou
     const int B = ctx->nstruc, N = ctx->natoms;
     if (!ctx->G_valid) {
         if ((e = symm_fwd(ctx, form, ctx->d_G)) != hipSuccess) return e;
+        if ((e = precise_inputs(ctx, ctx->d_G, nullptr)) != hipSuccess) return e;
         ctx->G_valid = true;
     }
-    if ((e = net_fwd(ctx, form, ctx->d_G, ctx->d_Eatom, ctx->d_dEdG)) != hipSuccess) return e;
+    if ((e = net_fwd(ctx, form, ctx->d_G, ctx->d_Eatom, ctx->d_dEdG, true)) != hipSuccess) return e;
     if ((e = symm_bwd(ctx, form, ctx->d_dEdG, ctx->d_P)) != hipSuccess) return e;
     if ((e = force_reduce(ctx, ctx->d_P, ctx->d_Eatom, 0, ctx->d_F, ctx->d_E, nullptr)) != hipSuccess) return e;
     // per-structure loss terms live after the gradient scratch
@@ -865,7 +866,7 @@ hipError_t train_step_impl(ann_ctx* ctx, int form, const double* Eref, const dou
     if ((e = hipGetLastError()) != hipSuccess) return e;
     (void)N;
     if ((e = symm_tangent(ctx, form, ctx->d_u, ctx->d_v)) != hipSuccess) return e;
-    if ((e = net_bwd2(ctx, ctx->d_G, ctx->d_v, ctx->d_gE, g)) != hipSuccess) return e;
+    if ((e = net_bwd2(ctx, ctx->d_G, ctx->d_v, ctx->d_gE, g, true)) != hipSuccess) return e;
     if (gamma > 0) {
         Launch L(ctx, "adam");
         k_adam<<<(ctx->nparams + 255) / 256, 256, 0, ctx->stream>>>(ctx->nparams, ctx->da.params, ctx->da.adam_m,

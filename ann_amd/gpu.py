@@ -56,7 +56,8 @@ EXPORTS = ["ann_last_error", "ann_create", "ann_destroy", "ann_set_stream", "ann
            "ann_symm_tangent", "ann_net_bwd2", "ann_train_step", "ann_get_params", "ann_profile",
            "ann_profile_get", "ann_profile_only", "ann_refresh_geometry", "ann_wrap_positions", "ann_md_step", "ann_train_graph", "ann_ef_graph",
            "ann_autotune", "ann_get_tuning", "ann_set_tuning", "ann_input_stats",
-           "ann_get_optimizer_state", "ann_set_optimizer_state"]
+           "ann_get_optimizer_state", "ann_set_optimizer_state", "ann_symm_fwd64", "ann_net_fwd64",
+           "ann_set_precise_ratio", "ann_precise_count"]
 
 
 def lib():
@@ -211,6 +212,30 @@ class Engine:
         G = torch.empty((self.N, self.ldg), dtype=torch.float32, device=self.device)
         _check(lib().ann_symm_fwd(self.h, form, _ptr(G)))
         return G
+
+    def symm64(self, form: int = FAST) -> torch.Tensor:
+        """G in FP64 (ann_symm_fwd64): FP64-evaluated inputs exact to FP64, others widened."""
+        G = torch.empty((self.N, self.ldg), dtype=torch.float64, device=self.device)
+        _check(lib().ann_symm_fwd64(self.h, form, _ptr(G)))
+        return G
+
+    def net64(self, G: torch.Tensor, form: int = FAST, grad: bool = True):
+        """Network on FP64 rows (ann_net_fwd64)."""
+        E = torch.empty(self.N, dtype=torch.float32, device=self.device)
+        dE = torch.empty((self.N, self.ldg), dtype=torch.float32, device=self.device) if grad else None
+        _check(lib().ann_net_fwd64(self.h, form, _ptr(G.to(torch.float64).contiguous()), _ptr(E), _ptr(dE)))
+        return E, dE
+
+    def set_precise_ratio(self, ratio: float) -> int:
+        """Inputs with |inpw inpb| > ratio (all if ratio < 0) are evaluated in FP64;
+        returns how many inputs that flags."""
+        _check(lib().ann_set_precise_ratio(self.h, C.c_double(ratio)))
+        return self.precise_count()
+
+    def precise_count(self) -> int:
+        n = C.c_int()
+        _check(lib().ann_precise_count(self.h, C.byref(n)))
+        return n.value
 
     def net(self, G: torch.Tensor, form: int = FAST, grad: bool = True):
         E = torch.empty(self.N, dtype=torch.float32, device=self.device)

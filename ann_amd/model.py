@@ -30,8 +30,9 @@ import numpy as np
 CUT = {"NONE": 0, "STEP": 1, "COS": 2, "TANH": 3, "POLY3": 4}
 RAD = {"NONE": 0, "GAUSSIAN": 1, "SECH": 2, "LOGISTIC": 3, "TANH": 4, "LOGCOSH": 5, "LOGCOSH2": 6}
 ANG = {"NONE": 0, "GAUSS": 1, "GAUSS2": 2, "SECH": 3, "STUDENT3": 4, "STUDENT4": 5, "STUDENT5": 6}
-NEURON = {"LINEAR": 1, "SIGMOID": 2, "TANH": 3, "ARCTAN": 5, "RELU": 6, "ELU": 7,
-          "SWISH": 10, "SOFTPLUS": 16, "SQPLUS": 17}
+NEURON = {"LINEAR": 1, "SIGMOID": 2, "TANH": 3, "ISRU": 4, "ARCTAN": 5, "RELU": 6, "ELU": 7,
+          "TANHRE": 8, "SQRE": 9, "SWISH": 10, "GELU": 11, "MISH": 12, "PFLU": 13, "SERF": 14,
+          "LOGISH": 15, "SOFTPLUS": 16, "SQPLUS": 17, "ATISH": 18}
 
 
 def pair_index(j: int, k: int) -> int:

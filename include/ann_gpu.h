@@ -138,6 +138,28 @@ int ann_symm_bwd(ann_ctx* ctx, int form, const float* d_dEdG, float* d_P);
  * NNP::energy), virial W = -sum P (x) r; any output may be NULL */
 int ann_force_reduce(ann_ctx* ctx, const float* d_P, const float* d_Eatom, int with_evac,
                      double* d_F, double* d_E, double* d_virial);
+/* The same forward pair with G in FP64, as the reference stores it
+ * (Structure::symm, a VecXd per atom; NNP::symm src/nnp/nnp.cpp:333-375 and
+ * NNP::energy :436-447, which applies inpw (G + inpb) in FP64,
+ * src/ml/nn.cpp:1346).  Inputs flagged for FP64 evaluation (see
+ * ann_set_precise_ratio) are computed in FP64 throughout; the others are the
+ * FP32 values of ann_symm_fwd widened.  ann_net_fwd64 forms G + inpb in FP64
+ * for the flagged inputs before rounding to the FP32 network.  G [N][ldg]. */
+int ann_symm_fwd64(ann_ctx* ctx, int form, double* d_G);
+int ann_net_fwd64(ann_ctx* ctx, int form, const double* d_G, float* d_Eatom, float* d_dEdG);
+/* Inputs whose scaled value inpw (G + inpb) an FP32 G cannot resolve are
+ * evaluated in FP64: every input k with |inpw_k inpb_k| > ratio (default 32;
+ * ratio < 0 flags every input).  The reference trainer's preconditioning
+ * inpb = -<G>, inpw = 1/sigma(G) (src/nnp/nnptefr2.cpp:776-867) makes an
+ * almost constant input reach |inpw inpb| ~ 1e5, where the ~1e-7 relative
+ * rounding of an FP32 G becomes an O(1e-2) error of the network input.  The
+ * fused entry points (ann_energy_forces, ann_md_step, ann_train_step) and the
+ * FP64 pair above use the FP64 values; ann_symm_fwd / ann_net_fwd stay FP32.
+ * The set is recomputed by ann_set_scaling.  No counterpart in the reference
+ * (which is FP64 throughout). */
+int ann_set_precise_ratio(ann_ctx* ctx, double ratio);
+/* number of inputs (over all species) currently evaluated in FP64 */
+int ann_precise_count(ann_ctx* ctx, int* count);
 /* fused NNP::compute (src/nnp/nnp.cpp:601-665): energies, forces, virials */
 int ann_energy_forces(ann_ctx* ctx, int form, int with_evac, double* d_E, double* d_F,
                       double* d_virial);

@@ -3,11 +3,23 @@
  * ONLY (see oracle.h).
  *
  * Follows:
- *   activations a, da/dz, d2a/dz2   src/ml/nn.cpp:149-236 (linear, sigmoid,
- *                                   tanh), :282-400 (arctan, relu, elu),
- *                                   :480-500 (swish), :702-760 (softplus,
- *                                   sqplus); softplus uses the reference's
- *                                   truncated ln2 constant (src/math/const.hpp:15)
+ *   activations a, da/dz, d2a/dz2   src/ml/nn.cpp:149-237 (linear, sigmoid,
+ *                                   tanh), :238-266 (isru), :268-368 (arctan,
+ *                                   relu, elu), :369-449 (tanhre, sqre),
+ *                                   :451-498 (swish), :500-524 (gelu),
+ *                                   :526-586 (mish), :588-616 (pflu),
+ *                                   :618-643 (serf), :645-674 (logish),
+ *                                   :676-750 (softplus, sqplus), :752-780
+ *                                   (atish); softplus and logish use the
+ *                                   reference's truncated ln2 constant
+ *                                   (src/math/const.hpp:15).  Values and first
+ *                                   derivatives are the reference's (serf, logish,
+ *                                   atish and sqre ignore the sharpness c there,
+ *                                   and so here).  Second derivatives are the
+ *                                   true ones: the reference's are zero for
+ *                                   gelu, mish, serf, logish and atish, carry an
+ *                                   extra factor c for tanhre and have the
+ *                                   wrong sign for sqre (:432-449)
  *   ANN::fp / fpbp / fpbp2          src/ml/nn.cpp:1343-1413
  *   DODZ::grad (dE/dG)              src/ml/nn.cpp:1801-1830
  *   Cost::grad (energy backprop)    src/ml/nn.cpp:1712-1776, packing :846-865
@@ -33,6 +45,7 @@
 namespace orc {
 
 const double kLN2 = 0.693147180559945;  // truncated as in src/math/const.hpp:15
+const double kRadPI = 1.772453850905516; // sqrt(pi), src/math/const.hpp:10
 
 void activation(int neuron, double c, double z, double& a, double& d, double& d2) {
     switch (neuron) {
@@ -89,6 +102,68 @@ void activation(int neuron, double c, double z, double& a, double& d, double& d2
         case ORC_NEU_SQPLUS: {
             const double ci = 1.0 / c, zz = c * z, s = std::sqrt(1.0 + zz * zz);
             a = 0.5 * ci * (zz - 1.0 + s); d = 0.5 * (zz / s + 1.0); d2 = 0.5 * c / (s * s * s);
+        } break;
+        case ORC_NEU_ISRU: {
+            const double zz = c * z, isr = 1.0 / std::sqrt(1.0 + zz * zz), isr3 = isr * isr * isr;
+            a = zz * isr; d = c * isr3; d2 = -3.0 * c * c * zz * isr3 * isr * isr;
+        } break;
+        case ORC_NEU_TANHRE:
+            if (z >= 0.0) { a = z; d = 1.0; d2 = 0.0; }
+            else {
+                const double e = std::exp(2.0 * c * z);
+                a = (e - 1.0) / (e + 1.0); d = c * (1.0 - a * a); d2 = -2.0 * c * a * d;
+            }
+            break;
+        case ORC_NEU_SQRE:
+            if (z >= 0.0) { a = z; d = 1.0; d2 = 0.0; }
+            else {
+                const double den = 1.0 / std::sqrt(1.0 + z * z), den3 = den * den * den;
+                a = z * den; d = den3; d2 = -3.0 * z * den3 * den * den;
+            }
+            break;
+        case ORC_NEU_GELU: {
+            const double zz = 0.5 * kRadPI * c * z, g = std::exp(-zz * zz), ep1 = 1.0 + std::erf(zz);
+            a = 0.5 * z * ep1; d = 0.5 * (ep1 + c * z * g); d2 = c * g * (1.0 - zz * zz);
+        } break;
+        case ORC_NEU_MISH: {
+            // a = z tanh(softplus(c z)); with f = tanh(softplus(u)), s = sigmoid(u):
+            // f' = (1 - f^2) s, f'' = (1 - f^2) s ((1 - s) - 2 f s)
+            const double zz = c * z;
+            double f, s;
+            if (zz > 0.0) {
+                const double e = std::exp(-zz), e2 = e * e;
+                f = 1.0 - 2.0 * e2 / (1.0 + 2.0 * (e + e2));
+                s = 1.0 / (1.0 + e);
+            } else {
+                const double e = std::exp(zz), ff = e * (e + 2.0);
+                f = ff / (ff + 2.0);
+                s = e / (1.0 + e);
+            }
+            const double fp = (1.0 - f * f) * s, fpp = fp * ((1.0 - s) - 2.0 * f * s);
+            a = z * f; d = f + zz * fp; d2 = 2.0 * c * fp + c * zz * fpp;
+        } break;
+        case ORC_NEU_PFLU: {
+            const double zz = c * z, z2 = zz * zz, q = 1.0 / std::sqrt(z2 + 1.0), q3 = q * q * q;
+            a = 0.5 * z * (1.0 + zz * q); d = 0.5 * (zz * (z2 + 2.0) * q3 + 1.0);
+            d2 = -0.5 * c * (z2 - 2.0) * q3 * q * q;
+        } break;
+        case ORC_NEU_SERF: {
+            // a = z erf(softplus(z)); d = erf(sp) + z g with g = 2/sqrt(pi) exp(-sp^2) sigmoid(z)
+            const double sp = std::max(z, 0.0) + std::log1p(std::exp(-std::fabs(z)));
+            const double s = z >= 0.0 ? 1.0 / (1.0 + std::exp(-z)) : std::exp(z) / (1.0 + std::exp(z));
+            const double g = 2.0 / kRadPI * std::exp(-sp * sp) * s;
+            a = z * std::erf(sp); d = std::erf(sp) + z * g; d2 = 2.0 * g + z * g * ((1.0 - s) - 2.0 * sp * s);
+        } break;
+        case ORC_NEU_LOGISH: {
+            // a = z log(1 + sigmoid(z)) / ln2; h = s (1 - s) / (1 + s)
+            const double s = z >= 0.0 ? 1.0 / (1.0 + std::exp(-z)) : std::exp(z) / (1.0 + std::exp(z));
+            const double L = std::log1p(s), h = s * (1.0 - s) / (1.0 + s);
+            const double hp = s * (1.0 - s) * ((1.0 - 2.0 * s) / (1.0 + s) - s * (1.0 - s) / ((1.0 + s) * (1.0 + s)));
+            a = z * L / kLN2; d = (L + z * h) / kLN2; d2 = (2.0 * h + z * hp) / kLN2;
+        } break;
+        case ORC_NEU_ATISH: {
+            const double zz = 0.5 * kPI * z, q = 1.0 / (1.0 + zz * zz);
+            a = z * 0.5 * (1.0 + 2.0 / kPI * std::atan(zz)); d = 0.5 * (1.0 + z * q) + std::atan(zz) / kPI; d2 = q * q;
         } break;
         default: throw std::invalid_argument("orc: unsupported neuron");
     }

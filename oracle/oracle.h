@@ -31,9 +31,10 @@ enum { ORC_RAD_NONE = 0, ORC_RAD_GAUSSIAN = 1, ORC_RAD_SECH = 2, ORC_RAD_LOGISTI
        ORC_RAD_TANH = 4, ORC_RAD_LOGCOSH = 5, ORC_RAD_LOGCOSH2 = 6 };
 enum { ORC_ANG_NONE = 0, ORC_ANG_GAUSS = 1, ORC_ANG_GAUSS2 = 2, ORC_ANG_SECH = 3,
        ORC_ANG_STUDENT3 = 4, ORC_ANG_STUDENT4 = 5, ORC_ANG_STUDENT5 = 6 };
-enum { ORC_NEU_LINEAR = 1, ORC_NEU_SIGMOID = 2, ORC_NEU_TANH = 3, ORC_NEU_ARCTAN = 5,
-       ORC_NEU_RELU = 6, ORC_NEU_ELU = 7, ORC_NEU_SWISH = 10, ORC_NEU_SOFTPLUS = 16,
-       ORC_NEU_SQPLUS = 17 };
+enum { ORC_NEU_LINEAR = 1, ORC_NEU_SIGMOID = 2, ORC_NEU_TANH = 3, ORC_NEU_ISRU = 4,
+       ORC_NEU_ARCTAN = 5, ORC_NEU_RELU = 6, ORC_NEU_ELU = 7, ORC_NEU_TANHRE = 8, ORC_NEU_SQRE = 9,
+       ORC_NEU_SWISH = 10, ORC_NEU_GELU = 11, ORC_NEU_MISH = 12, ORC_NEU_PFLU = 13, ORC_NEU_SERF = 14,
+       ORC_NEU_LOGISH = 15, ORC_NEU_SOFTPLUS = 16, ORC_NEU_SQPLUS = 17, ORC_NEU_ATISH = 18 };
 
 /* Potential description.  S species; NP = S(S+1)/2 neighbour pairs stored in
  * TensorTri order p(j<=k) = k(k+1)/2 + j (src/mem/tensortri.hpp:196-201).

@@ -1,0 +1,128 @@
+"""FP64-evaluated network inputs under the reference trainer's input
+preconditioning (inpb = -<G>, inpw = 1/sigma(G); src/nnp/nnptefr2.cpp:776-867),
+against the FP64 oracle given the same scaling.
+
+On the rigid-geometry water workload two inputs per species are almost
+constant (<G>/sigma ~ 1e5): an FP32 G alone misses the force bar there by
+orders of magnitude.  Tolerances:
+  energies <= 1e-6 eV/atom, forces <= 1e-5 relative RMS (FAST and PLAIN);
+  every input FP64-evaluated (ratio < 0): G within 1e-10 x max |G| of the
+  oracle's G (the oracle's Pade exp and Taylor cosine differ from libm by
+  < 1e-11 relative), for every radial and angular family;
+  training gradient <= 1e-5 relative against the oracle's analytic gradient.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _engine(model, batch):
+    from ann_amd.gpu import Engine
+    eng = Engine(model, "cuda:0")
+    eng.set_batch(batch)
+    eng.neighbors()
+    return eng
+
+
+def _preconditioned(model, batch):
+    """Engine with MEAN / DEV scaling from its own input statistics, and the
+    same potential for the oracle."""
+    from ann_amd.precond import apply_preconditioning
+    eng = _engine(model, batch)
+    pre = apply_preconditioning(eng, "MEAN", "DEV")
+    m2 = model.copy()
+    for a, (w, b) in enumerate(pre):
+        m2.nets[a].inpw, m2.nets[a].inpb, m2.nets[a].outw, m2.nets[a].outb = w, b, 1.0, 0.0
+    return eng, m2, pre
+
+
+def _errors(eng, form, model, batch, oracle_mod):
+    E, F, _ = eng.energy_forces(form)
+    Eo, Fo, _ = oracle_mod.Oracle(model).compute_batch(batch, nthreads=4)
+    de = np.max(np.abs(E.cpu().numpy() - Eo) / batch.natoms)
+    rf = np.sqrt(((F.cpu().numpy() - Fo) ** 2).sum() / (Fo ** 2).sum())
+    return de, rf
+
+
+@pytest.mark.parametrize("form", [1, 0])
+def test_preconditioned_water_meets_the_bars(oracle_mod, form):
+    from ann_amd import configs
+    model, batch = configs.model_water(), configs.batch_water(4, seed=21)
+    print(f"water4 MEAN/DEV form {form}", flush=True)
+    eng, m2, pre = _preconditioned(model, batch)
+    ratio = [np.abs(w * b).max() for w, b in pre]
+    assert max(ratio) > 1e4                    # the workload does hit the ill-conditioned case
+    assert eng.precise_count() == 4            # the two short-range radial inputs of each species
+    de, rf = _errors(eng, form, m2, batch, oracle_mod)
+    print(f"  dE/atom={de:.3e} force relRMS={rf:.3e}", flush=True)
+    assert de <= 1e-6
+    assert rf <= 1e-5
+
+
+def test_fp32_inputs_alone_miss_the_bar(oracle_mod):
+    """The same potential with the FP64 inputs switched off: the error the
+    FP64 path removes (guards the test above against passing vacuously)."""
+    from ann_amd import configs
+    model, batch = configs.model_water(), configs.batch_water(4, seed=21)
+    eng, m2, _ = _preconditioned(model, batch)
+    assert eng.set_precise_ratio(1e30) == 0
+    de, rf = _errors(eng, 1, m2, batch, oracle_mod)
+    print(f"  FP32 only: dE/atom={de:.3e} force relRMS={rf:.3e}", flush=True)
+    assert rf > 1e-4
+    assert eng.set_precise_ratio(32.0) == 4
+    de, rf = _errors(eng, 1, m2, batch, oracle_mod)
+    assert rf <= 1e-5 and de <= 1e-6
+
+
+FAMILIES = [("GAUSSIAN", "GAUSS", "TANH"), ("SECH", "GAUSS2", "COS"), ("LOGISTIC", "SECH", "POLY3"),
+            ("TANH", "STUDENT3", "TANH"), ("LOGCOSH", "STUDENT4", "COS"), ("LOGCOSH2", "STUDENT5", "STEP")]
+
+
+@pytest.mark.parametrize("rad,ang,cut", FAMILIES)
+def test_every_input_in_fp64_matches_oracle_g(oracle_mod, rad, ang, cut):
+    from ann_amd import configs
+    model = configs.model_family(rad, ang, cutoff=cut)
+    batch = configs.batch_water(2, seed=31, nmol=16, L=8.0)
+    print(f"families {rad}/{ang}/{cut}", flush=True)
+    eng = _engine(model, batch)
+    n = eng.set_precise_ratio(-1.0)
+    assert n == sum(model.ninput(a) for a in range(model.nspecies))
+    G = eng.symm64().cpu().numpy()
+    orc = oracle_mod.Oracle(model)
+    off = batch.atom_off
+    for k in range(batch.nstruc):
+        Gr = orc.symm(*batch.struc(k))
+        err = np.abs(G[off[k]:off[k + 1], :Gr.shape[1]] - Gr).max() / np.abs(Gr).max()
+        assert err <= 1e-10, (k, err)
+
+
+def test_fp64_forward_pair_under_preconditioning(oracle_mod):
+    """ann_symm_fwd64 + ann_net_fwd64 (the C++ drop-in's NNP::symm / energy)."""
+    from ann_amd import configs
+    model, batch = configs.model_water(), configs.batch_water(2, seed=22)
+    eng, m2, _ = _preconditioned(model, batch)
+    G = eng.symm64(1)
+    E, dE = eng.net64(G, 1)
+    orc = oracle_mod.Oracle(m2)
+    Go = np.concatenate([orc.symm(*batch.struc(k)) for k in range(batch.nstruc)])
+    Eo, dEo = orc.net(batch.species, Go)
+    assert np.max(np.abs(E.cpu().numpy() - Eo)) <= 1e-6
+    d = dE.cpu().numpy()[:, :orc.ld]
+    assert np.sqrt(((d - dEo) ** 2).sum() / (dEo ** 2).sum()) <= 1e-5
+
+
+def test_preconditioned_training_gradient(oracle_mod):
+    import torch
+    from ann_amd import configs
+    model, batch = configs.model_water(), configs.batch_water(2, seed=23)
+    eng, m2, _ = _preconditioned(model, batch)
+    rng = np.random.default_rng(5)
+    Eref = rng.normal(0.0, 1.0, batch.nstruc) * batch.natoms * 0.01
+    Fref = rng.normal(0.0, 0.5, (batch.ntotal, 3))
+    dev = eng.device
+    loss, g = eng.train_step(torch.tensor(Eref, device=dev), torch.tensor(Fref, device=dev), 1.0, 0.1, 0.0)
+    L, go, _, _, parts = oracle_mod.Oracle(m2).train_loss_grad(batch, Eref, Fref, 1.0, 0.1)
+    gg = g.cpu().numpy()
+    assert np.abs(loss.cpu().numpy() - parts).max() <= 1e-6 * max(1.0, np.abs(parts).max())
+    assert np.linalg.norm(gg - go) / np.linalg.norm(go) <= 1e-5

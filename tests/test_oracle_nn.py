@@ -9,7 +9,8 @@ import torch
 
 from ann_amd.model import NEURON, Model, init_network
 
-NEURONS = ["TANH", "SOFTPLUS", "SIGMOID", "SWISH", "ARCTAN", "ELU", "SQPLUS"]
+NEURONS = ["TANH", "SOFTPLUS", "SIGMOID", "SWISH", "ARCTAN", "ELU", "SQPLUS", "ISRU", "TANHRE", "SQRE",
+           "GELU", "MISH", "PFLU", "SERF", "LOGISH", "ATISH"]
 
 
 def act(orc, neuron, c, z):
@@ -33,6 +34,26 @@ def torch_act(name, c, z):
         return torch.where(z >= 0, z, torch.expm1(c * z))
     if name == "SQPLUS":
         return 0.5 / c * (c * z - 1 + torch.sqrt(1 + (c * z) ** 2))
+    # the remaining neurons as the reference defines them (src/ml/nn.cpp:238-780;
+    # sqre, serf, logish and atish have no sharpness)
+    if name == "ISRU":
+        return c * z / torch.sqrt(1 + (c * z) ** 2)
+    if name == "TANHRE":
+        return torch.where(z >= 0, z, torch.tanh(c * z))
+    if name == "SQRE":
+        return torch.where(z >= 0, z, z / torch.sqrt(1 + z * z))
+    if name == "GELU":
+        return 0.5 * z * (1 + torch.erf(0.5 * np.sqrt(np.pi) * c * z))
+    if name == "MISH":
+        return z * torch.tanh(torch.nn.functional.softplus(c * z))
+    if name == "PFLU":
+        return 0.5 * z * (1 + c * z / torch.sqrt(1 + (c * z) ** 2))
+    if name == "SERF":
+        return z * torch.erf(torch.nn.functional.softplus(z))
+    if name == "LOGISH":
+        return z * torch.log(1 + torch.sigmoid(z)) / 0.693147180559945
+    if name == "ATISH":
+        return 0.5 * z * (1 + 2 / np.pi * torch.atan(np.pi / 2 * z))
     if name == "LINEAR":
         return z
     raise KeyError(name)
@@ -49,8 +70,8 @@ def test_activation_values_and_derivatives(oracle_mod, name):
         a, d, d2 = act(oracle_mod, NEURON[name], c, z)
         assert abs(a - a_t.item()) < 1e-12
         assert abs(d - d_t.item()) < 1e-12
-        if name == "ELU" and abs(z) < 1e-12:
-            continue
+        if name in ("ELU", "SQRE", "TANHRE") and abs(z) < 1e-12:
+            continue                    # a kink at z = 0: the second derivative jumps there
         assert abs(d2 - d2_t.item()) < 1e-10, (z, d2, d2_t.item())
 
 
@@ -116,7 +137,8 @@ def test_forward_dEdG_and_param_gradient_vs_autograd(oracle_mod, neuron):
         assert np.allclose(g, gE * gp.numpy(), atol=1e-12, rtol=1e-10)
 
 
-@pytest.mark.parametrize("neuron", ["TANH", "SOFTPLUS", "SWISH", "SIGMOID"])
+@pytest.mark.parametrize("neuron", ["TANH", "SOFTPLUS", "SWISH", "SIGMOID", "MISH", "GELU", "SERF", "LOGISH",
+                                    "ATISH", "ISRU", "PFLU", "TANHRE", "SQRE"])
 def test_second_backward_vs_double_autograd_and_fd(oracle_mod, neuron):
     """d/dp [gE E + dE/dG . v]: torch double backward (library) and central
     differences of the same scalar w.r.t. the parameters."""
