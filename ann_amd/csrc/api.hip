@@ -90,7 +90,7 @@ extern "C" const char* ann_last_error(void) { return g_err.c_str(); }
 static int update_precise(ann_ctx* ctx) {
     DevModel& M = ctx->hm;
     const size_t tot = ctx->h_inpw.size();
-    std::vector<int4> items;
+    std::vector<PreItem> items;
     std::vector<uint8_t> flag(std::max<size_t>(tot, 1), 0);
     std::vector<float> iw(tot), ib(tot), ibn(tot);
     for (int a = 0; a < M.S; ++a) {
@@ -104,16 +104,29 @@ static int update_precise(ann_ctx* ctx) {
             ibn[D.inoff + k] = f ? 0.0f : (float)b;
             if (!f) continue;
             flag[D.inoff + k] = 1;
-            int4 it = make_int4(k, -1, 0, 0);
-            for (int c = 0; c < M.S && it.y < 0; ++c) {
+            PreItem it{k, -1, 0, 0, 0, 0, 0, 0};
+            for (int c = 0; c < M.S && it.mode < 0; ++c) {
                 const RadBlock& rb = M.rad[a * M.S + c];
-                if (k >= rb.goff && k < rb.goff + rb.n) it = make_int4(k, 0, c, k - rb.goff);
+                if (k >= rb.goff && k < rb.goff + rb.n) { it.mode = 0; it.blk = c; it.fn = k - rb.goff; }
             }
-            for (int p = 0; p < M.NP && it.y < 0; ++p) {
+            for (int p = 0; p < M.NP && it.mode < 0; ++p) {
                 const AngBlock& ab = M.ang[a * M.NP + p];
-                if (k >= ab.goff && k < ab.goff + ab.n) it = make_int4(k, 1, p, k - ab.goff);
+                if (k < ab.goff || k >= ab.goff + ab.n) continue;
+                it.mode = 2;
+                it.blk = p;
+                it.fn = k - ab.goff;
+                if (M.moment_z > 0) {       // every block is a GAUSS grid with integer zeta <= kMomZ
+                    for (int g = 0; g < ab.ng; ++g)
+                        for (int q = 0; q < ab.nk; ++q)
+                            if (ab.outm[g * ab.nk + q] == it.fn) {
+                                it.mode = 1;
+                                it.u = ab.gu[g];
+                                it.zint = ab.zint[q];
+                                it.lam = ab.lamh[q] > 0 ? 1 : (ab.lamh[q] < 0 ? -1 : 0);
+                            }
+                }
             }
-            if (it.y < 0) return fail(ANN_ERR_ARG, "input outside every basis block");
+            if (it.mode < 0) return fail(ANN_ERR_ARG, "input outside every basis block");
             items.push_back(it);
         }
     }
@@ -129,7 +142,7 @@ static int update_precise(ann_ctx* ctx) {
         ANN_HIP(hipMemcpyAsync(ctx->da.preflag, flag.data(), tot, hipMemcpyHostToDevice, st));
     }
     if (!items.empty())
-        ANN_HIP(hipMemcpyAsync(ctx->da.pitems, items.data(), sizeof(int4) * items.size(), hipMemcpyHostToDevice, st));
+        ANN_HIP(hipMemcpyAsync(ctx->da.pitems, items.data(), sizeof(PreItem) * items.size(), hipMemcpyHostToDevice, st));
     ANN_HIP(hipMemcpyAsync(ctx->dm, &M, sizeof(DevModel), hipMemcpyHostToDevice, st));
     ANN_HIP(hipStreamSynchronize(st));
     ctx->scaling_gen++;
@@ -250,7 +263,10 @@ static int upload_model(ann_ctx* ctx, const ann_model_desc* d) {
             if (!grid) M.all_grid = 0;
         }
         M.nunion[a] = (int)uni.size();
-        for (size_t u = 0; u < uni.size(); ++u) M.uieta2[a][u] = (float)uni[u];
+        for (size_t u = 0; u < uni.size(); ++u) {
+            M.uieta2[a][u] = (float)uni[u];
+            M.uieta2_64[a][u] = uni[u];
+        }
         M.nin[a] = nin[a];
     }
     // the fast kernels are compiled per grid shape: every non-empty block must share it
@@ -384,7 +400,7 @@ static int upload_model(ann_ctx* ctx, const ann_model_desc* d) {
         ANN_HIP(up64(ctx->da.ang_zeta64, azeta64));
         ANN_HIP(up64(ctx->da.inpb64, ctx->h_inpb));
         const size_t nin_tot = std::max<size_t>(ctx->h_inpw.size(), 1);
-        ANN_HIP(hipMalloc(&ctx->da.pitems, sizeof(int4) * nin_tot));
+        ANN_HIP(hipMalloc(&ctx->da.pitems, sizeof(PreItem) * nin_tot));
         ANN_HIP(hipMalloc(&ctx->da.preflag, nin_tot));
     }
     ANN_HIP(hipMalloc(&ctx->da.params, sizeof(double) * ptot));

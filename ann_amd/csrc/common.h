@@ -81,6 +81,16 @@ struct DevModel {
     int fast_ng, fast_nk;                       // the common grid shape (0 if blocks differ)
     int moment_z;                               // largest zeta if the moment form applies, else 0
     int pre_off[kMaxSpecies + 1];               // FP64-evaluated inputs of species a: items [pre_off[a], pre_off[a+1])
+    double uieta2_64[kMaxSpecies][kMaxUnionEta];  // union widths 1/eta^2 in FP64 (FP64-evaluated inputs)
+};
+
+// one FP64-evaluated input of a species: its index k, how it is computed
+// (0 radial, 1 angular by FP64 moments, 2 angular by FP64 triplets), the
+// basis block (neighbour species / pair) and function within it; for the
+// moment form also the union width, integer zeta and lambda
+struct PreItem {
+    int k, mode, blk, fn;
+    int u, zint, lam, pad;
 };
 
 // device pointers of the variable-size tables
@@ -95,7 +105,7 @@ struct DevArrays {
     float* inpb = nullptr;
     double* params = nullptr;    // FP64 master parameters
     // inputs evaluated in FP64 (large |inpw inpb|): items (input, kind, block, function)
-    int4* pitems = nullptr;
+    PreItem* pitems = nullptr;
     uint8_t* preflag = nullptr;  // per input (inoff-indexed): 1 if evaluated in FP64
     double* inpb64 = nullptr;    // input bias in FP64
     float* inpb_net = nullptr;   // input bias the network applies to centred rows (0 where FP64-evaluated)
@@ -146,7 +156,7 @@ struct ann_ctx {
     std::vector<double> params;
     double rc64 = 0;                  // model cutoff in FP64
     std::vector<double> h_inpw, h_inpb;   // input scaling (FP64, inoff-indexed)
-    double pre_ratio = 32.0;          // inputs with |inpw inpb| above this are evaluated in FP64 (< 0: all)
+    double pre_ratio = 1.0;           // inputs with |inpw inpb| above this are evaluated in FP64 (< 0: all)
     int npre = 0;                     // number of FP64-evaluated inputs over all species
     uint64_t scaling_gen = 0;         // bumped when the input scaling or its FP64 set changes
     uint64_t batch_gen = 0;           // bumped by every ann_set_batch

@@ -148,11 +148,12 @@ int ann_force_reduce(ann_ctx* ctx, const float* d_P, const float* d_Eatom, int w
 int ann_symm_fwd64(ann_ctx* ctx, int form, double* d_G);
 int ann_net_fwd64(ann_ctx* ctx, int form, const double* d_G, float* d_Eatom, float* d_dEdG);
 /* Inputs whose scaled value inpw (G + inpb) an FP32 G cannot resolve are
- * evaluated in FP64: every input k with |inpw_k inpb_k| > ratio (default 32;
+ * evaluated in FP64: every input k with |inpw_k inpb_k| > ratio (default 1;
  * ratio < 0 flags every input).  The reference trainer's preconditioning
- * inpb = -<G>, inpw = 1/sigma(G) (src/nnp/nnptefr2.cpp:776-867) makes an
- * almost constant input reach |inpw inpb| ~ 1e5, where the ~1e-7 relative
- * rounding of an FP32 G becomes an O(1e-2) error of the network input.  The
+ * inpb = -<G>, inpw = 1/sigma(G) (src/nnp/nnptefr2.cpp:776-867) gives
+ * |inpw inpb| = <G>/sigma, which reaches ~1e5 for an almost constant input:
+ * the ~1e-7 relative rounding of an FP32 G then becomes an O(1e-2) error of
+ * the network input (and ratios of 2-20, common, cost the energy bar).  The
  * fused entry points (ann_energy_forces, ann_md_step, ann_train_step) and the
  * FP64 pair above use the FP64 values; ann_symm_fwd / ann_net_fwd stay FP32.
  * The set is recomputed by ann_set_scaling.  No counterpart in the reference

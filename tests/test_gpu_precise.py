@@ -3,8 +3,10 @@ preconditioning (inpb = -<G>, inpw = 1/sigma(G); src/nnp/nnptefr2.cpp:776-867),
 against the FP64 oracle given the same scaling.
 
 On the rigid-geometry water workload two inputs per species are almost
-constant (<G>/sigma ~ 1e5): an FP32 G alone misses the force bar there by
-orders of magnitude.  Tolerances:
+constant (<G>/sigma ~ 1e5) and most others have <G>/sigma of 2-20: an FP32 G
+alone misses the force bar by orders of magnitude and the energy bar by
+tens.  Inputs with |inpw inpb| > 1 (the default) are evaluated in FP64.
+Tolerances:
   energies <= 1e-6 eV/atom, forces <= 1e-5 relative RMS (FAST and PLAIN);
   every input FP64-evaluated (ratio < 0): G within 1e-10 x max |G| of the
   oracle's G (the oracle's Pade exp and Taylor cosine differ from libm by
@@ -53,7 +55,7 @@ def test_preconditioned_water_meets_the_bars(oracle_mod, form):
     eng, m2, pre = _preconditioned(model, batch)
     ratio = [np.abs(w * b).max() for w, b in pre]
     assert max(ratio) > 1e4                    # the workload does hit the ill-conditioned case
-    assert eng.precise_count() == 4            # the two short-range radial inputs of each species
+    assert eng.precise_count() == sum(int((np.abs(w * b) > 1.0).sum()) for w, b in pre)
     de, rf = _errors(eng, form, m2, batch, oracle_mod)
     print(f"  dE/atom={de:.3e} force relRMS={rf:.3e}", flush=True)
     assert de <= 1e-6
@@ -70,7 +72,7 @@ def test_fp32_inputs_alone_miss_the_bar(oracle_mod):
     de, rf = _errors(eng, 1, m2, batch, oracle_mod)
     print(f"  FP32 only: dE/atom={de:.3e} force relRMS={rf:.3e}", flush=True)
     assert rf > 1e-4
-    assert eng.set_precise_ratio(32.0) == 4
+    assert eng.set_precise_ratio(1.0) > 100
     de, rf = _errors(eng, 1, m2, batch, oracle_mod)
     assert rf <= 1e-5 and de <= 1e-6
 
@@ -107,9 +109,17 @@ def test_fp64_forward_pair_under_preconditioning(oracle_mod):
     orc = oracle_mod.Oracle(m2)
     Go = np.concatenate([orc.symm(*batch.struc(k)) for k in range(batch.nstruc)])
     Eo, dEo = orc.net(batch.species, Go)
-    assert np.max(np.abs(E.cpu().numpy() - Eo)) <= 1e-6
+    Eg = E.cpu().numpy().astype(np.float64)
+    # atom energies as in the network parity tests; structure energies per atom
+    err_atom = float(np.max(np.abs(Eg - Eo)))
+    assert err_atom <= 1e-6 * max(1.0, float(np.abs(Eo).max())), err_atom
+    off = batch.atom_off
+    err_struc = max(abs(Eg[off[k]:off[k + 1]].sum() - Eo[off[k]:off[k + 1]].sum()) / batch.natoms[k]
+                    for k in range(batch.nstruc))
+    assert err_struc <= 1e-6, err_struc
     d = dE.cpu().numpy()[:, :orc.ld]
-    assert np.sqrt(((d - dEo) ** 2).sum() / (dEo ** 2).sum()) <= 1e-5
+    err = float(np.sqrt(((d - dEo) ** 2).sum() / (dEo ** 2).sum()))
+    assert err <= 1e-5, err
 
 
 def test_preconditioned_training_gradient(oracle_mod):
@@ -124,5 +134,7 @@ def test_preconditioned_training_gradient(oracle_mod):
     loss, g = eng.train_step(torch.tensor(Eref, device=dev), torch.tensor(Fref, device=dev), 1.0, 0.1, 0.0)
     L, go, _, _, parts = oracle_mod.Oracle(m2).train_loss_grad(batch, Eref, Fref, 1.0, 0.1)
     gg = g.cpu().numpy()
-    assert np.abs(loss.cpu().numpy() - parts).max() <= 1e-6 * max(1.0, np.abs(parts).max())
-    assert np.linalg.norm(gg - go) / np.linalg.norm(go) <= 1e-5
+    assert np.allclose(loss.cpu().numpy(), parts, rtol=1e-6, atol=0), (loss.cpu().numpy(), parts)
+    err = float(np.linalg.norm(gg - go) / np.linalg.norm(go))
+    print(f"  gradient rel err {err:.2e}", flush=True)
+    assert err <= 1e-5, err
