@@ -428,6 +428,12 @@ extern "C" int ann_create(const ann_model_desc* desc, ann_ctx** out) {
     if (hipGetDevice(&dev) != hipSuccess) return fail(ANN_ERR_HIP, "no HIP device");
     ann_ctx* ctx = new ann_ctx;
     ctx->device = dev;
+    // the allocation-free entry point's scratch address on this device
+    const hipError_t es = noalloc_init(dev);
+    if (es != hipSuccess) {
+        delete ctx;
+        return fail(ANN_ERR_HIP, std::string("scratch address: ") + hipGetErrorString(es));
+    }
     const int rc = upload_model(ctx, desc);
     if (rc != ANN_OK) {
         ann_destroy(ctx);
@@ -806,6 +812,19 @@ extern "C" int ann_force_reduce(ann_ctx* ctx, const float* P, const float* Eatom
     if (!P || (E && !Eatom)) return fail(ANN_ERR_ARG, "null argument");
     RET_HIP(ensure_work(ctx), "allocation");
     RET_HIP(force_reduce(ctx, P, Eatom, with_evac, F, E, V), "force_reduce");
+    return ANN_OK;
+}
+
+extern "C" int ann_force_reduce_noalloc(ann_ctx* ctx, const float* P, const float* Eatom, int with_evac,
+                                        double* F, double* E, double* V) {
+    int rc = need_nlist(ctx); if (rc) return rc;
+    if (!P || (E && !Eatom)) return fail(ANN_ERR_ARG, "null argument");
+    const int64_t need = noalloc_need(ctx, V != nullptr);
+    if (need > kNoAllocScratch)
+        return fail(ANN_ERR_CAPACITY, "ann_force_reduce_noalloc: the batch needs " + std::to_string(need) +
+                                          " doubles of scratch; the static scratch holds " +
+                                          std::to_string(kNoAllocScratch) + " (32 MiB; 9 per atom with the virial)");
+    RET_HIP(force_reduce_noalloc(ctx, P, Eatom, with_evac, F, E, V), "force_reduce_noalloc");
     return ANN_OK;
 }
 

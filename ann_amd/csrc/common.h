@@ -24,6 +24,10 @@ constexpr int kMomZ = 4;           // largest integer zeta of the moment form
 constexpr int kSumChunk = 4096;    // atoms per block of the per-structure sums
 constexpr int kSumQ = 11;          // summed quantities: energy, virial[9], kinetic energy
 constexpr int kReadbackWords = 16; // pinned host words of a context (flags and counts for the host)
+constexpr int kMaxDevices = 64;    // devices a process may create contexts on
+// doubles of the static scratch of the allocation-free force reduction (32 MiB
+// per GPU): 9 per atom for the virial rows plus 11 per chunk partial
+constexpr int64_t kNoAllocScratch = int64_t(4) << 20;
 
 // Workgroups are dispatched round robin over the 8 XCDs, each with its own L2.
 // Map dispatch order b of nb workgroups to a logical index so that XCD x runs
@@ -324,6 +328,12 @@ hipError_t force_reduce(ann_ctx* ctx, const float* P, const float* Eatom, int wi
                         double* E, double* V);
 hipError_t struc_sums(ann_ctx* ctx, const float* Eatom, int with_evac, const double* avir, const double* keatom,
                       double* E, double* V, double* KE);
+// allocation-free force reduction (static scratch): set-up once per device
+// (that device current), scratch words a call needs, and the call itself
+hipError_t noalloc_init(int device);
+int64_t noalloc_need(const ann_ctx* ctx, bool virial);
+hipError_t force_reduce_noalloc(ann_ctx* ctx, const float* P, const float* Eatom, int with_evac, double* F,
+                                double* E, double* V);
 hipError_t refresh_geometry(ann_ctx* ctx);
 hipError_t wrap_positions(ann_ctx* ctx, double* pos);
 hipError_t md_kick_drift(ann_ctx* ctx, double dt, const double* mass, const double* F, double* vel, double* pos,

@@ -12,6 +12,8 @@
 // W = -sum_slots P (x) r are FP64 tree reductions over fixed atom chunks.
 // HBM-bound gather: 12 B (own P) + 12 B (reverse P) + 4 B (rev) + 16 B
 // (geometry, virial) per slot.
+#include <mutex>
+
 #include "common.h"
 
 namespace ann {
@@ -150,37 +152,96 @@ __global__ void k_chunk_combine(int S, const int* __restrict__ chunk_off, const 
     else KE[s] = acc;
 }
 
-hipError_t struc_sums(ann_ctx* ctx, const float* Eatom, int with_evac, const double* avir, const double* keatom,
-                      double* E, double* V, double* KE) {
+// the chunk sums with the chunk partials in `part` (the context's, or scratch)
+static hipError_t launch_sums(ann_ctx* ctx, const float* Eatom, int with_evac, const double* avir,
+                              const double* keatom, double* part, double* E, double* V, double* KE) {
     const int qmask = (E ? 1 : 0) | (V && avir ? 2 : 0) | (KE && keatom ? 4 : 0);
     if (!qmask || ctx->natoms == 0) return hipSuccess;
-    Launch L(ctx, "struc_sums");
     const bool direct = ctx->nchunks == ctx->nstruc;
     k_chunk_sums<<<ctx->nchunks, 256, 0, ctx->stream>>>(ctx->d_chunks, Eatom, ctx->d_species, ctx->dm, with_evac,
-                                                       avir, keatom, qmask, direct ? nullptr : ctx->d_part, E, V,
-                                                       KE);
+                                                       avir, keatom, qmask, direct ? nullptr : part, E, V, KE);
     if (!direct)
-        k_chunk_combine<<<(ctx->nstruc + 15) / 16, 256, 0, ctx->stream>>>(ctx->nstruc, ctx->d_chunk_off,
-                                                                          ctx->d_part, qmask, E, V, KE);
+        k_chunk_combine<<<(ctx->nstruc + 15) / 16, 256, 0, ctx->stream>>>(ctx->nstruc, ctx->d_chunk_off, part, qmask,
+                                                                          E, V, KE);
     return hipGetLastError();
+}
+
+// the per-atom gather of forces (and virial rows into avir)
+static hipError_t launch_gather(ann_ctx* ctx, const float* P, double* F, double* avir) {
+    const int N = ctx->natoms;
+    if (avir)
+        k_force_atoms<true><<<(N + 15) / 16, 256, 0, ctx->stream>>>(N, ctx->nl.off, ctx->nl.rev, ctx->nl.geo, P, F,
+                                                                    avir);
+    else
+        k_force_atoms<false><<<(N + 15) / 16, 256, 0, ctx->stream>>>(N, ctx->nl.off, ctx->nl.rev, ctx->nl.geo, P, F,
+                                                                     nullptr);
+    return hipGetLastError();
+}
+
+hipError_t struc_sums(ann_ctx* ctx, const float* Eatom, int with_evac, const double* avir, const double* keatom,
+                      double* E, double* V, double* KE) {
+    if (!(E || (V && avir) || (KE && keatom)) || ctx->natoms == 0) return hipSuccess;
+    Launch L(ctx, "struc_sums");
+    return launch_sums(ctx, Eatom, with_evac, avir, keatom, ctx->d_part, E, V, KE);
 }
 
 hipError_t force_reduce(ann_ctx* ctx, const float* P, const float* Eatom, int with_evac, double* F, double* E,
                         double* V) {
     const int N = ctx->natoms;
     if (N == 0) return hipSuccess;
+    hipError_t e;
     {
         Launch L(ctx, "force_gather");
-        if (V)
-            k_force_atoms<true><<<(N + 15) / 16, 256, 0, ctx->stream>>>(N, ctx->nl.off, ctx->nl.rev, ctx->nl.geo, P,
-                                                                        F, ctx->d_atomvir);
-        else
-            k_force_atoms<false><<<(N + 15) / 16, 256, 0, ctx->stream>>>(N, ctx->nl.off, ctx->nl.rev, ctx->nl.geo, P,
-                                                                         F, nullptr);
+        e = launch_gather(ctx, P, F, V ? ctx->d_atomvir : nullptr);
     }
-    hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return struc_sums(ctx, Eatom, with_evac, V ? ctx->d_atomvir : nullptr, nullptr, E, V, nullptr);
+}
+
+// ---- allocation-free force reduction ----
+//
+// For callers that cannot allocate device memory (code under HIP-graph
+// capture, tight serving loops): the per-atom virial rows and the chunk
+// partials live in this file-scope array instead of context buffers.  Each GPU
+// holds its own copy for the life of the process; its device address is taken
+// once per device at context creation.  Every call on a GPU shares the array,
+// so calls that overlap on one GPU race.  Nothing needs clearing: each call
+// writes every scratch word it reads before reading it.
+static __device__ __align__(256) double scratch[kNoAllocScratch];
+
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices] = {};
+
+hipError_t noalloc_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    double* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void**)&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = p;
+    return hipSuccess;
+}
+
+int64_t noalloc_need(const ann_ctx* ctx, bool virial) {
+    return (virial ? 9 * (int64_t)ctx->natoms : 0) +
+           (ctx->nchunks != ctx->nstruc ? (int64_t)kSumQ * ctx->nchunks : 0);
+}
+
+hipError_t force_reduce_noalloc(ann_ctx* ctx, const float* P, const float* Eatom, int with_evac, double* F, double* E,
+                                double* V) {
+    double* scr = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        if (ctx->device >= 0 && ctx->device < kMaxDevices) scr = g_scratch_ptr[ctx->device];
+    }
+    if (!scr) return hipErrorNotInitialized;
+    if (ctx->natoms == 0) return hipSuccess;
+    double* avir = V ? scr : nullptr;
+    double* part = scr + (V ? 9 * (size_t)ctx->natoms : 0);
+    hipError_t e = launch_gather(ctx, P, F, avir);
+    if (e != hipSuccess) return e;
+    return launch_sums(ctx, Eatom, with_evac, avir, nullptr, part, E, V, nullptr);
 }
 
 }  // namespace ann

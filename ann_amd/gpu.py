@@ -57,7 +57,7 @@ EXPORTS = ["ann_last_error", "ann_create", "ann_destroy", "ann_set_stream", "ann
            "ann_profile_get", "ann_profile_only", "ann_refresh_geometry", "ann_wrap_positions", "ann_md_step", "ann_train_graph", "ann_ef_graph",
            "ann_autotune", "ann_get_tuning", "ann_set_tuning", "ann_input_stats",
            "ann_get_optimizer_state", "ann_set_optimizer_state", "ann_symm_fwd64", "ann_net_fwd64",
-           "ann_set_precise_ratio", "ann_precise_count"]
+           "ann_set_precise_ratio", "ann_precise_count", "ann_force_reduce_noalloc"]
 
 
 def lib():
@@ -255,6 +255,18 @@ class Engine:
         V = torch.empty((self.S, 3, 3), dtype=torch.float64, device=self.device)
         _check(lib().ann_force_reduce(self.h, _ptr(P.contiguous()), _ptr(Eatom), int(with_evac), _ptr(F),
                                       _ptr(E), _ptr(V)))
+        return E, F, V
+
+    def force_reduce_noalloc(self, P: torch.Tensor, Eatom: torch.Tensor, with_evac: bool = True, out=None,
+                             virial: bool = True):
+        """ann_force_reduce_noalloc into caller-owned buffers ``out`` = (E, F, V)."""
+        if out is None:
+            out = (torch.empty(self.S, dtype=torch.float64, device=self.device),
+                   torch.empty((self.N, 3), dtype=torch.float64, device=self.device),
+                   torch.empty((self.S, 3, 3), dtype=torch.float64, device=self.device))
+        E, F, V = out
+        _check(lib().ann_force_reduce_noalloc(self.h, _ptr(P.contiguous()), _ptr(Eatom), int(with_evac), _ptr(F),
+                                              _ptr(E), _ptr(V if virial else None)))
         return E, F, V
 
     def energy_forces(self, form: int = FAST, with_evac: bool = True, out=None):

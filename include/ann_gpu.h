@@ -138,6 +138,18 @@ int ann_symm_bwd(ann_ctx* ctx, int form, const float* d_dEdG, float* d_P);
  * NNP::energy), virial W = -sum P (x) r; any output may be NULL */
 int ann_force_reduce(ann_ctx* ctx, const float* d_P, const float* d_Eatom, int with_evac,
                      double* d_F, double* d_E, double* d_virial);
+/* ann_force_reduce without any allocation: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code under HIP-graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and results as ann_force_reduce; the per-atom virial rows
+ * and the per-chunk partial sums it needs live in a static per-GPU scratch
+ * array of 32 MiB (4 Mi doubles: 9 per atom when d_virial is set, plus 11 per
+ * chunk of at most 4096 atoms when a structure spans several chunks); larger
+ * batches are rejected with ANN_ERR_CAPACITY.  Calls that overlap on one GPU
+ * (any context) share that scratch and race.  Kernel timing (ann_profile)
+ * does not cover this entry point. */
+int ann_force_reduce_noalloc(ann_ctx* ctx, const float* d_P, const float* d_Eatom, int with_evac,
+                             double* d_F, double* d_E, double* d_virial);
 /* The same forward pair with G in FP64, as the reference stores it
  * (Structure::symm, a VecXd per atom; NNP::symm src/nnp/nnp.cpp:333-375 and
  * NNP::energy :436-447, which applies inpw (G + inpb) in FP64,
