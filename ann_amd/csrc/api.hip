@@ -1021,6 +1021,26 @@ extern "C" int ann_train_step(ann_ctx* ctx, int form, const double* Eref, const 
     return ANN_OK;
 }
 
+extern "C" int ann_set_loss(ann_ctx* ctx, int loss, double delta, int norm) {
+    CHECK_CTX(ctx);
+    if (loss < ANN_LOSS_MSE || loss > ANN_LOSS_ASINH) return fail(ANN_ERR_ARG, "invalid loss");
+    if (norm < ANN_NORM_IDENTITY || norm > ANN_NORM_LOG) return fail(ANN_ERR_ARG, "invalid normalization");
+    if (!(delta > 0) || std::isinf(delta)) return fail(ANN_ERR_ARG, "loss width delta must be positive");
+    ctx->loss_type = loss;
+    ctx->loss_delta = delta;
+    ctx->loss_norm = norm;
+    ctx->gen++;                   // a captured training step bakes the loss in: capture again
+    return ANN_OK;
+}
+
+extern "C" int ann_set_struc_weights(ann_ctx* ctx, const double* wE, const double* wF) {
+    CHECK_CTX(ctx);
+    ctx->d_wEs = wE;
+    ctx->d_wFs = wF;
+    ctx->gen++;
+    return ANN_OK;
+}
+
 extern "C" int ann_train_graph(ann_ctx* ctx, int enable) {
     CHECK_CTX(ctx);
     if (enable && !ctx->gstream) {

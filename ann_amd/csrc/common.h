@@ -165,6 +165,11 @@ struct ann_ctx {
     uint64_t scaling_gen = 0;         // bumped when the input scaling or its FP64 set changes
     uint64_t batch_gen = 0;           // bumped by every ann_set_batch
     int nb_status = 0;                // ANN_* status of the last neighbour build (capacity, image limit, symmetry)
+    // training loss (ann_set_loss / ann_set_struc_weights)
+    int loss_type = ANN_LOSS_MSE, loss_norm = ANN_NORM_LINEAR;
+    double loss_delta = 1.0;
+    const double* d_wEs = nullptr;    // per-structure energy / force weights (caller's device arrays) or null
+    const double* d_wFs = nullptr;
     // device model
     ann::DevModel* dm = nullptr;
     ann::DevArrays da;

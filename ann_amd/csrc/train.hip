@@ -768,14 +768,56 @@ hipError_t net_bwd2(ann_ctx* ctx, const float* G, const float* v, const double* 
 }
 
 // ---- loss: one block per structure, FP64 ----
+// Energy loss on dU = (E - Eref) norm(N) (NNPTEFR::error_cost_o1,
+// src/nnp/nnptefr2.cpp:1336-1372): MSE dU^2/2, MAE |dU|, HUBER
+// delta^2 (sqrt(1 + a^2) - 1), ASINH delta^2 (1 - sqrt(1 + a^2) + a asinh a),
+// a = dU / delta; norm IDENTITY 1, LINEAR 1/N, SQRT, CBRT, LOG 1/(ln N + 1)
+// (:602-612).  Per-structure weights scale both terms (a structure without
+// reference forces, as the Taylor-augmented copies of :2466-2473, has force
+// weight 0).
+struct LossCfg {
+    int loss, norm;
+    double delta;
+    const double* wEs;
+    const double* wFs;
+};
+
+__device__ inline void energy_loss(const LossCfg& c, double dU, double& l, double& g) {
+    switch (c.loss) {
+        case ANN_LOSS_MAE: l = fabs(dU); g = (double)((dU > 0) - (dU < 0)); break;
+        case ANN_LOSS_HUBER: {
+            const double a = dU / c.delta, q = sqrt(1.0 + a * a);
+            l = c.delta * c.delta * (q - 1.0);
+            g = dU / q;
+        } break;
+        case ANN_LOSS_ASINH: {
+            const double a = dU / c.delta, q = sqrt(1.0 + a * a), as = asinh(a);
+            l = c.delta * c.delta * (1.0 - q + a * as);
+            g = c.delta * as;
+        } break;
+        default: l = 0.5 * dU * dU; g = dU; break;
+    }
+}
+
+__device__ inline double energy_norm(int norm, double n) {
+    switch (norm) {
+        case ANN_NORM_IDENTITY: return 1.0;
+        case ANN_NORM_SQRT: return 1.0 / sqrt(n);
+        case ANN_NORM_CBRT: return 1.0 / cbrt(n);
+        case ANN_NORM_LOG: return 1.0 / (log(n) + 1.0);
+        default: return 1.0 / n;
+    }
+}
+
 __global__ __launch_bounds__(256) void k_loss(const int* __restrict__ aoff, const double* __restrict__ E,
                                               const double* __restrict__ F, const double* __restrict__ Eref,
-                                              const double* __restrict__ Fref, double wE, double wF, int B,
+                                              const double* __restrict__ Fref, double wE0, double wF0, int B,
                                               double* __restrict__ lossE, double* __restrict__ lossF,
-                                              double* __restrict__ gE, float* __restrict__ uneg) {
+                                              double* __restrict__ gE, float* __restrict__ uneg, LossCfg cfg) {
     __shared__ double red[256];
     const int s = blockIdx.x, t = threadIdx.x;
     const int a0 = aoff[s], a1 = aoff[s + 1], N = a1 - a0;
+    const double wE = wE0 * (cfg.wEs ? cfg.wEs[s] : 1.0), wF = wF0 * (cfg.wFs ? cfg.wFs[s] : 1.0);
     const double scale = wF * 2.0 / (3.0 * N) / B;
     double acc = 0.0;
     for (int x = 3 * a0 + t; x < 3 * a1; x += blockDim.x) {
@@ -790,10 +832,12 @@ __global__ __launch_bounds__(256) void k_loss(const int* __restrict__ aoff, cons
         __syncthreads();
     }
     if (t == 0) {
-        const double dEn = (E[s] - Eref[s]) / N;
-        lossE[s] = 0.5 * wE * dEn * dEn / B;
+        const double nrm = energy_norm(cfg.norm, (double)N);
+        double l, g;
+        energy_loss(cfg, (E[s] - Eref[s]) * nrm, l, g);
+        lossE[s] = wE * l / B;
         lossF[s] = wF * red[0] / (3.0 * N) / B;
-        gE[s] = wE * dEn / N / B;
+        gE[s] = wE * g * nrm / B;
     }
 }
 
@@ -859,8 +903,9 @@ hipError_t train_step_impl(ann_ctx* ctx, int form, const double* Eref, const dou
     double* g = grad ? grad : lt + 2;
     {
         Launch L(ctx, "loss");
+        const LossCfg cfg{ctx->loss_type, ctx->loss_norm, ctx->loss_delta, ctx->d_wEs, ctx->d_wFs};
         k_loss<<<B, 256, 0, ctx->stream>>>(ctx->d_atom_off, ctx->d_E, ctx->d_F, Eref, Fref, wE, wF, B, lE, lF,
-                                           ctx->d_gE, ctx->d_u);
+                                           ctx->d_gE, ctx->d_u, cfg);
         k_loss_total<<<1, 256, 0, ctx->stream>>>(B, lE, lF, loss ? loss : lt);
     }
     if ((e = hipGetLastError()) != hipSuccess) return e;

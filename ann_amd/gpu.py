@@ -57,7 +57,8 @@ EXPORTS = ["ann_last_error", "ann_create", "ann_destroy", "ann_set_stream", "ann
            "ann_profile_get", "ann_profile_only", "ann_refresh_geometry", "ann_wrap_positions", "ann_md_step", "ann_train_graph", "ann_ef_graph",
            "ann_autotune", "ann_get_tuning", "ann_set_tuning", "ann_input_stats",
            "ann_get_optimizer_state", "ann_set_optimizer_state", "ann_symm_fwd64", "ann_net_fwd64",
-           "ann_set_precise_ratio", "ann_precise_count", "ann_force_reduce_noalloc"]
+           "ann_set_precise_ratio", "ann_precise_count", "ann_force_reduce_noalloc", "ann_set_loss",
+           "ann_set_struc_weights"]
 
 
 def lib():
@@ -186,6 +187,22 @@ class Engine:
                                 _ptr(self.d_cells), _ptr(self.d_species))
         _check(lib().ann_set_batch(self.h, C.byref(self._bdesc)))
 
+    def set_batch_tensors(self, atom_off: np.ndarray, pos: torch.Tensor, cells: torch.Tensor,
+                          species: torch.Tensor) -> None:
+        """ann_set_batch on arrays already in device memory (positions [N, 3] FP64,
+        cells [S, 9] FP64, species [N] int32; host atom offsets [S + 1])."""
+        self.batch = None
+        self._off_h = np.ascontiguousarray(atom_off, dtype=np.int32)
+        self.d_off = torch.from_numpy(self._off_h).to(self.device, non_blocking=True)
+        self.d_pos = pos.contiguous()
+        self.d_cells = cells.contiguous()
+        self.d_species = species.to(torch.int32).contiguous()
+        self.N = int(self._off_h[-1])
+        self.S = len(self._off_h) - 1
+        self._bdesc = BatchDesc(self.S, self._off_h.ctypes.data_as(P_I), _ptr(self.d_off), _ptr(self.d_pos),
+                                _ptr(self.d_cells), _ptr(self.d_species))
+        _check(lib().ann_set_batch(self.h, C.byref(self._bdesc)))
+
     def neighbors(self, rc: float = 0.0) -> None:
         _check(lib().ann_neighbors(self.h, C.c_double(rc)))
 
@@ -300,6 +317,19 @@ class Engine:
         _check(lib().ann_train_step(self.h, form, _ptr(Eref.contiguous()), _ptr(Fref.contiguous()),
                                     C.c_double(wE), C.c_double(wF), C.c_double(gamma), _ptr(loss), _ptr(g)))
         return loss, g
+
+    LOSS = {"MSE": 1, "MAE": 2, "HUBER": 3, "ASINH": 4}
+    NORM = {"IDENTITY": 0, "LINEAR": 1, "SQRT": 2, "CBRT": 3, "LOG": 4}
+
+    def set_loss(self, loss: str = "MSE", delta: float = 1.0, norm: str = "LINEAR") -> None:
+        """Energy loss of the training step (ann_set_loss)."""
+        _check(lib().ann_set_loss(self.h, self.LOSS[loss.upper()], C.c_double(delta), self.NORM[norm.upper()]))
+
+    def set_struc_weights(self, wE: Optional[torch.Tensor] = None, wF: Optional[torch.Tensor] = None) -> None:
+        """Per-structure energy / force weights (FP64 device tensors [nstruc], kept alive here)."""
+        self._wts = (None if wE is None else wE.to(torch.float64).contiguous(),
+                     None if wF is None else wF.to(torch.float64).contiguous())
+        _check(lib().ann_set_struc_weights(self.h, _ptr(self._wts[0]), _ptr(self._wts[1])))
 
     def train_graph(self, enable: bool = True) -> None:
         """Capture the training step as a HIP graph after its first call and replay it."""

@@ -48,7 +48,10 @@ def work_model(model, batch, nl):
       symm_fwd_mom = P(10 + 6 nR) + sum_i [n_i (M(2 + 2U) + 2U) + NP U 3M + sum_f 2(zeta_f + 1)]
       symm_bwd_mom = P(20 + 6 nR) + sum_i [n_i (M(4U + 16) + 20) + NP U (Z+1) 2 nk + S M U 2S]
                      (the moments themselves come from the G kernel)
+      symm_tan_mom = P(20 + 8 nR) + sum_i [n_i M (9 + 6U) + NP U 6M + sum_f 2(zeta_f + 1)]
+      symm_tan (triplet tiles / PLAIN) = symm_fwd + symm_bwd
     networks: 4 MAC per atom and parameter-matrix entry (forward + input backward);
+    second backward (net_bwd2) 8 MAC, parameter-gradient GEMMs (grad_reduce) 4 MAC;
     HBM kernels in bytes: nb_search 24 B/atom + 8 B/slot, nb_compact (CSR fill and
     reverse slots) 28 B/atom + 57 B/slot, force_gather 96 B/atom + 44 B/slot, struc_sums
     76 B/atom."""
@@ -56,7 +59,7 @@ def work_model(model, batch, nl):
     nsp = batch.species[nl["idx"]]
     S = model.nspecies
     NP = S * (S + 1) // 2
-    fwd = bwd = fwd_m = bwd_m = 0.0
+    fwd = bwd = fwd_m = bwd_m = tan_m = 0.0
     P = T = 0
     for i in range(batch.ntotal):
         a = batch.species[i]
@@ -85,6 +88,9 @@ def work_model(model, batch, nl):
         # the adjoint reads the moments the G kernel stored (no moment sums of its own)
         bwd_m += n_i * 20 + 6 * pair + n_i * (M * (4 * U + 16) + 20) \
             + NP * U * (Z + 1) * 2 * nk + S * M * U * 2 * S
+        # tangent (forward-mode G): per (neighbour, monomial) the monomial, its
+        # directional derivative and U value / derivative accumulations
+        tan_m += n_i * 20 + 8 * pair + n_i * M * (9 + 6 * U) + NP * U * 6 * M + 2 * zsum
     mac = 0
     for i in range(batch.ntotal):
         n = model.nets[batch.species[i]]
@@ -93,7 +99,9 @@ def work_model(model, batch, nl):
             mac += prev * h
             prev = h
     N = batch.ntotal
-    return {"symm_fwd": fwd, "symm_bwd": bwd, "symm_fwd_mom": fwd_m, "symm_bwd_mom": bwd_m, "net": 4.0 * mac,
+    return {"symm_fwd": fwd, "symm_bwd": bwd, "symm_fwd_mom": fwd_m, "symm_bwd_mom": bwd_m,
+            "symm_tan": fwd + bwd, "symm_tan_mom": tan_m, "net": 4.0 * mac,
+            "net_bwd2": 8.0 * mac, "grad_reduce": 4.0 * mac,
             "pairs": P, "triplets": T,
             "bytes": {"nb_search": 24.0 * N + 8.0 * P, "nb_compact": 28.0 * N + 57.0 * P,
                       "force_gather": 96.0 * N + 44.0 * P, "struc_sums": 76.0 * N}}
@@ -113,6 +121,11 @@ def kernel_rooflines(kern, work):
             ach = fl / sec / 1e12
             out[k] = {"bound": "valu", "work": fl, "unit": "TFLOP/s", "achieved": ach, "peak": FP32_PEAK_TFLOPS,
                       "frac": ach / FP32_PEAK_TFLOPS}
+        elif k in ("net_bwd2", "grad_reduce"):
+            # FP32 matrix-core products (v_mfma_f32_16x16x4_f32); peak = the FP32 figure
+            ach = work[k] / sec / 1e12
+            out[k] = {"bound": "mfma", "work": work[k], "unit": "TFLOP/s", "achieved": ach,
+                      "peak": FP32_PEAK_TFLOPS, "frac": ach / FP32_PEAK_TFLOPS}
         elif k.startswith("net_"):
             ach = work["net"] / sec / 1e12
             peak = MFMA_BF16_PEAK_TFLOPS if k == "net_mfma" else FP32_PEAK_TFLOPS
@@ -139,19 +152,27 @@ def time_ef(eng, form, steps, warmup, out):
     return (time.perf_counter() - t0) / steps
 
 
-def run_train(steps, warmup, form, graph=False):
-    import torch
+def train_setup(nstruc=128, seed=34):
+    """Li/P/S training workload: model, seeded teacher labels (E without vacuum
+    energies, F) from the teacher network on the GPU."""
     from ann_amd import configs
     from ann_amd.gpu import Engine, FAST
     model = configs.model_lips()
     teacher = configs.model_lips(seed=33)
-    batch = configs.batch_lips(128, seed=34)
+    batch = configs.batch_lips(nstruc, seed=seed)
     te = Engine(teacher)
     te.set_batch(batch)
     te.neighbors()
-    Eref, Fref, _ = te.energy_forces(form, with_evac=False)
+    Eref, Fref, _ = te.energy_forces(FAST, with_evac=False)
     Eref, Fref = Eref.clone(), Fref.clone()
     del te
+    return model, batch, Eref, Fref
+
+
+def run_train(steps, warmup, form, graph=False, kernels=False):
+    import torch
+    from ann_amd.gpu import Engine, FAST
+    model, batch, Eref, Fref = train_setup()
     eng = Engine(model)
     eng.set_batch(batch)
     eng.neighbors()
@@ -172,9 +193,36 @@ def run_train(steps, warmup, form, graph=False):
         loss, _ = eng.train_step(Eref, Fref, 1.0, 0.1, 1e-3, form, out=out)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / steps
-    return {"structs_per_s": batch.nstruc / dt, "ms_per_step": dt * 1e3, "batch": batch.nstruc,
-            "atoms": batch.ntotal, "loss_first": (loss0.tolist() if loss0 is not None else None),
-            "loss_last": loss.cpu().numpy().tolist()}
+    res = {"structs_per_s": batch.nstruc / dt, "ms_per_step": dt * 1e3, "batch": batch.nstruc,
+           "atoms": batch.ntotal, "loss_first": (loss0.tolist() if loss0 is not None else None),
+           "loss_last": loss.cpu().numpy().tolist()}
+    if kernels:
+        # per-kernel table of the training step (events around every launch, a
+        # separate pass), then the dominant kernel timed alone over the steps
+        eng.profile(True)
+        for _ in range(max(3, steps // 2)):
+            eng.train_step(Eref, Fref, 1.0, 0.1, 1e-3, form, out=out)
+        kern = {}
+        for k in ("symm_fwd_mom", "symm_bwd_mom", "symm_tan_mom", "symm_fwd_fast", "symm_bwd_fast", "symm_tan_fast",
+                  "net_mfma", "net_bwd2", "grad_reduce", "loss", "adam", "force_gather", "struc_sums"):
+            ms, n = eng.profile_get(k)
+            if n:
+                kern[k] = {"avg_us": ms / n * 1e3, "launches": n}
+        eng.profile(False)
+        dom = max(kern, key=lambda k: kern[k]["avg_us"])
+        eng.profile(True)
+        eng.profile_only(dom)
+        for _ in range(steps):
+            eng.train_step(Eref, Fref, 1.0, 0.1, 1e-3, form, out=out)
+        ms, n = eng.profile_get(dom)
+        eng.profile(False)
+        eng.profile_only("")
+        nl = eng.nlist()
+        work = work_model(model, batch, nl)
+        res["kernels"] = kern
+        res["dominant"] = {"kernel": dom, "avg_launch_us": ms / n * 1e3 if n else kern[dom]["avg_us"]}
+        res["work"] = work
+    return res
 
 
 def run_uneven(steps, warmup):
@@ -200,6 +248,13 @@ def run_uneven(steps, warmup):
         dt = time_ef(eng, form, k, max(1, warmup // 2), out)
         res[name] = {"atoms_per_s": batch.ntotal / dt, "ms_per_step": dt * 1e3, "steps": k}
     res["fast_over_plain"] = res["fast"]["atoms_per_s"] / res["plain"]["atoms_per_s"]
+    # correctness at full size: the 4 smallest and 4 largest structures of the
+    # batch against the FP64 oracle (FAST form, the timed configuration)
+    E, F, _ = eng.energy_forces(FAST, out=out)
+    E, F = E.cpu().numpy(), F.cpu().numpy()
+    order = np.argsort(batch.natoms, kind="stable")
+    pick = np.concatenate([order[:4], order[-4:]])
+    res["errors_vs_fp64_oracle"] = oracle_errors(model, batch, pick, E, F)
     # the same FAST step with the E+F kernels replayed from a captured HIP graph
     eng.ef_graph(True)
     dt = time_ef(eng, FAST, steps, max(2, warmup // 2), out)
@@ -260,6 +315,148 @@ def run_md(nsteps, stride, plain_steps, skin=0.5, dt=0.02, reps=16):
         torch.cuda.empty_cache()
     out["fast_over_plain"] = out["fast"]["atom_steps_per_s"] / out["plain"]["atom_steps_per_s"]
     return out
+
+
+def oracle_errors(model, batch, pick, E, F):
+    """Energy (eV/atom) and force (relative RMS) errors of GPU results E, F on
+    the structures `pick` of `batch` against the FP64 oracle."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    sub = batch.subset(pick)
+    Eo, Fo, _ = pyoracle.Oracle(model).compute_batch(sub, nthreads=os.cpu_count() or 1)
+    off = batch.atom_off
+    Fg = np.concatenate([F[off[k]:off[k + 1]] for k in pick])
+    return {"structures": [int(k) for k in pick], "atoms": [int(batch.natoms[k]) for k in pick],
+            "energy_per_atom": float(np.max(np.abs(E[pick] - Eo) / sub.natoms)),
+            "force_rel_rms": float(np.sqrt(((Fg - Fo) ** 2).sum() / (Fo ** 2).sum()))}
+
+
+def run_precond(steps, warmup):
+    """The water E+F step with the reference trainer's MEAN / DEV input
+    preconditioning (the scaling a trained reference potential carries): its
+    cost, the number of inputs evaluated in FP64, and the errors against the
+    FP64 oracle under the same scaling on 8 of the 64 structures, with and
+    without the FP64 inputs."""
+    import torch
+    from ann_amd import configs
+    from ann_amd.gpu import Engine, FAST
+    from ann_amd.precond import apply_preconditioning
+    model = configs.model_water()
+    batch = configs.batch_water(64, seed=12)
+    eng = Engine(model)
+    eng.set_batch(batch)
+    eng.neighbors()
+    pre = apply_preconditioning(eng, "MEAN", "DEV")
+    eng.autotune(FAST)
+    m2 = model.copy()
+    for a, (w, b) in enumerate(pre):
+        m2.nets[a].inpw, m2.nets[a].inpb, m2.nets[a].outw, m2.nets[a].outb = w, b, 1.0, 0.0
+    dev = eng.device
+    out = (torch.empty(batch.nstruc, dtype=torch.float64, device=dev),
+           torch.empty((batch.ntotal, 3), dtype=torch.float64, device=dev),
+           torch.empty((batch.nstruc, 3, 3), dtype=torch.float64, device=dev))
+    dt = time_ef(eng, FAST, steps, warmup, out)
+    res = {"fp64_inputs": eng.precise_count(), "inputs": 2 * model.max_ninput,
+           "max_abs_inpw_inpb": float(max(np.abs(w * b).max() for w, b in pre)),
+           "atoms_per_s": batch.ntotal / dt, "ms_per_step": dt * 1e3}
+    eng.profile(True)
+    time_ef(eng, FAST, 3, 1, out)
+    ms, n = eng.profile_get("precise")
+    eng.profile(False)
+    res["precise_kernel_us"] = ms / n * 1e3 if n else None
+    pick = np.arange(8)
+    E, F, _ = eng.energy_forces(FAST, out=out)
+    res["errors_vs_fp64_oracle"] = oracle_errors(m2, batch, pick, E.cpu().numpy(), F.cpu().numpy())
+    eng.set_precise_ratio(1e300)
+    E, F, _ = eng.energy_forces(FAST, out=out)
+    res["errors_fp32_inputs_only"] = oracle_errors(m2, batch, pick, E.cpu().numpy(), F.cpu().numpy())
+    return res
+
+
+def train_gradient_check(nstruc=2):
+    """The training gradient of 2 of the 128 Li/P/S structures (GPU step
+    without the update) against the oracle's analytic gradient."""
+    import torch
+    from ann_amd.gpu import Engine, FAST
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    model, batch, Eref, Fref = train_setup()
+    sub = batch.subset(range(nstruc))
+    na = sub.ntotal
+    Er, Fr = Eref[:nstruc].clone(), Fref[:na].clone()
+    eng = Engine(model)
+    eng.set_batch(sub)
+    eng.neighbors()
+    loss, g = eng.train_step(Er, Fr, 1.0, 0.1, 0.0, FAST)
+    _, go, _, _, parts = pyoracle.Oracle(model).train_loss_grad(sub, Er.cpu().numpy(), Fr.cpu().numpy(), 1.0, 0.1)
+    return {"structures": nstruc, "atoms": na,
+            "gradient_rel_err": float(np.linalg.norm(g.cpu().numpy() - go) / np.linalg.norm(go)),
+            "loss_rel_err": float(np.max(np.abs(loss.cpu().numpy() - parts) / np.abs(parts)))}
+
+
+def train_cpu_baseline():
+    """The FP64 oracle's training step (E + F, loss, force adjoint, second
+    backward; the port of the reference algorithm) on all 128 Li/P/S
+    structures, structures split over every host core (one call per thread;
+    the gradient sums are the step's, the Adam update is negligible)."""
+    import concurrent.futures as cf
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    model, batch, Eref, Fref = train_setup()
+    Er, Fr = Eref.cpu().numpy(), Fref.cpu().numpy()
+    cores = os.cpu_count() or 1
+    parts = [list(range(k, batch.nstruc, cores)) for k in range(cores)]
+    parts = [p for p in parts if p]
+    orc = pyoracle.Oracle(model)
+    off = batch.atom_off
+
+    def one(idx):
+        sub = batch.subset(idx)
+        Fs = np.concatenate([Fr[off[k]:off[k + 1]] for k in idx])
+        return orc.train_loss_grad(sub, Er[idx], Fs, 1.0, 0.1)[1] * len(idx) / batch.nstruc
+
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(max_workers=len(parts)) as ex:
+        sum(ex.map(one, parts))
+    dt = time.perf_counter() - t0
+    return {"value": batch.nstruc / dt, "unit": "structs/s", "cores": len(parts), "kind": "port",
+            "sample": f"all 128 Li/P/S structures ({batch.ntotal} atoms), one training step (E + F, loss, "
+                      f"force adjoint, second backward) in FP64, {len(parts)} threads, {dt:.2f} s wall"}
+
+
+def run_rotating(steps, warmup, nset=512, size=128):
+    """Training with a new batch every step: the reference's sampler draws 128
+    of 512 Li/P/S structures resident in HBM, the batch is gathered on the
+    GPU, and the step includes ann_set_batch, the neighbour build, G and the
+    training step (FAST, Adam)."""
+    import torch
+    from ann_amd.batch import BatchSampler, ResidentSet
+    from ann_amd.gpu import Engine, FAST
+    model, full, Eref, Fref = train_setup(nset, seed=35)
+    rs = ResidentSet(full, Eref.cpu().numpy(), Fref.cpu().numpy())
+    sampler = BatchSampler(size, nset, seed=7)
+    eng = Engine(model)
+    out = (torch.empty(2, dtype=torch.float64, device=eng.device),
+           torch.empty(eng.nparams, dtype=torch.float64, device=eng.device))
+
+    def step():
+        b = rs.gather(sampler.next())
+        eng.set_batch_tensors(b["atom_off"], b["pos"], b["cells"], b["species"])
+        eng.neighbors()
+        eng.train_step(b["energy"], b["forces"], 1.0, 0.1, 1e-3, FAST, out=out)
+        return b
+
+    for _ in range(max(warmup, 2)):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    nstr = 0
+    for _ in range(steps):
+        nstr += len(step()["atom_off"]) - 1
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return {"structs_per_s": nstr / dt, "ms_per_step": dt / steps * 1e3, "steps": steps, "batch": size,
+            "training_set": nset, "loss_last": out[0].cpu().numpy().tolist()}
 
 
 def cpu_baseline(model, batch, gpu_forces):
@@ -376,12 +573,28 @@ def main():
             roof["traffic"] = tj[dom]["bytes_per_launch"]
     train = None
     if not args.no_train:
-        train = run_train(max(3, args.steps // 2), max(1, args.warmup // 2), FAST)
+        train = run_train(max(3, args.steps // 2), max(1, args.warmup // 2), FAST, kernels=True)
+        tw = train.pop("work")
+        tk = train["dominant"]["kernel"]
+        tr = kernel_rooflines({tk: {"avg_us": train["dominant"]["avg_launch_us"]}}, tw).get(tk)
+        if tr:
+            train["roofline"] = {"bound": tr["bound"], "kernel": tk, "achieved": tr["achieved"], "peak": tr["peak"],
+                                 "unit": tr["unit"], "frac": tr["frac"], "algorithmic_work_per_launch": tr["work"],
+                                 "avg_launch_us": train["dominant"]["avg_launch_us"], "traffic": None}
+            if os.path.exists(traffic_file):
+                tj = json.load(open(traffic_file))
+                if tk in tj and "train" in tj[tk].get("workload", ""):
+                    train["roofline"]["traffic"] = tj[tk]["bytes_per_launch"]
+        train["kernel_rooflines"] = kernel_rooflines(train["kernels"], tw)
         train["graph_structs_per_s"] = run_train(max(3, args.steps // 2), 1, FAST, graph=True)["structs_per_s"]
         train_plain = run_train(max(3, args.steps // 4), 1, PLAIN)
         train["plain_structs_per_s"] = train_plain["structs_per_s"]
         train["triplet_structs_per_s"] = run_train(max(3, args.steps // 2), 1, TRIPLET)["structs_per_s"]
         train["fast_over_plain"] = train["structs_per_s"] / train["plain_structs_per_s"]
+        train["gradient_check"] = train_gradient_check()
+        train["rotating_batch"] = run_rotating(max(3, args.steps // 2), max(1, args.warmup // 2))
+        if not args.no_cpu:
+            train["cpu_baseline"] = train_cpu_baseline()
     def note(msg):
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -391,6 +604,8 @@ def main():
     uneven = None if args.no_uneven else run_uneven(max(3, args.steps // 2), args.warmup)
     if uneven:
         note(f"uneven batch fast {uneven['fast']['atoms_per_s']:.4g} atoms/s")
+    precond = run_precond(max(3, args.steps // 2), max(1, args.warmup // 2))
+    note(f"preconditioned E+F {precond['atoms_per_s']:.4g} atoms/s, {precond['fp64_inputs']} FP64 inputs")
     md = None if args.no_md else run_md(args.md_steps, 10, 2)
     if md:
         note(f"md fast {md['fast']['atom_steps_per_s']:.4g} atom-steps/s")
@@ -424,6 +639,7 @@ def main():
         "md": md,
         "cpu_baseline": cpu,
         "errors_vs_fp64_oracle": errs,
+        "preconditioned": precond,
     }
     def _np(o):
         if isinstance(o, np.integer):

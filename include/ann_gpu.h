@@ -211,12 +211,29 @@ int ann_symm_tangent(ann_ctx* ctx, int form, const float* d_u, float* d_v);
  * replaces Cost::grad + D2ODZDI::grad (src/ml/nn.cpp:1712-1776, 2012-2103) */
 int ann_net_bwd2(ann_ctx* ctx, const float* d_G, const float* d_v, const double* d_gE,
                  double* d_grad);
-/* one training step on the current batch: loss
+/* one training step on the current batch: loss (defaults of ann_set_loss and
+ * ann_set_struc_weights)
  * L = (1/B) sum_s [ wE/2 ((E_s-Eref_s)/N_s)^2 + wF/(3N_s) sum |F-Fref|^2 ],
  * gradient, then the Adam update of the FP64 master parameters (ADAM::step,
  * src/opt/algo.cpp:280-296) when gamma > 0.  d_loss [2] = (energy, force) */
 int ann_train_step(ann_ctx* ctx, int form, const double* d_Eref, const double* d_Fref, double wE,
                    double wF, double gamma, double* d_loss, double* d_grad);
+/* Energy loss of ann_train_step (NNPTEFR::error_cost_o1,
+ * src/nnp/nnptefr2.cpp:1336-1372): on dU = (E - Eref) norm(N_s),
+ * MSE dU^2/2 (default), MAE |dU|, HUBER delta^2 (sqrt(1 + a^2) - 1) or ASINH
+ * delta^2 (1 - sqrt(1 + a^2) + a asinh a), a = dU / delta (opt::Loss numbering,
+ * src/opt/loss.hpp:18-24); norm(N) = 1 (IDENTITY), 1/N (LINEAR, default),
+ * 1/sqrt N, 1/cbrt N, 1/(ln N + 1) (Norm numbering, src/nnp/nnptefr2.hpp:
+ * 117-126, :602-612).  The energy term of the loss becomes
+ * (1/B) sum_s wE wE_s loss(dU_s); the force term is unchanged. */
+enum { ANN_LOSS_MSE = 1, ANN_LOSS_MAE = 2, ANN_LOSS_HUBER = 3, ANN_LOSS_ASINH = 4 };
+enum { ANN_NORM_IDENTITY = 0, ANN_NORM_LINEAR = 1, ANN_NORM_SQRT = 2, ANN_NORM_CBRT = 3, ANN_NORM_LOG = 4 };
+int ann_set_loss(ann_ctx* ctx, int loss, double delta, int norm);
+/* Per-structure weights of the energy and force terms of ann_train_step
+ * (device arrays [nstruc] that must stay valid while used; NULL = all 1): the
+ * reference trains Taylor-augmented copies on energies only
+ * (src/nnp/nnptefr2.cpp:2466-2473), i.e. with force weight 0. */
+int ann_set_struc_weights(ann_ctx* ctx, const double* d_wE, const double* d_wF);
 /* enable (1) or disable (0) HIP-graph capture of ann_train_step: after one
  * eager call the step is captured on a private stream and later calls with the
  * same form, buffers and weights replay it (ordered after prior work on the

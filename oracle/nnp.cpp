@@ -22,7 +22,11 @@
  *                                mode 0 folds every image reaction back, the
  *                                exact adjoint)
  *   thread::Dist block split     src/thread/dist.cpp:29-41 (orc_compute_batch)
- *   energy loss                  src/nnp/nnptefr2.cpp:1321-1373 (MSE, Norm::LINEAR)
+ *   energy loss                  src/nnp/nnptefr2.cpp:1321-1373: MSE, MAE, HUBER,
+ *                                ASINH on dU = (E - Eref) norm, Norm IDENTITY /
+ *                                LINEAR / SQRT / CBRT / LOG (:602-612), per-
+ *                                structure weights (energy-only structures of
+ *                                the Taylor augmentation, :2466-2473)
  *
  * Parity pins (tests/test_oracle_nnp.py): neighbour sets brute force = cell
  * list, reverse-slot involution; G supercell invariance on the NaCl fixtures
@@ -30,7 +34,9 @@
  * two- and three-atom clusters; forces = -dE/dr by central differences
  * (src/nnp/test_nnp_force.cpp pattern, eps 1e-6); virial = -dE/dstrain by
  * central differences; sum_i F_i = 0; tangent = directional derivative of G
- * by central differences; training gradient by central differences (h=1e-4).
+ * by central differences; training gradient by central differences (h=1e-4),
+ * for every loss variant, normalisation and per-structure weights; HUBER and
+ * ASINH tend to MSE for large delta.
  */
 #include "oracle.h"
 #include "oracle_impl.h"
@@ -493,10 +499,48 @@ extern "C" void orc_symm_tangent(const orc_model* m, const orc_struc* s, void* n
 //   L = (1/B) sum_s [ wE/2 ((E_s - Eref_s)/N_s)^2 + wF/(3 N_s) sum_a |F_a - Fref_a|^2 ]
 // E_s is the sum of network outputs (vacuum energies excluded, as in the
 // reference trainer which subtracts them from the targets, nnptefr2.cpp:2781-2794).
+// Energy loss of one structure on dU = (E - Eref) * norm and its derivative
+// d loss / d dU (NNPTEFR::error_cost_o1, src/nnp/nnptefr2.cpp:1336-1372;
+// opt::Loss numbering src/opt/loss.hpp:18-24, Norm src/nnp/nnptefr2.hpp:117-126)
+static void energy_loss(int loss, double delta, double dU, double& l, double& g) {
+    switch (loss) {
+        case ORC_LOSS_MAE: l = std::fabs(dU); g = (dU > 0) - (dU < 0); break;
+        case ORC_LOSS_HUBER: {
+            const double a = dU / delta, q = std::sqrt(1.0 + a * a);
+            l = delta * delta * (q - 1.0); g = dU / q;
+        } break;
+        case ORC_LOSS_ASINH: {
+            const double a = dU / delta, q = std::sqrt(1.0 + a * a), as = std::asinh(a);
+            l = delta * delta * (1.0 - q + a * as); g = delta * as;
+        } break;
+        default: l = 0.5 * dU * dU; g = dU; break;
+    }
+}
+
+static double energy_norm(int norm, double n) {
+    switch (norm) {
+        case ORC_NORM_IDENTITY: return 1.0;
+        case ORC_NORM_SQRT: return 1.0 / std::sqrt(n);
+        case ORC_NORM_CBRT: return 1.0 / std::cbrt(n);
+        case ORC_NORM_LOG: return 1.0 / (std::log(n) + 1.0);
+        default: return 1.0 / n;   // LINEAR
+    }
+}
+
 extern "C" double orc_train_loss_grad(const orc_model* m, int B, const int* off, const double* pos,
                                       const double* cells, const int* species, const double* Eref,
                                       const double* Fref, double wE, double wF, double* grad,
                                       double* Eout, double* Fout, double* lossparts) {
+    return orc_train_loss_grad_ex(m, B, off, pos, cells, species, Eref, Fref, wE, wF, ORC_LOSS_MSE, 1.0,
+                                  ORC_NORM_LINEAR, nullptr, nullptr, grad, Eout, Fout, lossparts);
+}
+
+// L = (1/B) sum_s [ wE wE_s loss((E_s - Eref_s) norm_s) + wF wF_s / (3 N_s) sum |F - Fref|^2 ]
+extern "C" double orc_train_loss_grad_ex(const orc_model* m, int B, const int* off, const double* pos,
+                                         const double* cells, const int* species, const double* Eref,
+                                         const double* Fref, double wE, double wF, int loss, double delta,
+                                         int norm, const double* wEs, const double* wFs, double* grad,
+                                         double* Eout, double* Fout, double* lossparts) {
     const int S = m->nspecies;
     std::vector<int> poff(S + 1, 0);
     for (int a = 0; a < S; ++a) poff[a + 1] = poff[a] + orc_nparams(m, a);
@@ -514,16 +558,19 @@ extern "C" double orc_train_loss_grad(const orc_model* m, int B, const int* off,
         double E = 0;
         for (int i = 0; i < N; ++i) E += orc_nn_forward(m, s.species[i], &G[(size_t)i * ld], &dE[(size_t)i * ld]);
         force(m, &s, L, dE.data(), ld, 0, F.data(), nullptr, nullptr);
-        const double dEn = (E - Eref[k]) / N;
-        lossE += 0.5 * wE * dEn * dEn / B;
-        const double gE = wE * dEn / N / B;
+        const double we = wE * (wEs ? wEs[k] : 1.0), wf = wF * (wFs ? wFs[k] : 1.0);
+        const double nrm = energy_norm(norm, (double)N);
+        double l, gl;
+        energy_loss(loss, delta, (E - Eref[k]) * nrm, l, gl);
+        lossE += we * l / B;
+        const double gE = we * gl * nrm / B;
         double sf = 0;
         for (int x = 0; x < 3 * N; ++x) {
             const double df = F[x] - Fref[3 * (size_t)off[k] + x];
             sf += df * df;
-            u[x] = wF * 2.0 * df / (3.0 * N) / B;
+            u[x] = wf * 2.0 * df / (3.0 * N) / B;
         }
-        lossF += wF * sf / (3.0 * N) / B;
+        lossF += wf * sf / (3.0 * N) / B;
         tangent(m, &s, L, u.data(), v.data(), ld);
         std::vector<double> g;
         for (int i = 0; i < N; ++i) {

@@ -128,6 +128,18 @@ double orc_train_loss_grad(const orc_model* m, int nstruc, const int* atom_off, 
                            const double* cells, const int* species, const double* Eref,
                            const double* Fref, double wE, double wF, double* grad,
                            double* E, double* F, double* lossparts);
+/* the same with the reference trainer's energy loss variants and per-structure
+ * normalisation (opt::Loss src/opt/loss.hpp:18-24: MSE 1, MAE 2, HUBER 3,
+ * ASINH 4 with width delta; Norm src/nnp/nnptefr2.hpp:117-126: IDENTITY 0,
+ * LINEAR 1, SQRT 2, CBRT 3, LOG 4), and per-structure weights wEs / wFs
+ * (either may be NULL: all 1) */
+enum { ORC_LOSS_MSE = 1, ORC_LOSS_MAE = 2, ORC_LOSS_HUBER = 3, ORC_LOSS_ASINH = 4 };
+enum { ORC_NORM_IDENTITY = 0, ORC_NORM_LINEAR = 1, ORC_NORM_SQRT = 2, ORC_NORM_CBRT = 3, ORC_NORM_LOG = 4 };
+double orc_train_loss_grad_ex(const orc_model* m, int nstruc, const int* atom_off, const double* pos,
+                              const double* cells, const int* species, const double* Eref,
+                              const double* Fref, double wE, double wF, int loss, double delta, int norm,
+                              const double* wEs, const double* wFs, double* grad, double* E, double* F,
+                              double* lossparts);
 void   orc_adam_step(int n, double* p, double* mom1, double* mom2, const double* g, double gamma,
                      double eps, int t);
 

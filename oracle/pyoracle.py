@@ -49,6 +49,7 @@ def lib():
         L.orc_compute.restype = C.c_double
         L.orc_nn_forward.restype = C.c_double
         L.orc_train_loss_grad.restype = C.c_double
+        L.orc_train_loss_grad_ex.restype = C.c_double
         _LIB = L
     return _LIB
 
@@ -178,7 +179,7 @@ class Oracle:
                                   _p(g, P_D))
         return g
 
-    def train_loss_grad(self, batch, Eref, Fref, wE=1.0, wF=0.1):
+    def train_loss_grad(self, batch, Eref, Fref, wE=1.0, wF=0.1, loss=1, delta=1.0, norm=1, wEs=None, wFs=None):
         off = np.ascontiguousarray(batch.atom_off, dtype=np.int32)
         pos = np.ascontiguousarray(batch.pos, dtype=np.float64)
         cells = np.ascontiguousarray(batch.cells, dtype=np.float64)
@@ -190,10 +191,14 @@ class Oracle:
         E = np.zeros(batch.nstruc)
         F = np.zeros((batch.ntotal, 3))
         parts = np.zeros(2)
-        L = lib().orc_train_loss_grad(C.byref(self.c), batch.nstruc, _p(off, P_I), _p(pos, P_D),
-                                      _p(cells, P_D), _p(sp, P_I), _p(Eref, P_D), _p(Fref, P_D),
-                                      C.c_double(wE), C.c_double(wF), _p(g, P_D), _p(E, P_D),
-                                      _p(F, P_D), _p(parts, P_D))
+        wEs = None if wEs is None else np.ascontiguousarray(wEs, dtype=np.float64)
+        wFs = None if wFs is None else np.ascontiguousarray(wFs, dtype=np.float64)
+        L = lib().orc_train_loss_grad_ex(C.byref(self.c), batch.nstruc, _p(off, P_I), _p(pos, P_D),
+                                         _p(cells, P_D), _p(sp, P_I), _p(Eref, P_D), _p(Fref, P_D),
+                                         C.c_double(wE), C.c_double(wF), int(loss), C.c_double(delta), int(norm),
+                                         None if wEs is None else _p(wEs, P_D),
+                                         None if wFs is None else _p(wFs, P_D), _p(g, P_D), _p(E, P_D),
+                                         _p(F, P_D), _p(parts, P_D))
         return L, g, E, F, parts
 
 

@@ -271,3 +271,52 @@ def test_training_gradient_central_differences(oracle_mod):
         m.set_param_vector(p0)
         fd = (vals[0] - vals[1]) / (2 * h)
         assert abs(fd - g[k]) < 1e-6 * max(1.0, abs(g[k])), (k, fd, g[k])
+
+
+@pytest.mark.parametrize("loss,norm", [(2, 1), (3, 1), (4, 1), (1, 0), (1, 2), (1, 3), (1, 4), (3, 2)])
+def test_training_loss_variants_central_differences(oracle_mod, loss, norm):
+    """The energy loss variants of the reference trainer (MAE 2, HUBER 3,
+    ASINH 4; normalisations IDENTITY 0 .. LOG 4) with per-structure weights
+    (an energy-only structure: force weight 0): dL/dp by central differences."""
+    m = model3(seed=9, neuron="TANH", hidden=(4,))
+    b = configs.Batch.concat([configs.small_random(23, natoms=8, a0=9.0), configs.small_random(24, natoms=6, a0=8.0),
+                              configs.small_random(25, natoms=7, a0=8.5)])
+    rng = np.random.default_rng(5)
+    Eref = rng.normal(size=3) * 2.0
+    Fref = rng.normal(size=(b.ntotal, 3)) * 0.1
+    wEs, wFs = np.array([1.0, 0.5, 2.0]), np.array([1.0, 0.0, 1.5])
+    kw = dict(loss=loss, delta=0.7, norm=norm, wEs=wEs, wFs=wFs)
+    L, g, _, _, parts = oracle_mod.Oracle(m).train_loss_grad(b, Eref, Fref, 1.0, 0.1, **kw)
+    assert np.isclose(L, parts.sum())
+    p0 = m.param_vector()
+    h = 1e-4
+    for k in rng.choice(len(p0), 12, replace=False):
+        vals = []
+        for s in (1, -1):
+            p = p0.copy()
+            p[k] += s * h
+            m.set_param_vector(p)
+            vals.append(oracle_mod.Oracle(m).train_loss_grad(b, Eref, Fref, 1.0, 0.1, **kw)[0])
+        m.set_param_vector(p0)
+        fd = (vals[0] - vals[1]) / (2 * h)
+        assert abs(fd - g[k]) < 1e-6 * max(1.0, abs(g[k])), (k, fd, g[k])
+
+
+def test_training_loss_limits(oracle_mod):
+    """HUBER and ASINH tend to MSE for a large width; a structure with weights
+    0 contributes nothing; MAE of one structure is |dE| norm."""
+    m = model3(seed=9, neuron="TANH", hidden=(4,))
+    b = configs.Batch.concat([configs.small_random(23, natoms=8, a0=9.0), configs.small_random(24, natoms=6, a0=8.0)])
+    rng = np.random.default_rng(6)
+    Eref = rng.normal(size=2)
+    Fref = rng.normal(size=(b.ntotal, 3)) * 0.1
+    orc = oracle_mod.Oracle(m)
+    Lm, gm, E, _, pm = orc.train_loss_grad(b, Eref, Fref, 1.0, 0.1)
+    for loss in (3, 4):
+        L, g, _, _, p = orc.train_loss_grad(b, Eref, Fref, 1.0, 0.1, loss=loss, delta=1e4)
+        assert np.isclose(p[0], pm[0], rtol=1e-6) and np.allclose(g, gm, rtol=1e-6, atol=1e-12)
+    L0, g0, _, _, _ = orc.train_loss_grad(b, Eref, Fref, 1.0, 0.1, wEs=[1.0, 0.0], wFs=[1.0, 0.0])
+    L1, g1, _, _, _ = orc.train_loss_grad(b.subset([0]), Eref[:1], Fref[:8], 1.0, 0.1)
+    assert np.isclose(L0, L1 / 2) and np.allclose(g0, g1 / 2, rtol=1e-12, atol=1e-15)
+    _, _, _, _, pa = orc.train_loss_grad(b, Eref, Fref, 1.0, 0.1, loss=2, norm=0)
+    assert np.isclose(pa[0], np.abs(E - Eref).sum() / 2)
