@@ -130,9 +130,20 @@ static int update_precise(ann_ctx* ctx) {
             items.push_back(it);
         }
     }
+    // when more than half of the inputs need FP64, all of them are evaluated in
+    // FP64: the FP64 kernel's cost is per neighbour rather than per input, and
+    // the FP32 basis pass is then skipped altogether
+    if (ctx->pre_ratio >= 0 && 2 * items.size() > tot) {
+        const double keep = ctx->pre_ratio;
+        ctx->pre_ratio = -1.0;
+        const int rc = update_precise(ctx);
+        ctx->pre_ratio = keep;
+        return rc;
+    }
     M.pre_off[M.S] = (int)items.size();
     for (int a = M.S + 1; a <= kMaxSpecies; ++a) M.pre_off[a] = (int)items.size();
     ctx->npre = (int)items.size();
+    ctx->nin_total = (int)tot;
     hipStream_t st = ctx->stream;
     if (tot > 0) {
         ANN_HIP(hipMemcpyAsync(ctx->da.inpw, iw.data(), sizeof(float) * tot, hipMemcpyHostToDevice, st));
@@ -830,8 +841,7 @@ extern "C" int ann_force_reduce_noalloc(ann_ctx* ctx, const float* P, const floa
 
 static hipError_t ef_body(ann_ctx* ctx, int form, int with_evac, double* E, double* F, double* V) {
     hipError_t e;
-    if ((e = symm_fwd(ctx, form, ctx->d_G)) != hipSuccess) return e;
-    if ((e = precise_inputs(ctx, ctx->d_G, nullptr)) != hipSuccess) return e;
+    if ((e = network_rows(ctx, form, ctx->d_G)) != hipSuccess) return e;
     if ((e = net_fwd(ctx, form, ctx->d_G, ctx->d_Eatom, F || V ? ctx->d_dEdG : nullptr, true)) != hipSuccess) return e;
     if (F || V) {
         if ((e = symm_bwd(ctx, form, ctx->d_dEdG, ctx->d_P)) != hipSuccess) return e;
@@ -940,8 +950,7 @@ extern "C" int ann_md_step(ann_ctx* ctx, int form, double dt, const double* mass
     RET_HIP(refresh_geometry(ctx), "refresh_geometry");
     ctx->G_valid = false;
     ctx->gen++;
-    RET_HIP(symm_fwd(ctx, form, ctx->d_G), "symm_fwd");
-    RET_HIP(precise_inputs(ctx, ctx->d_G, nullptr), "precise_inputs");
+    RET_HIP(network_rows(ctx, form, ctx->d_G), "symm_fwd");
     RET_HIP(net_fwd(ctx, form, ctx->d_G, ctx->d_Eatom, ctx->d_dEdG, true), "net_fwd");
     RET_HIP(symm_bwd(ctx, form, ctx->d_dEdG, ctx->d_P), "symm_bwd");
     RET_HIP(force_reduce(ctx, ctx->d_P, ctx->d_Eatom, 1, F, E, V), "force_reduce");

@@ -162,6 +162,7 @@ struct ann_ctx {
     std::vector<double> h_inpw, h_inpb;   // input scaling (FP64, inoff-indexed)
     double pre_ratio = 1.0;           // inputs with |inpw inpb| above this are evaluated in FP64 (< 0: all)
     int npre = 0;                     // number of FP64-evaluated inputs over all species
+    int nin_total = 0;                // inputs over all species
     uint64_t scaling_gen = 0;         // bumped when the input scaling or its FP64 set changes
     uint64_t batch_gen = 0;           // bumped by every ann_set_batch
     int nb_status = 0;                // ANN_* status of the last neighbour build (capacity, image limit, symmetry)
@@ -318,7 +319,12 @@ hipError_t symm_fwd(ann_ctx* ctx, int form, float* G);
 // inputs flagged for FP64 evaluation: with Gc set, Gc[i][k] = G_k + inpb_k (FP64
 // sum rounded to FP32) for every flagged k (other entries untouched); with G64
 // set, G64[i][k] = G_k in FP64 for every flagged k
-hipError_t precise_inputs(ann_ctx* ctx, float* Gc, double* G64);
+hipError_t precise_inputs(ann_ctx* ctx, float* Gc, double* G64, float* Tout = nullptr);
+// G rows of the context's batch for the network, centred where FP64-evaluated:
+// the FP32 basis kernel then the FP64 inputs, or, when every input is
+// FP64-evaluated, the FP64 kernel alone (it then also stores the moments the
+// force adjoint reads)
+hipError_t network_rows(ann_ctx* ctx, int form, float* G);
 // G64 = G (widened), then the flagged entries recomputed in FP64
 hipError_t widen_rows(ann_ctx* ctx, const float* G, double* G64);
 // Gc = G64 rounded to FP32, centred (G + inpb in FP64) where flagged

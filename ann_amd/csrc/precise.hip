@@ -41,6 +41,8 @@ struct PreciseArgs {
     const int* idx;
     const uint32_t* img;
     const uint8_t* nsp;
+    const float4* geo;        // FP32 pair geometry (prefilter of the cutoff test)
+    float* Tout;              // FP32 moments for the force adjoint (or null)
     float* Gc;                // centred FP32 output (or null)
     double* G64;              // FP64 output (or null)
 };
@@ -100,18 +102,12 @@ __device__ inline double angular64(int name, double ieta2, double zeta, double l
     return pow(fabs(0.5 + lamh * c), zeta) * fr * f0 * f1;
 }
 
-__device__ inline double ipow64(double x, int e) {
-    double r = 1.0;
-    for (int q = 0; q < e; ++q) r *= x;
-    return r;
-}
-
 // monomials x^a y^b z^c of total degree <= 4 in degree order (as the FP32
 // moment kernels number them) and their multinomial weights m!/(a! b! c!)
-struct Mono64 {
+struct MonoFixed {
     int a[35], b[35], c[35], m[35];
     double w[35];
-    constexpr Mono64() : a(), b(), c(), m(), w() {
+    constexpr MonoFixed() : a(), b(), c(), m(), w() {
         int q = 0;
         for (int mm = 0; mm <= 4; ++mm)
             for (int aa = mm; aa >= 0; --aa)
@@ -128,7 +124,6 @@ struct Mono64 {
                 }
     }
 };
-__constant__ Mono64 c_mono64 = Mono64();
 
 __device__ inline double wave_sum64(double v) {
 #pragma unroll
@@ -136,19 +131,52 @@ __device__ inline double wave_sum64(double v) {
     return v;
 }
 
-// LDS of one atom: neighbours sorted by species (stable), FP64
-struct PreSmem {
-    double4* v;      // [maxn] (unit vector, r)
-    double* fc;      // [maxn]
-    double4* ef;     // [maxn] exp(-r^2 / eta_u^2) fc for the union widths u (moment form)
-    int* seg;        // [kMaxSpecies + 1]
-    double* T;       // [S][4][NM] moments (moment form)
-    double* Dg;      // [S][4] diagonal sums
-    double* Am;      // [NP][4][Z + 1] degree-m pair sums
-};
+// Sum NV per-lane doubles over the wavefront with a transposing butterfly:
+// afterwards lane l holds the total of value index idx (lanes l with
+// l % (64 / NV) == 0 hold distinct indices).  NV a power of two <= 64.
+template <int NV>
+__device__ __forceinline__ double wave_sum_multi64(double (&v)[NV], int& idx) {
+    const int lane = threadIdx.x % kWave;
+    int base = 0;
+#pragma unroll
+    for (int h = NV / 2, bit = 32; h >= 1; h >>= 1, bit >>= 1) {
+        const bool upper = (lane & bit) != 0;
+#pragma unroll
+        for (int q = 0; q < h; ++q) {
+            const double send = upper ? v[q] : v[q + h];
+            const double keep = upper ? v[q + h] : v[q];
+            v[q] = keep + __shfl_xor(send, bit);
+        }
+        if (upper) base += h;
+    }
+    double s = v[0];
+    constexpr int steps = (NV >= 64) ? 0 : 6 - __builtin_ctz(NV);
+    int bit = 1;
+#pragma unroll
+    for (int q = 0; q < steps; ++q, bit <<= 1) s += __shfl_xor(s, bit);
+    idx = base;
+    return s;
+}
+
+// tanh(x) for x >= 0 from expm1, accurate to a few ulp
+__device__ inline double tanh64(double x) {
+    const double em = expm1(-2.0 * x);
+    return -em / (2.0 + em);
+}
+
+__device__ inline double cut64f(int name, double rc, double r) {
+    if (name == CUT_TANH) {
+        if (!(r < rc)) return 0.0;
+        const double t = tanh64(1.0 - r / rc);
+        return t * t * t;
+    }
+    return cut64(name, rc, r);
+}
 
 // One wavefront per atom.  Z: largest zeta of the moment form (0: the model
-// has no moment form; angular inputs are then summed over triplets).
+// has no moment form; angular inputs are then summed over triplets).  With
+// A.Tout set (moment form only) the moments of every atom are also written in
+// FP32, in the layout of the FP32 moment kernels, for the force adjoint.
 template <int Z>
 __global__ __launch_bounds__(64) void k_precise(PreciseArgs A, int maxn) {
     extern __shared__ double4 smem64[];
@@ -157,21 +185,31 @@ __global__ __launch_bounds__(64) void k_precise(PreciseArgs A, int maxn) {
     const DevModel& M = *A.dm;
     const int a = A.species[i];
     const int q0 = M.pre_off[a], q1 = M.pre_off[a + 1];
-    if (q0 == q1) return;
+    const bool wantT = Z > 0 && A.Tout != nullptr;
+    if (q0 == q1 && !wantT) return;
     const int S = M.S, NP = M.NP;
+    constexpr int ZP = Z + 1;
     constexpr int NM = Z > 0 ? (Z + 1) * (Z + 2) * (Z + 3) / 6 : 1;
-    PreSmem sm;
-    sm.v = smem64;
-    sm.ef = sm.v + maxn;
-    sm.fc = reinterpret_cast<double*>(sm.ef + (Z > 0 ? maxn : 0));
-    sm.T = sm.fc + maxn;
-    sm.Dg = sm.T + (Z > 0 ? S * 4 * NM : 0);
-    sm.Am = sm.Dg + (Z > 0 ? S * 4 : 0);
-    sm.seg = reinterpret_cast<int*>(sm.Am + (Z > 0 ? NP * 4 * (Z + 1) : 0));
+    constexpr int PWS = Z > 0 ? (3 * Z) | 1 : 1;          // powers k = 1..Z of the three components
+    constexpr int NMH = Z > 1 ? NM - 4 : 1;               // monomials of degree >= 2
+    constexpr int GH = Z > 1 ? kWave / NMH : 1;           // neighbour groups of the degree >= 2 sums
+    // LDS: (r, fc) and, moment form, widths and powers; triplet form, unit vectors
+    double2* rf = reinterpret_cast<double2*>(smem64);
+    double4* ef = reinterpret_cast<double4*>(rf + maxn);                   // [maxn] (Z > 0)
+    double4* uv = ef;                                                        // [maxn] unit vectors (Z == 0)
+    double* pw = reinterpret_cast<double*>(ef + maxn);                      // [maxn][PWS] (Z > 0)
+    double* T = pw + (Z > 0 ? (size_t)maxn * PWS : 0);                      // [S][4][NM]
+    double* Dg = T + (Z > 0 ? S * 4 * NM : 0);                              // [S][4]
+    double* Am = Dg + (Z > 0 ? S * 4 : 0);                                  // [NP][4][ZP]
+    double* part = Am + (Z > 0 ? NP * 4 * ZP : 0);                          // [GH][NMH][4]
+    int* seg = reinterpret_cast<int*>(part + (Z > 1 ? GH * NMH * 4 : 0));   // [kMaxSpecies + 1]
     const int o = A.off[i], n = A.off[i + 1] - o;
     const double* R = A.cells + 9 * A.sof[i];
     const uint64_t lt = (1ull << lane) - 1ull;
-    // neighbours within the cutoff, regrouped by species (stable): count, then place
+    const double rc = A.rc;
+    const float rc_lo = (float)(rc * (1.0 - 1e-5)), rc_hi = (float)(rc * (1.0 + 1e-5));
+    // neighbours within the cutoff (the FP32 distance decides except in a thin
+    // shell around rc, where the FP64 one does), regrouped by species (stable)
     int cnt[kMaxSpecies];
 #pragma unroll
     for (int b = 0; b < kMaxSpecies; ++b) cnt[b] = 0;
@@ -179,11 +217,16 @@ __global__ __launch_bounds__(64) void k_precise(PreciseArgs A, int maxn) {
         const int t = q + lane;
         int sp = -1;
         if (t < n) {
-            int n0, n1, n2;
-            unpack_img(A.img[o + t], n0, n1, n2);
-            double x, y, z, d2;
-            pair_vec(A.pos, i, A.idx[o + t], R, n0, n1, n2, x, y, z, d2);
-            if (d2 > 0.0 && sqrt(d2) <= A.rc) sp = A.nsp[o + t];
+            const float r32 = A.geo[o + t].w;
+            bool keep = r32 < rc_lo;
+            if (!keep && r32 <= rc_hi) {
+                int n0, n1, n2;
+                unpack_img(A.img[o + t], n0, n1, n2);
+                double x, y, z, d2;
+                pair_vec(A.pos, i, A.idx[o + t], R, n0, n1, n2, x, y, z, d2);
+                keep = sqrt(d2) <= rc;
+            }
+            if (keep) sp = A.nsp[o + t];
         }
 #pragma unroll
         for (int b = 0; b < kMaxSpecies; ++b)
@@ -198,18 +241,21 @@ __global__ __launch_bounds__(64) void k_precise(PreciseArgs A, int maxn) {
     }
 #pragma unroll
     for (int b = 0; b < kMaxSpecies; ++b)
-        if (b < S && lane == b) sm.seg[b] = base[b];
-    if (lane == S) sm.seg[S] = tot;
+        if (b < S && lane == b) seg[b] = base[b];
+    if (lane == S) seg[S] = tot;
     const int U = M.nunion[a];
     for (int q = 0; q < n; q += kWave) {
         const int t = q + lane;
         int sp = -1;
         double x = 0, y = 0, z = 0, d2 = 0;
         if (t < n) {
-            int n0, n1, n2;
-            unpack_img(A.img[o + t], n0, n1, n2);
-            pair_vec(A.pos, i, A.idx[o + t], R, n0, n1, n2, x, y, z, d2);
-            if (d2 > 0.0 && sqrt(d2) <= A.rc) sp = A.nsp[o + t];
+            const float r32 = A.geo[o + t].w;
+            if (r32 <= rc_hi) {
+                int n0, n1, n2;
+                unpack_img(A.img[o + t], n0, n1, n2);
+                pair_vec(A.pos, i, A.idx[o + t], R, n0, n1, n2, x, y, z, d2);
+                if (r32 < rc_lo || sqrt(d2) <= rc) sp = A.nsp[o + t];
+            }
         }
         int p = 0;
 #pragma unroll
@@ -222,14 +268,27 @@ __global__ __launch_bounds__(64) void k_precise(PreciseArgs A, int maxn) {
         }
         if (sp >= 0) {
             const double r = sqrt(d2), ir = 1.0 / r;
-            const double f = cut64(M.cutoff, A.rc, r);
-            sm.v[p] = make_double4(x * ir, y * ir, z * ir, r);
-            sm.fc[p] = f;
+            const double f = cut64f(M.cutoff, rc, r);
+            rf[p] = make_double2(r, f);
+            const double h[3] = {x * ir, y * ir, z * ir};
             if (Z > 0) {
                 double e[4];
+                const double r2 = r * r;
 #pragma unroll
-                for (int u = 0; u < 4; ++u) e[u] = u < U ? exp(-r * r * M.uieta2_64[a][u]) * f : 0.0;
-                sm.ef[p] = make_double4(e[0], e[1], e[2], e[3]);
+                for (int u = 0; u < 4; ++u) e[u] = u < U ? exp(-r2 * M.uieta2_64[a][u]) * f : 0.0;
+                ef[p] = make_double4(e[0], e[1], e[2], e[3]);
+                double* row = pw + (size_t)p * PWS;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    double qv = h[c];
+#pragma unroll
+                    for (int k = 0; k < Z; ++k) {
+                        row[c * Z + k] = qv;
+                        qv *= h[c];
+                    }
+                }
+            } else {
+                uv[p] = make_double4(h[0], h[1], h[2], 0.0);
             }
         }
     }
@@ -260,93 +319,144 @@ __global__ __launch_bounds__(64) void k_precise(PreciseArgs A, int maxn) {
         }
         double acc = 0.0;
         if (f < nf)
-            for (int J = sm.seg[blk] + g; J < sm.seg[blk + 1]; J += G) acc += radial64(rb.name, eta, rs, sm.v[J].w, sm.fc[J]);
+            for (int J = seg[blk] + g; J < seg[blk + 1]; J += G) {
+                const double2 x = rf[J];
+                acc += radial64(rb.name, eta, rs, x.x, x.y);
+            }
         for (int bit = W; bit < kWave; bit <<= 1) acc += __shfl_xor(acc, bit);
         if (g == 0 && f < nf) put(A.items[q + f].k, acc);
         q = qe;
     }
-    if (q == q1) return;
-    if (Z > 0) {
-        // moments T_b[u][alpha] = sum_{j in b} e_u(r_j) r^_j^alpha: lanes over alpha
-        if (lane < NM) {
-            const int ea = c_mono64.a[lane], eb = c_mono64.b[lane], ec = c_mono64.c[lane];
-            for (int b = 0; b < S; ++b) {
-                double t4[4] = {0.0, 0.0, 0.0, 0.0};
-                for (int J = sm.seg[b]; J < sm.seg[b + 1]; ++J) {
-                    const double4 v = sm.v[J], e = sm.ef[J];
-                    const double mono = ipow64(v.x, ea) * ipow64(v.y, eb) * ipow64(v.z, ec);
-                    t4[0] += e.x * mono; t4[1] += e.y * mono; t4[2] += e.z * mono; t4[3] += e.w * mono;
-                }
+    if (q == q1 && !wantT) return;
+    if constexpr (Z > 0) {
+        constexpr MonoFixed mo{};
+        for (int b = 0; b < S; ++b) {
+            const int s0 = seg[b], s1 = seg[b + 1];
+            // degree <= 1 moments (1, x, y, z) and the diagonal sums: lanes over
+            // neighbours, one transposing reduction of 4 x 5 values (padded to 32)
+            {
+                double v[32];
 #pragma unroll
-                for (int u = 0; u < 4; ++u) sm.T[(b * 4 + u) * NM + lane] = t4[u];
+                for (int x = 0; x < 32; ++x) v[x] = 0.0;
+                for (int J = s0 + lane; J < s1; J += kWave) {
+                    const double4 e4 = ef[J];
+                    const double* row = pw + (size_t)J * PWS;
+                    const double hx = row[0], hy = row[Z], hz = row[2 * Z];
+                    const double ev[4] = {e4.x, e4.y, e4.z, e4.w};
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        v[5 * u] += ev[u];
+                        v[5 * u + 1] += ev[u] * hx;
+                        v[5 * u + 2] += ev[u] * hy;
+                        v[5 * u + 3] += ev[u] * hz;
+                        v[5 * u + 4] += ev[u] * ev[u];
+                    }
+                }
+                int idx;
+                const double sv = wave_sum_multi64<32>(v, idx);
+                if ((lane & 1) == 0 && idx < 20) {
+                    const int u = idx / 5, c = idx % 5;
+                    if (c < 4) T[(b * 4 + u) * NM + c] = sv;
+                    else Dg[b * 4 + u] = sv;
+                }
             }
-        }
-        if (lane < S * 4) {
-            const int b = lane / 4, u = lane % 4;
-            double s = 0.0;
-            for (int J = sm.seg[b]; J < sm.seg[b + 1]; ++J) {
-                const double4 e = sm.ef[J];
-                const double x = u == 0 ? e.x : (u == 1 ? e.y : (u == 2 ? e.z : e.w));
-                s += x * x;
+            // degree >= 2: lane = (monomial, neighbour group), powers from the rows
+            if (Z > 1) {
+                const int al = 4 + lane % NMH, g = lane / NMH;
+                if (g < GH) {
+                    const int ea = mo.a[al], eb = mo.b[al], ec = mo.c[al];
+                    double t4[4] = {0.0, 0.0, 0.0, 0.0};
+                    auto pv = [&](const double* row, int c, int e) { return e == 0 ? 1.0 : row[c * Z + e - 1]; };
+                    for (int J = s0 + g; J < s1; J += GH) {
+                        const double* row = pw + (size_t)J * PWS;
+                        const double mono = pv(row, 0, ea) * pv(row, 1, eb) * pv(row, 2, ec);
+                        const double4 e4 = ef[J];
+                        t4[0] += e4.x * mono; t4[1] += e4.y * mono; t4[2] += e4.z * mono; t4[3] += e4.w * mono;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) part[(g * NMH + al - 4) * 4 + u] = t4[u];
+                }
+                __syncthreads();
+                for (int it = lane; it < NMH * 4; it += kWave) {
+                    double sum = 0.0;
+                    for (int gg = 0; gg < GH; ++gg) sum += part[gg * NMH * 4 + it];
+                    T[(b * 4 + it % 4) * NM + 4 + it / 4] = sum;
+                }
+                __syncthreads();
             }
-            sm.Dg[lane] = s;
         }
         __syncthreads();
+        if (wantT) {
+            const int TS = S * 4 * NM;
+            for (int x = lane; x < TS; x += kWave) A.Tout[(size_t)i * TS + x] = (float)T[x];
+        }
         // A_m(p, u) = sum_{|alpha| = m} w_alpha T_b T_c (b != c); (... - D_b) / 2 for b == c
-        for (int it = lane; it < NP * 4 * (Z + 1); it += kWave) {
-            const int p = it / (4 * (Z + 1)), u = (it / (Z + 1)) % 4, m = it % (Z + 1);
+        for (int it = lane; it < NP * 4 * ZP; it += kWave) {
+            const int p = it / (4 * ZP), u = (it / ZP) % 4, m = it % ZP;
             int c = 0;
             while ((c + 1) * (c + 2) / 2 <= p) ++c;
             const int b = p - c * (c + 1) / 2;
             const int qa = m * (m + 1) * (m + 2) / 6, qb = (m + 1) * (m + 2) * (m + 3) / 6;
-            double s = 0.0;
-            for (int x = qa; x < qb; ++x) s += c_mono64.w[x] * sm.T[(b * 4 + u) * NM + x] * sm.T[(c * 4 + u) * NM + x];
-            if (b == c) s = 0.5 * (s - sm.Dg[b * 4 + u]);
-            sm.Am[it] = s;
+            double sum = 0.0;
+            for (int x = qa; x < qb; ++x) sum += mo.w[x] * T[(b * 4 + u) * NM + x] * T[(c * 4 + u) * NM + x];
+            if (b == c) sum = 0.5 * (sum - Dg[b * 4 + u]);
+            Am[it] = sum;
         }
         __syncthreads();
         // G = sum_m C(zeta, m) 2^-zeta lambda^m A_m: one lane per item
         for (int t = q + lane; t < q1; t += kWave) {
             const PreItem it = A.items[t];
-            if (it.mode != 1) continue;
             double g = 0.0, binom = 1.0, lp = 1.0;
             for (int m = 0; m <= it.zint; ++m) {
-                g += binom * lp * sm.Am[(it.blk * 4 + it.u) * (Z + 1) + m];
+                g += binom * lp * Am[(it.blk * 4 + it.u) * ZP + m];
                 binom = binom * (it.zint - m) / (m + 1);
                 lp *= (double)it.lam;
             }
             put(it.k, g * ldexp(1.0, -it.zint));
         }
-    }
-    // angular items outside the moment form: lanes over the first leg, the
-    // second leg looped (each unordered pair of the block's species once)
-    for (; q < q1; ++q) {
-        const PreItem it = A.items[q];
-        if (it.mode != 2) continue;
-        const AngBlock& ab = M.ang[a * NP + it.blk];
-        int c = 0;
-        while ((c + 1) * (c + 2) / 2 <= it.blk) ++c;
-        const int b = it.blk - c * (c + 1) / 2;
-        const double eta = A.ang_eta[ab.par + it.fn];
-        const double ieta2 = 1.0 / (eta * eta), zeta = A.ang_zeta[ab.par + it.fn];
-        const double lamh = 0.5 * (double)A.ang_lam[ab.par + it.fn];
-        double acc = 0.0;
-        const int kb0 = sm.seg[c], kb1 = sm.seg[c + 1];
-        for (int J = sm.seg[b] + lane; J < sm.seg[b + 1]; J += kWave) {
-            const double4 vj = sm.v[J];
-            for (int K = (b == c ? J + 1 : kb0); K < kb1; ++K) {
-                const double4 vk = sm.v[K];
-                const double cs = vj.x * vk.x + vj.y * vk.y + vj.z * vk.z;
-                acc += angular64(ab.name, ieta2, zeta, lamh, vj.w, vk.w, cs, sm.fc[J], sm.fc[K]);
+    } else {
+        // angular items outside the moment form: lanes over the first leg, the
+        // second leg looped (each unordered pair of the block's species once)
+        for (; q < q1; ++q) {
+            const PreItem it = A.items[q];
+            const AngBlock& ab = M.ang[a * NP + it.blk];
+            int c = 0;
+            while ((c + 1) * (c + 2) / 2 <= it.blk) ++c;
+            const int b = it.blk - c * (c + 1) / 2;
+            const double eta = A.ang_eta[ab.par + it.fn];
+            const double ieta2 = 1.0 / (eta * eta), zeta = A.ang_zeta[ab.par + it.fn];
+            const double lamh = 0.5 * (double)A.ang_lam[ab.par + it.fn];
+            double acc = 0.0;
+            const int kb0 = seg[c], kb1 = seg[c + 1];
+            for (int J = seg[b] + lane; J < seg[b + 1]; J += kWave) {
+                const double4 vj = uv[J];
+                const double2 xj = rf[J];
+                for (int K = (b == c ? J + 1 : kb0); K < kb1; ++K) {
+                    const double4 vk = uv[K];
+                    const double2 xk = rf[K];
+                    const double cs = vj.x * vk.x + vj.y * vk.y + vj.z * vk.z;
+                    acc += angular64(ab.name, ieta2, zeta, lamh, xj.x, xk.x, cs, xj.y, xk.y);
+                }
             }
+            acc = wave_sum64(acc);
+            if (lane == 0) put(it.k, acc);
         }
-        acc = wave_sum64(acc);
-        if (lane == 0) put(it.k, acc);
     }
 }
 
-hipError_t precise_inputs(ann_ctx* ctx, float* Gc, double* G64) {
-    if (ctx->npre == 0 || ctx->natoms == 0) return hipSuccess;
+static size_t precise_smem(int maxn, int Z, int S, int NP) {
+    const int NM = Z > 0 ? (Z + 1) * (Z + 2) * (Z + 3) / 6 : 1;
+    size_t b = (size_t)maxn * sizeof(double2) + (size_t)maxn * sizeof(double4);
+    if (Z > 0) {
+        const int PWS = (3 * Z) | 1, NMH = Z > 1 ? NM - 4 : 1, GH = Z > 1 ? kWave / NMH : 1;
+        b += sizeof(double) * ((size_t)maxn * PWS + (size_t)S * 4 * NM + S * 4 + (size_t)NP * 4 * (Z + 1) +
+                               (Z > 1 ? (size_t)GH * NMH * 4 : 0));
+    }
+    return b + sizeof(int) * (kMaxSpecies + 1);
+}
+
+hipError_t precise_inputs(ann_ctx* ctx, float* Gc, double* G64, float* Tout) {
+    if ((ctx->npre == 0 && !Tout) || ctx->natoms == 0) return hipSuccess;
     PreciseArgs A{};
     A.dm = ctx->dm;
     A.items = ctx->da.pitems;
@@ -367,13 +477,13 @@ hipError_t precise_inputs(ann_ctx* ctx, float* Gc, double* G64) {
     A.idx = ctx->nl.idx;
     A.img = ctx->nl.img;
     A.nsp = ctx->nl.nsp;
+    A.geo = ctx->nl.geo;
     A.Gc = Gc;
     A.G64 = G64;
+    const int Z = ctx->hm.moment_z;
+    A.Tout = Z > 0 ? Tout : nullptr;
     const int maxn = std::max(ctx->nl.maxn, 1);
-    const int Z = ctx->hm.moment_z, S = ctx->hm.S, NP = ctx->hm.NP;
-    const int NM = Z > 0 ? (Z + 1) * (Z + 2) * (Z + 3) / 6 : 0;
-    size_t smem = (size_t)maxn * (sizeof(double4) + sizeof(double)) + sizeof(int) * (kMaxSpecies + 1);
-    if (Z > 0) smem += (size_t)maxn * sizeof(double4) + sizeof(double) * ((size_t)S * 4 * NM + S * 4 + (size_t)NP * 4 * (Z + 1));
+    const size_t smem = precise_smem(maxn, Z, ctx->hm.S, ctx->hm.NP);
     Launch L(ctx, "precise");
     switch (Z) {
         case 0: k_precise<0><<<ctx->natoms, 64, smem, ctx->stream>>>(A, maxn); break;
@@ -401,6 +511,18 @@ __global__ void k_centre(int N, int ldg, const DevModel* __restrict__ dm, const 
     const NetDesc& D = dm->net[species[i]];
     const double g = G64[e];
     Gc[e] = (k < D.nin && preflag[D.inoff + k]) ? (float)(g + inpb64[D.inoff + k]) : (float)g;
+}
+
+hipError_t network_rows(ann_ctx* ctx, int form, float* G) {
+    if (ctx->npre > 0 && ctx->npre == ctx->nin_total) {
+        float* T = (ctx->d_T && ctx->work_atoms >= ctx->natoms) ? ctx->d_T : nullptr;
+        const hipError_t e = precise_inputs(ctx, G, nullptr, T);
+        ctx->T_gen = (e == hipSuccess && T && ctx->hm.moment_z > 0) ? ctx->gen : ~0ull;
+        return e;
+    }
+    const hipError_t e = symm_fwd(ctx, form, G);
+    if (e != hipSuccess) return e;
+    return precise_inputs(ctx, G, nullptr);
 }
 
 hipError_t widen_rows(ann_ctx* ctx, const float* G, double* G64) {

@@ -884,8 +884,7 @@ hipError_t train_step_impl(ann_ctx* ctx, int form, const double* Eref, const dou
     hipError_t e;
     const int B = ctx->nstruc, N = ctx->natoms;
     if (!ctx->G_valid) {
-        if ((e = symm_fwd(ctx, form, ctx->d_G)) != hipSuccess) return e;
-        if ((e = precise_inputs(ctx, ctx->d_G, nullptr)) != hipSuccess) return e;
+        if ((e = network_rows(ctx, form, ctx->d_G)) != hipSuccess) return e;
         ctx->G_valid = true;
     }
     if ((e = net_fwd(ctx, form, ctx->d_G, ctx->d_Eatom, ctx->d_dEdG, true)) != hipSuccess) return e;

@@ -55,7 +55,10 @@ def test_preconditioned_water_meets_the_bars(oracle_mod, form):
     eng, m2, pre = _preconditioned(model, batch)
     ratio = [np.abs(w * b).max() for w, b in pre]
     assert max(ratio) > 1e4                    # the workload does hit the ill-conditioned case
-    assert eng.precise_count() == sum(int((np.abs(w * b) > 1.0).sum()) for w, b in pre)
+    flagged = sum(int((np.abs(w * b) > 1.0).sum()) for w, b in pre)
+    total = sum(len(w) for w, _ in pre)
+    # more than half qualify: every input is evaluated in FP64
+    assert eng.precise_count() == (total if 2 * flagged > total else flagged)
     de, rf = _errors(eng, form, m2, batch, oracle_mod)
     print(f"  dE/atom={de:.3e} force relRMS={rf:.3e}", flush=True)
     assert de <= 1e-6
