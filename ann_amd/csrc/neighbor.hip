@@ -46,15 +46,22 @@ __global__ void k_struc_of(const int* __restrict__ off, int nstruc, int* __restr
 
 // bin of atom i from its fractional coordinates; writes the atom's integer
 // shift (the lattice vector that wraps it into the cell)
+// A position whose fractional coordinate is not finite (or beyond 1e8 cells)
+// is binned as if at the origin and reported through flag bit 16: the build
+// stays in bounds and ann_neighbors returns an argument error.
 __device__ __forceinline__ int bin_atom(int i, const double* __restrict__ pos, const int* __restrict__ sof,
                                         const CellInfo* __restrict__ ci, const double* __restrict__ rinv,
-                                        int* __restrict__ shift) {
+                                        int* __restrict__ shift, int* __restrict__ flag) {
     const int s = sof[i];
     const CellInfo c = ci[s];
     const double* Ri = rinv + 9 * s;
     int b[3];
     for (int a = 0; a < 3; ++a) {
-        const double f = Ri[3 * a] * pos[3 * i] + Ri[3 * a + 1] * pos[3 * i + 1] + Ri[3 * a + 2] * pos[3 * i + 2];
+        double f = Ri[3 * a] * pos[3 * i] + Ri[3 * a + 1] * pos[3 * i + 1] + Ri[3 * a + 2] * pos[3 * i + 2];
+        if (!(fabs(f) < 1e8)) {
+            f = 0.0;
+            if (flag) atomicOr(flag, 16);
+        }
         const double fl = floor(f);
         shift[3 * i + a] = (int)fl;
         int bb = (int)floor((f - fl) * c.nb[a]);
@@ -70,10 +77,11 @@ __device__ __forceinline__ double4 bin_record(int i, const double* __restrict__ 
 
 __global__ void k_bin(int N, const double* __restrict__ pos, const int* __restrict__ sof,
                       const CellInfo* __restrict__ ci, const double* __restrict__ rinv,
-                      int* __restrict__ bin_of, int* __restrict__ shift, int* __restrict__ bin_cnt) {
+                      int* __restrict__ bin_of, int* __restrict__ shift, int* __restrict__ bin_cnt,
+                      int* __restrict__ flag) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
-    const int g = bin_atom(i, pos, sof, ci, rinv, shift);
+    const int g = bin_atom(i, pos, sof, ci, rinv, shift, flag);
     bin_of[i] = g;
     atomicAdd(&bin_cnt[g], 1);
 }
@@ -774,7 +782,7 @@ hipError_t neighbors_build(ann_ctx* ctx, double rc) {
         {
             Launch L(ctx, "nb_bin");
             k_bin<<<(N + 255) / 256, 256, 0, st>>>(N, ctx->d_pos, ctx->d_struc_of, dci, ctx->d_cellmat,
-                                                  ctx->d_bin_of, ctx->d_shift, ctx->d_bin_cnt);
+                                                  ctx->d_bin_of, ctx->d_shift, ctx->d_bin_cnt, ctx->d_flag);
         }
         HIPCHK(exclusive_scan(ctx->d_bin_cnt, ctx->d_bin_start, nbins + 1, st));
         k_fill_bins<<<(N + 255) / 256, 256, 0, st>>>(N, ctx->d_bin_of, ctx->d_bin_start, bin_fill,
@@ -824,6 +832,11 @@ hipError_t neighbors_build(ann_ctx* ctx, double rc) {
         HIPCHK(wait_marker(ctx, 5, seq));
         for (int q = 0; q < 4; ++q) flags[q] = rb[q];
         total = rb[4];
+    }
+    if (flags[0] & 16) {
+        ctx->nb_status = ANN_ERR_ARG;
+        set_error("an atom position is not finite (or lies beyond 1e8 cells)");
+        return hipErrorInvalidValue;
     }
     if (flags[0] & 2) {
         ctx->nb_status = ANN_ERR_UNSUPPORTED;
