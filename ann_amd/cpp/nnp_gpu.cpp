@@ -100,6 +100,18 @@ struct DeviceStruc {
     }
     ~DeviceStruc() { free_all(); }
     void upload(ann_ctx* ctx, const Structure& s, double rc) {
+        set(ctx, s);
+        check(ann_neighbors(ctx, rc), "ann_neighbors");
+    }
+    // positions only (same atoms, same list): the Verlet-list reuse between builds
+    void positions(const Structure& s) {
+        const int N = s.nAtoms();
+        std::vector<double> p(3 * N);
+        for (int i = 0; i < N; ++i)
+            for (int x = 0; x < 3; ++x) p[3 * i + x] = s.posn(i)[x];
+        hipcheck(hipMemcpy(pos, p.data(), 3 * sizeof(double) * N, hipMemcpyHostToDevice), "hipMemcpy");
+    }
+    void set(ann_ctx* ctx, const Structure& s) {
         const int N = s.nAtoms();
         if (N < 1) throw std::invalid_argument("structure has no atoms");
         if (N > cap) {
@@ -124,7 +136,6 @@ struct DeviceStruc {
         hipcheck(hipMemcpy(species, sp.data(), sizeof(int) * N, hipMemcpyHostToDevice), "hipMemcpy");
         ann_batch b{1, hoff, off, pos, cell, species};
         check(ann_set_batch(ctx, &b), "ann_set_batch");
-        check(ann_neighbors(ctx, rc), "ann_neighbors");
     }
 };
 
@@ -165,7 +176,10 @@ void Structure::resize(int n) {
 
 // ------------------------------------------------------------------ NeighborList
 
-void NeighborList::build(const Structure& struc, double rc) {
+namespace {
+// the GPU's canonical list of a structure at rc: CSR offsets, partners, images, reverse slots
+void gpu_list(const Structure& struc, double rc, std::vector<int>& off, std::vector<int>& idx, std::vector<int>& img,
+              std::vector<int>& rev) {
     if (!(rc > 0)) throw std::invalid_argument("NeighborList: invalid cutoff");
     int S = 1;
     for (int i = 0; i < struc.nAtoms(); ++i) S = std::max(S, struc.type(i) + 1);
@@ -177,18 +191,26 @@ void NeighborList::build(const Structure& struc, double rc) {
     check(ann_nlist_size(ctx, &P, &maxn), "ann_nlist_size");
     const int N = struc.nAtoms();
     DevBuf<int> doff, didx, dimg, drev;
-    DevBuf<float> dgeo;
-    check(ann_nlist_copy(ctx, doff.get(N + 1), didx.get(P), dimg.get(3 * P), drev.get(P), dgeo.get(4 * P)),
-          "ann_nlist_copy");
-    off_.assign(N + 1, 0);
-    std::vector<int> idx(P), img(3 * P), rev(P);
+    check(ann_nlist_copy(ctx, doff.get(N + 1), didx.get(P), dimg.get(3 * P), drev.get(P), nullptr), "ann_nlist_copy");
+    off.assign(N + 1, 0);
+    idx.assign(P, 0);
+    img.assign(3 * P, 0);
+    rev.assign(P, 0);
     hipcheck(hipDeviceSynchronize(), "sync");
-    hipcheck(hipMemcpy(off_.data(), doff.p, sizeof(int) * (N + 1), hipMemcpyDeviceToHost), "hipMemcpy");
+    hipcheck(hipMemcpy(off.data(), doff.p, sizeof(int) * (N + 1), hipMemcpyDeviceToHost), "hipMemcpy");
     if (P > 0) {
         hipcheck(hipMemcpy(idx.data(), didx.p, sizeof(int) * P, hipMemcpyDeviceToHost), "hipMemcpy");
         hipcheck(hipMemcpy(img.data(), dimg.p, sizeof(int) * 3 * P, hipMemcpyDeviceToHost), "hipMemcpy");
         hipcheck(hipMemcpy(rev.data(), drev.p, sizeof(int) * P, hipMemcpyDeviceToHost), "hipMemcpy");
     }
+}
+}  // namespace
+
+void NeighborList::build(const Structure& struc, double rc) {
+    std::vector<int> idx, img, rev;
+    gpu_list(struc, rc, off_, idx, img, rev);
+    const int N = struc.nAtoms();
+    const int64_t P = off_[N];
     list_.assign(P, Neighbor{});
     const auto& R = struc.R();
     for (int i = 0; i < N; ++i) {
@@ -220,6 +242,33 @@ void NeighborList::build(const Structure& struc, double rc) {
     rc_ = rc;
 }
 
+// ------------------------------------------------------------------ verlet::List
+
+namespace verlet {
+
+void List::build(const Structure& struc) {
+    static uint64_t builds = 0;
+    std::vector<int> idx, img, rev;
+    gpu_list(struc, rc_, off_, idx, img, rev);
+    list_.assign(idx.size(), Neighbor{});
+    for (size_t q = 0; q < idx.size(); ++q) {
+        list_[q].index_ = idx[q];
+        list_[q].cell_ = {img[3 * q], img[3 * q + 1], img[3 * q + 2]};
+    }
+    id_ = ++builds;
+}
+
+void List::flat(std::vector<int>& idx, std::vector<int>& img) const {
+    idx.resize(list_.size());
+    img.resize(3 * list_.size());
+    for (size_t q = 0; q < list_.size(); ++q) {
+        idx[q] = list_[q].index_;
+        for (int x = 0; x < 3; ++x) img[3 * q + x] = list_[q].cell_[x];
+    }
+}
+
+}  // namespace verlet
+
 // ------------------------------------------------------------------ NNP tables
 
 struct NNP::Tables {
@@ -239,8 +288,10 @@ struct NNP::Tables {
 
 struct NNP::Device {
     DeviceStruc ds;
-    DevBuf<float> G, dE, Ea, P;
-    DevBuf<double> F, E;
+    DevBuf<float> dE, Ea, P;
+    DevBuf<double> G64, F, E;
+    uint64_t vlist_id = 0;        // the Verlet list the context's neighbour list was taken from (0: none)
+    int natoms = 0;
 };
 
 NNP::NNP() : t_(new Tables), dev_(new Device) {}
@@ -276,6 +327,29 @@ void NNP::upload(const Structure& struc) {
     for (int i = 0; i < struc.nAtoms(); ++i)
         if (struc.type(i) < 0 || struc.type(i) >= ntypes()) throw std::invalid_argument("structure species out of range");
     dev_->ds.upload(ctx(), struc, rc_);
+    dev_->vlist_id = 0;
+    dev_->natoms = struc.nAtoms();
+}
+
+// the context's list from a Verlet list: taken over (ann_set_neighbors) once per
+// build of the list; later calls with the same build move the positions only
+// and refresh the pair geometry from the list's image vectors
+void NNP::upload(const Structure& struc, const verlet::List& vlist) {
+    if (vlist.nAtoms() != struc.nAtoms() || vlist.id() == 0)
+        throw std::invalid_argument("verlet::List: build the list for this structure first");
+    if (dev_->vlist_id == vlist.id() && dev_->natoms == struc.nAtoms()) {
+        dev_->ds.positions(struc);
+        check(ann_refresh_geometry(ctx()), "ann_refresh_geometry");
+        return;
+    }
+    for (int i = 0; i < struc.nAtoms(); ++i)
+        if (struc.type(i) < 0 || struc.type(i) >= ntypes()) throw std::invalid_argument("structure species out of range");
+    dev_->ds.set(ctx(), struc);
+    std::vector<int> idx, img;
+    vlist.flat(idx, img);
+    check(ann_set_neighbors(ctx(), vlist.off().data(), idx.data(), img.data(), vlist.rc()), "ann_set_neighbors");
+    dev_->vlist_id = vlist.id();
+    dev_->natoms = struc.nAtoms();
 }
 
 // ------------------------------------------------------------------ file format
@@ -419,6 +493,138 @@ void NNP::read(const char* file, NNP& nnp) {
     if (nnp.ctx_) { ann_destroy(nnp.ctx_); nnp.ctx_ = nullptr; }
 }
 
+// NNP::read_basis (src/nnp/nnp.cpp:96-156): the lines after the "basis NAME"
+// line of an nnp file -- "rc", "ntypes", "basis NAME", then per partner type
+// "basis_radial J" + BasisR block and per pair j <= k "basis_angular J K" +
+// BasisA block.  The cutoff and type count must match the potential.
+void NNP::read_basis(const char* file, NNP& nnp, const char* name) {
+    std::ifstream in(file);
+    if (!in)
+        throw std::runtime_error(std::string("NNP::read_basis(const char*,NNP&,const char*): Could not open nnp file: \"") +
+                                 file + "\"");
+    std::vector<std::vector<std::string>> L;
+    std::string line;
+    while (std::getline(in, line)) {
+        auto t = tokens(line);
+        if (!t.empty()) L.push_back(t);
+    }
+    size_t q = 0;
+    auto next = [&]() -> const std::vector<std::string>& {
+        if (q >= L.size()) throw std::runtime_error("NNP::read_basis: unexpected end of file");
+        return L[q++];
+    };
+    const int II = nnp.index(name);
+    const double rc = std::atof(next().at(1).c_str());
+    if (rc != nnp.rc()) throw std::invalid_argument("NNP::read_basis(FILE*,NNP&,const char*): invalid cutoff.");
+    const int S = std::atoi(next().at(1).c_str());
+    if (S != nnp.ntypes()) throw std::invalid_argument("NNP::read_basis(FILE*,NNP&,const char*): invalid number of types.");
+    if (nnp.index(next().at(1)) != II)
+        throw std::invalid_argument("NNP::read_basis(FILE*,NNP&,const char*): invalid central types.\n");
+    Tables& T = *nnp.t_;
+    const int NP = S * (S + 1) / 2;
+    // unpack the flat basis tables into one block per (centre, slot)
+    std::vector<std::vector<double>> rrs(S * S), reta(S * S), aeta(S * NP), azeta(S * NP);
+    std::vector<std::vector<int>> alam(S * NP);
+    for (int t = 0; t < S * S; ++t)
+        for (int m = 0; m < T.rad_n[t]; ++m) {
+            rrs[t].push_back(T.rad_rs[T.rad_off[t] + m]);
+            reta[t].push_back(T.rad_eta[T.rad_off[t] + m]);
+        }
+    for (int t = 0; t < S * NP; ++t)
+        for (int m = 0; m < T.ang_n[t]; ++m) {
+            aeta[t].push_back(T.ang_eta[T.ang_off[t] + m]);
+            azeta[t].push_back(T.ang_zeta[T.ang_off[t] + m]);
+            alam[t].push_back(T.ang_lambda[T.ang_off[t] + m]);
+        }
+    for (int j = 0; j < S; ++j) {
+        const int t = II * S + nnp.index(next().at(1));
+        const auto& h = next();
+        const int cut = lookup(kCut, 5, h.at(1), "cutoff");
+        if (cut != T.cutoff) throw std::invalid_argument("NNP::read_basis: the cutoff function differs from the potential's");
+        const int n = std::atoi(h.at(4).c_str());
+        if (n < 0) throw std::invalid_argument("BasisR::read: invalid number of functions");
+        T.rad_name[t] = lookup(kRad, 7, h.at(3), "radial function");
+        T.rad_n[t] = n;
+        rrs[t].clear(); reta[t].clear();
+        for (int m = 0; m < n; ++m) {
+            const auto& v = next();
+            rrs[t].push_back(std::atof(v.at(0).c_str()));
+            reta[t].push_back(std::atof(v.at(1).c_str()));
+        }
+    }
+    for (int j = 0; j < S; ++j)
+        for (int k = j; k < S; ++k) {
+            const auto& pj = next();
+            const int t = II * NP + pair_index(nnp.index(pj.at(1)), nnp.index(pj.at(2)));
+            const auto& h = next();
+            const int n = std::atoi(h.at(4).c_str());
+            if (n < 0) throw std::invalid_argument("BasisA::read: invalid number of functions");
+            T.ang_name[t] = lookup(kAng, 7, h.at(3), "angular function");
+            T.ang_n[t] = n;
+            aeta[t].clear(); azeta[t].clear(); alam[t].clear();
+            for (int m = 0; m < n; ++m) {
+                const auto& v = next();
+                aeta[t].push_back(std::atof(v.at(0).c_str()));
+                azeta[t].push_back(std::atof(v.at(1).c_str()));
+                alam[t].push_back(std::atoi(v.at(2).c_str()));
+            }
+        }
+    T.rad_rs.clear(); T.rad_eta.clear(); T.ang_eta.clear(); T.ang_zeta.clear(); T.ang_lambda.clear();
+    for (int t = 0; t < S * S; ++t) {
+        T.rad_off[t] = (int)T.rad_rs.size();
+        T.rad_rs.insert(T.rad_rs.end(), rrs[t].begin(), rrs[t].end());
+        T.rad_eta.insert(T.rad_eta.end(), reta[t].begin(), reta[t].end());
+    }
+    for (int t = 0; t < S * NP; ++t) {
+        T.ang_off[t] = (int)T.ang_eta.size();
+        T.ang_eta.insert(T.ang_eta.end(), aeta[t].begin(), aeta[t].end());
+        T.ang_zeta.insert(T.ang_zeta.end(), azeta[t].begin(), azeta[t].end());
+        T.ang_lambda.insert(T.ang_lambda.end(), alam[t].begin(), alam[t].end());
+    }
+    if (T.rad_rs.empty()) { T.rad_rs.push_back(0); T.rad_eta.push_back(0); }
+    if (T.ang_eta.empty()) { T.ang_eta.push_back(0); T.ang_zeta.push_back(0); T.ang_lambda.push_back(0); }
+    int nin = 0;
+    for (int b = 0; b < S; ++b) nin += T.rad_n[II * S + b];
+    for (int p = 0; p < NP; ++p) nin += T.ang_n[II * NP + p];
+    if (nin != T.nin[II]) {
+        // the network's input layer follows the basis (NNH::init_input, src/nnp/nnh.cpp:94-112,
+        // then NN::ANN::resize at the training call site, src/nnp/nnptefr2.cpp): hidden
+        // layers and neuron are kept, parameters zeroed, input scaling reset to identity
+        const int old = T.nin[II];
+        const int L = T.nn_nlayer[II];
+        int np_old = 0, np_new = 0, prev_o = old, prev_n = nin;
+        for (int l = 0; l < L; ++l) {
+            const int no = T.nn_nodes[T.nn_nodes_off[II] + l];
+            np_old += no + no * prev_o;
+            np_new += no + no * prev_n;
+            prev_o = prev_n = no;
+        }
+        std::vector<double> params, inpw, inpb;
+        for (int a = 0; a < S; ++a) {
+            const int off = (int)params.size(), ioff = (int)inpw.size();
+            if (a == II) {
+                params.insert(params.end(), np_new, 0.0);
+                inpw.insert(inpw.end(), nin, 1.0);
+                inpb.insert(inpb.end(), nin, 0.0);
+            } else {
+                const int cnt = (a + 1 < S ? T.nn_par_off[a + 1] : (int)T.params.size()) - T.nn_par_off[a];
+                params.insert(params.end(), T.params.begin() + T.nn_par_off[a], T.params.begin() + T.nn_par_off[a] + cnt);
+                inpw.insert(inpw.end(), T.inpw.begin() + T.nn_inp_off[a], T.inpw.begin() + T.nn_inp_off[a] + T.nin[a]);
+                inpb.insert(inpb.end(), T.inpb.begin() + T.nn_inp_off[a], T.inpb.begin() + T.nn_inp_off[a] + T.nin[a]);
+            }
+            T.nn_par_off[a] = off;
+            T.nn_inp_off[a] = ioff;
+        }
+        (void)np_old;
+        T.params = params;
+        T.inpw = inpw;
+        T.inpb = inpb;
+        T.nin[II] = nin;
+    }
+    if (nnp.ctx_) { ann_destroy(nnp.ctx_); nnp.ctx_ = nullptr; }
+    nnp.dev_->vlist_id = 0;
+}
+
 void NNP::write(const char* file, const NNP& nnp) {
     FILE* w = std::fopen(file, "w");
     if (!w) throw std::runtime_error(std::string("NNP::write(const char*,const NNP&): Could not write to nnh file: \"") + file + "\"");
@@ -449,7 +655,7 @@ void NNP::write(const char* file, const NNP& nnp) {
                 }
             }
     }
-    const char* neuname[18] = {};
+    const char* neuname[19] = {};
     for (const auto& kv : kNeuron) neuname[kv.second] = kv.first.c_str();
     for (int a = 0; a < S; ++a) {
         std::fprintf(w, "nn %s\nnn %i ", nnp.types_[a].name.c_str(), T.nin[a]);
@@ -487,17 +693,18 @@ void NNP::init(const NNP& nnp, Structure& struc) {
     for (int i = 0; i < struc.nAtoms(); ++i) struc.symm(i).assign(nnp.nInput(struc.type(i)), 0.0);
 }
 
-void NNP::symm(NNP& nnp, Structure& struc, const NeighborList& nlist) {
-    (void)nlist;  // the list is rebuilt on the device (bit-identical to nlist for the same rc)
-    nnp.upload(struc);
+namespace {
+
+// G of the context's batch in FP64 into struc.symm
+void symm_out(NNP& nnp, ann_ctx* ctx, std::vector<double>& h, Structure& struc, DevBuf<double>& G64) {
     int S, nin, ldg, np;
-    check(ann_info(nnp.ctx(), &S, &nin, &ldg, &np), "ann_info");
+    check(ann_info(ctx, &S, &nin, &ldg, &np), "ann_info");
     const int N = struc.nAtoms();
-    float* G = nnp.dev_->G.get((size_t)N * ldg);
-    check(ann_symm_fwd(nnp.ctx(), form_from_env(), G), "ann_symm_fwd");
-    std::vector<float> h((size_t)N * ldg);
+    double* G = G64.get((size_t)N * ldg);
+    check(ann_symm_fwd64(ctx, form_from_env(), G), "ann_symm_fwd64");
+    h.resize((size_t)N * ldg);
     hipcheck(hipDeviceSynchronize(), "sync");
-    hipcheck(hipMemcpy(h.data(), G, sizeof(float) * h.size(), hipMemcpyDeviceToHost), "hipMemcpy");
+    hipcheck(hipMemcpy(h.data(), G, sizeof(double) * h.size(), hipMemcpyDeviceToHost), "hipMemcpy");
     for (int i = 0; i < N; ++i) {
         auto& g = struc.symm(i);
         g.assign(nnp.nInput(struc.type(i)), 0.0);
@@ -505,27 +712,55 @@ void NNP::symm(NNP& nnp, Structure& struc, const NeighborList& nlist) {
     }
 }
 
-static void upload_symm(NNP& nnp, const Structure& struc, float* G, int ldg) {
+// struc.symm (FP64, as the reference keeps it) into the device rows
+double* symm_in(NNP& nnp, ann_ctx* ctx, const Structure& struc, DevBuf<double>& G64, int& ldg) {
+    int S, nin, np;
+    check(ann_info(ctx, &S, &nin, &ldg, &np), "ann_info");
     const int N = struc.nAtoms();
-    std::vector<float> h((size_t)N * ldg, 0.0f);
+    std::vector<double> h((size_t)N * ldg, 0.0);
     for (int i = 0; i < N; ++i) {
         const auto& g = struc.symm(i);
         if ((int)g.size() != nnp.nInput(struc.type(i)))
             throw std::invalid_argument("structure symmetry functions not initialised (NNP::init / NNP::symm)");
-        for (size_t k = 0; k < g.size(); ++k) h[(size_t)i * ldg + k] = (float)g[k];
+        for (size_t k = 0; k < g.size(); ++k) h[(size_t)i * ldg + k] = g[k];
     }
-    hipcheck(hipMemcpy(G, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice), "hipMemcpy");
+    double* G = G64.get(h.size());
+    hipcheck(hipMemcpy(G, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice), "hipMemcpy");
+    return G;
+}
+
+void forces_out(ann_ctx* ctx, Structure& struc, double* F, double* E) {
+    const int N = struc.nAtoms();
+    std::vector<double> h(3 * (size_t)N);
+    hipcheck(hipDeviceSynchronize(), "sync");
+    hipcheck(hipMemcpy(h.data(), F, sizeof(double) * h.size(), hipMemcpyDeviceToHost), "hipMemcpy");
+    if (E) hipcheck(hipMemcpy(&struc.pe(), E, sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
+    for (int i = 0; i < N; ++i) struc.force(i) = Vec3{h[3 * i], h[3 * i + 1], h[3 * i + 2]};
+    (void)ctx;
+}
+
+}  // namespace
+
+void NNP::symm(NNP& nnp, Structure& struc, const NeighborList& nlist) {
+    (void)nlist;  // the list is rebuilt on the device (bit-identical to nlist for the same rc)
+    nnp.upload(struc);
+    std::vector<double> h;
+    symm_out(nnp, nnp.ctx(), h, struc, nnp.dev_->G64);
+}
+
+void NNP::symm(NNP& nnp, Structure& struc, const verlet::List& vlist) {
+    nnp.upload(struc, vlist);
+    std::vector<double> h;
+    symm_out(nnp, nnp.ctx(), h, struc, nnp.dev_->G64);
 }
 
 double NNP::energy(NNP& nnp, const Structure& struc) {
     nnp.upload(struc);
-    int S, nin, ldg, np;
-    check(ann_info(nnp.ctx(), &S, &nin, &ldg, &np), "ann_info");
     const int N = struc.nAtoms();
-    float* G = nnp.dev_->G.get((size_t)N * ldg);
+    int ldg = 0;
+    double* G = symm_in(nnp, nnp.ctx(), struc, nnp.dev_->G64, ldg);
     float* Ea = nnp.dev_->Ea.get(N);
-    upload_symm(nnp, struc, G, ldg);
-    check(ann_net_fwd(nnp.ctx(), form_from_env(), G, Ea, nullptr), "ann_net_fwd");
+    check(ann_net_fwd64(nnp.ctx(), form_from_env(), G, Ea, nullptr), "ann_net_fwd64");
     std::vector<float> e(N);
     hipcheck(hipDeviceSynchronize(), "sync");
     hipcheck(hipMemcpy(e.data(), Ea, sizeof(float) * N, hipMemcpyDeviceToHost), "hipMemcpy");
@@ -534,43 +769,57 @@ double NNP::energy(NNP& nnp, const Structure& struc) {
     return pe;
 }
 
-void NNP::force(NNP& nnp, Structure& struc, const NeighborList& nlist) {
-    (void)nlist;
-    nnp.upload(struc);
-    int S, nin, ldg, np;
-    check(ann_info(nnp.ctx(), &S, &nin, &ldg, &np), "ann_info");
+// forces from struc.symm through the context's current list (NNP::force,
+// src/nnp/nnp.cpp:461-520, 522-587)
+static void force_from_symm(NNP& nnp, ann_ctx* ctx, Structure& struc, DevBuf<double>& G64, DevBuf<float>& dEb,
+                            DevBuf<float>& Eab, DevBuf<float>& Pb, DevBuf<double>& Fb) {
     const int N = struc.nAtoms();
     int64_t P = 0;
     int maxn = 0;
-    check(ann_nlist_size(nnp.ctx(), &P, &maxn), "ann_nlist_size");
+    check(ann_nlist_size(ctx, &P, &maxn), "ann_nlist_size");
     const int form = form_from_env();
-    float* G = nnp.dev_->G.get((size_t)N * ldg);
-    float* dE = nnp.dev_->dE.get((size_t)N * ldg);
-    float* Ea = nnp.dev_->Ea.get(N);
-    float* Pd = nnp.dev_->P.get(3 * (size_t)std::max<int64_t>(P, 1));
-    double* F = nnp.dev_->F.get(3 * (size_t)N);
-    upload_symm(nnp, struc, G, ldg);
-    check(ann_net_fwd(nnp.ctx(), form, G, Ea, dE), "ann_net_fwd");
-    check(ann_symm_bwd(nnp.ctx(), form, dE, Pd), "ann_symm_bwd");
-    check(ann_force_reduce(nnp.ctx(), Pd, Ea, 0, F, nullptr, nullptr), "ann_force_reduce");
-    std::vector<double> h(3 * (size_t)N);
-    hipcheck(hipDeviceSynchronize(), "sync");
-    hipcheck(hipMemcpy(h.data(), F, sizeof(double) * h.size(), hipMemcpyDeviceToHost), "hipMemcpy");
-    for (int i = 0; i < N; ++i) struc.force(i) = Vec3{h[3 * i], h[3 * i + 1], h[3 * i + 2]};
+    int ldg = 0;
+    double* G = symm_in(nnp, ctx, struc, G64, ldg);
+    float* dE = dEb.get((size_t)N * ldg);
+    float* Ea = Eab.get(N);
+    float* Pd = Pb.get(3 * (size_t)std::max<int64_t>(P, 1));
+    double* F = Fb.get(3 * (size_t)N);
+    check(ann_net_fwd64(ctx, form, G, Ea, dE), "ann_net_fwd64");
+    check(ann_symm_bwd(ctx, form, dE, Pd), "ann_symm_bwd");
+    check(ann_force_reduce(ctx, Pd, Ea, 0, F, nullptr, nullptr), "ann_force_reduce");
+    forces_out(ctx, struc, F, nullptr);
+}
+
+void NNP::force(NNP& nnp, Structure& struc, const NeighborList& nlist) {
+    (void)nlist;
+    nnp.upload(struc);
+    Device& d = *nnp.dev_;
+    force_from_symm(nnp, nnp.ctx(), struc, d.G64, d.dE, d.Ea, d.P, d.F);
+}
+
+void NNP::force(NNP& nnp, Structure& struc, const verlet::List& vlist) {
+    nnp.upload(struc, vlist);
+    Device& d = *nnp.dev_;
+    force_from_symm(nnp, nnp.ctx(), struc, d.G64, d.dE, d.Ea, d.P, d.F);
+}
+
+static void compute_dev(ann_ctx* ctx, Structure& struc, DevBuf<double>& Fb, DevBuf<double>& Eb) {
+    const int N = struc.nAtoms();
+    double* F = Fb.get(3 * (size_t)N);
+    double* E = Eb.get(1);
+    check(ann_energy_forces(ctx, form_from_env(), 0, E, F, nullptr), "ann_energy_forces");
+    forces_out(ctx, struc, F, E);
 }
 
 void NNP::compute(NNP& nnp, Structure& struc, const NeighborList& nlist) {
     (void)nlist;
     nnp.upload(struc);
-    const int N = struc.nAtoms();
-    double* F = nnp.dev_->F.get(3 * (size_t)N);
-    double* E = nnp.dev_->E.get(1);
-    check(ann_energy_forces(nnp.ctx(), form_from_env(), 0, E, F, nullptr), "ann_energy_forces");
-    std::vector<double> h(3 * (size_t)N);
-    hipcheck(hipDeviceSynchronize(), "sync");
-    hipcheck(hipMemcpy(h.data(), F, sizeof(double) * h.size(), hipMemcpyDeviceToHost), "hipMemcpy");
-    hipcheck(hipMemcpy(&struc.pe(), E, sizeof(double), hipMemcpyDeviceToHost), "hipMemcpy");
-    for (int i = 0; i < N; ++i) struc.force(i) = Vec3{h[3 * i], h[3 * i + 1], h[3 * i + 2]};
+    compute_dev(nnp.ctx(), struc, nnp.dev_->F, nnp.dev_->E);
+}
+
+void NNP::compute(NNP& nnp, Structure& struc, const verlet::List& vlist) {
+    nnp.upload(struc, vlist);
+    compute_dev(nnp.ctx(), struc, nnp.dev_->F, nnp.dev_->E);
 }
 
 }  // namespace ann_gpu

@@ -15,6 +15,7 @@
 // the reference.  Vectors are std::array / std::vector instead of Eigen types.
 #pragma once
 #include <array>
+#include <cstdint>
 #include <memory>
 #include <string>
 #include <vector>
@@ -90,6 +91,49 @@ private:
     double rc_ = 0;
 };
 
+namespace verlet {
+
+// one entry of a Verlet list: partner index and image vector (the reference
+// reads index() and cell(), src/nnp/nnp.cpp:389-395)
+class Neighbor {
+public:
+    int index() const { return index_; }
+    const std::array<int, 3>& cell() const { return cell_; }
+    int index_ = 0;
+    std::array<int, 3> cell_{};
+};
+
+// Verlet list (src/struc absent; contract from src/nnp/nnp.cpp:377-427 and its
+// engine src/torch/engine.cpp:64-125: rc() set to the cutoff plus skin, build()
+// every stride() steps, entries reused between builds while atoms move).  Built
+// on the GPU with the canonical order of NeighborList; the pair vectors are
+// r_i - r_j - R n from the structure's current positions.
+class List {
+public:
+    double& rc() { return rc_; }
+    double rc() const { return rc_; }
+    int& stride() { return stride_; }
+    int stride() const { return stride_; }
+    void build(const Structure& struc);
+    void clear() { off_.assign(1, 0); list_.clear(); id_ = 0; }
+    int size(int i) const { return off_[i + 1] - off_[i]; }
+    const Neighbor& neigh(int i, int j) const { return list_[off_[i] + j]; }
+    int nAtoms() const { return (int)off_.size() - 1; }
+    // flat CSR view (off [N+1], idx, img [P][3]) and a build counter, for the GPU context
+    const std::vector<int>& off() const { return off_; }
+    uint64_t id() const { return id_; }
+    void flat(std::vector<int>& idx, std::vector<int>& img) const;
+
+private:
+    double rc_ = 0;
+    int stride_ = 1;
+    uint64_t id_ = 0;
+    std::vector<int> off_{0};
+    std::vector<Neighbor> list_;
+};
+
+}  // namespace verlet
+
 struct TypeInfo {
     std::string name;
     double mass = 0, energy = 0;
@@ -115,11 +159,21 @@ public:
 
     static void read(const char* file, NNP& nnp);
     static void write(const char* file, const NNP& nnp);
+    // basis of one species from a basis file (src/nnp/nnp.cpp:96-156); the
+    // species' network is resized to the new input count with zero parameters
+    // when that count changes
+    static void read_basis(const char* file, NNP& nnp, const char* name);
     static void init(const NNP& nnp, Structure& struc);
+    // symmetry functions in FP64 as the reference stores them (inputs whose
+    // scaling needs it are evaluated in FP64 throughout, the rest in FP32)
     static void symm(NNP& nnp, Structure& struc, const NeighborList& nlist);
+    static void symm(NNP& nnp, Structure& struc, const verlet::List& vlist);
     static double energy(NNP& nnp, const Structure& struc);
     static void force(NNP& nnp, Structure& struc, const NeighborList& nlist);
+    static void force(NNP& nnp, Structure& struc, const verlet::List& vlist);
     static void compute(NNP& nnp, Structure& struc, const NeighborList& nlist);
+    static void compute(NNP& nnp, Structure& struc, const verlet::List& vlist);
+    ann_ctx* context() { return ctx(); }     // the GPU context evaluating the potential
 
     // flat tables in the layout of ann_model_desc (public for the C ABI)
     struct Tables;
@@ -128,6 +182,7 @@ public:
 private:
     ann_ctx* ctx();
     void upload(const Structure& struc);
+    void upload(const Structure& struc, const verlet::List& vlist);
     std::vector<TypeInfo> types_;
     double rc_ = 0;
     ann_ctx* ctx_ = nullptr;

@@ -649,6 +649,22 @@ extern "C" int ann_neighbors(ann_ctx* ctx, double rc) {
     return ANN_OK;
 }
 
+extern "C" int ann_set_neighbors(ann_ctx* ctx, const int* off, const int* idx, const int* img, double rc) {
+    CHECK_CTX(ctx);
+    if (!ctx->have_batch) return fail(ANN_ERR_STATE, "no batch set");
+    if (!off || (off[ctx->natoms] > 0 && (!idx || !img))) return fail(ANN_ERR_ARG, "null neighbour list");
+    if (!(rc > 0) || std::isinf(rc)) return fail(ANN_ERR_ARG, "invalid neighbour cutoff");
+    ctx->have_nlist = false;
+    const int r = set_neighbors_host(ctx, off, idx, img, rc);
+    if (r != ANN_OK) return r;
+    ctx->have_nlist = true;
+    ctx->G_valid = false;
+    ctx->nl_rc = rc;
+    ctx->pos_ref_valid = false;
+    ctx->gen++;
+    return ANN_OK;
+}
+
 extern "C" int ann_refresh_geometry(ann_ctx* ctx) {
     CHECK_CTX(ctx);
     if (!ctx->have_nlist) return fail(ANN_ERR_STATE, "build the neighbour list first (ann_neighbors)");

@@ -346,6 +346,8 @@ int64_t noalloc_need(const ann_ctx* ctx, bool virial);
 hipError_t force_reduce_noalloc(ann_ctx* ctx, const float* P, const float* Eatom, int with_evac, double* F,
                                 double* E, double* V);
 hipError_t refresh_geometry(ann_ctx* ctx);
+// the caller's host CSR list (off, idx, img [P][3]) for the current batch (ANN status)
+int set_neighbors_host(ann_ctx* ctx, const int* off, const int* idx, const int* img, double rc);
 hipError_t wrap_positions(ann_ctx* ctx, double* pos);
 hipError_t md_kick_drift(ann_ctx* ctx, double dt, const double* mass, const double* F, double* vel, double* pos,
                          double lim2);

@@ -25,6 +25,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <tuple>
 
 namespace ann {
 
@@ -861,6 +862,96 @@ hipError_t neighbors_build(ann_ctx* ctx, double rc) {
     nl.cap = std::min(std::min(c1, c2), std::min(std::min(c3, c4), c5));
     launch_compact(ctx, (int)std::min<int64_t>(nl.cap, INT32_MAX));
     return hipGetLastError();
+}
+
+// ---- a neighbour list supplied by the caller ----
+
+__global__ void k_neigh_species(int64_t P, const int* __restrict__ idx, const int* __restrict__ species,
+                                uint8_t* __restrict__ nsp) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < P) nsp[s] = (uint8_t)species[idx[s]];
+}
+
+// Host CSR (off [N+1], idx [P], img [P][3]) of the current batch into the
+// context's list: packed images, reverse slots found by sorting the entries,
+// neighbour species and pair geometry on the device.  Returns an ANN status.
+int set_neighbors_host(ann_ctx* ctx, const int* off, const int* idx, const int* img, double rc) {
+    const int N = ctx->natoms;
+    if (off[0] != 0) return fail(ANN_ERR_ARG, "neighbour list: off[0] must be 0");
+    int maxn = 0;
+    for (int i = 0; i < N; ++i) {
+        if (off[i + 1] < off[i]) return fail(ANN_ERR_ARG, "neighbour list: offsets must not decrease");
+        maxn = std::max(maxn, off[i + 1] - off[i]);
+    }
+    if (maxn > kMaxNeigh)
+        return fail(ANN_ERR_CAPACITY, "an atom has " + std::to_string(maxn) + " neighbours; capacity is " +
+                                          std::to_string(kMaxNeigh));
+    const int64_t P = off[N];
+    std::vector<uint32_t> pimg(std::max<int64_t>(P, 1));
+    std::vector<int> rev(std::max<int64_t>(P, 1), -1), owner(std::max<int64_t>(P, 1));
+    int s = 0;
+    for (int i = 0; i < N; ++i) {
+        while (ctx->atom_off[s + 1] <= i) ++s;
+        for (int q = off[i]; q < off[i + 1]; ++q) {
+            const int j = idx[q];
+            if (j < ctx->atom_off[s] || j >= ctx->atom_off[s + 1])
+                return fail(ANN_ERR_ARG, "neighbour list: partner outside the centre's structure");
+            for (int x = 0; x < 3; ++x)
+                if (img[3 * (int64_t)q + x] < -511 || img[3 * (int64_t)q + x] > 511)
+                    return fail(ANN_ERR_UNSUPPORTED, "neighbour list: image vector beyond +-511 cells");
+            pimg[q] = pack_img(img[3 * (int64_t)q], img[3 * (int64_t)q + 1], img[3 * (int64_t)q + 2]);
+            owner[q] = i;
+            if (j == i && pimg[q] == pack_img(0, 0, 0))
+                return fail(ANN_ERR_ARG, "neighbour list: atom " + std::to_string(i) + " lists itself in its own cell");
+        }
+    }
+    // reverse slots: entry (i -> j, n) pairs with (j -> i, -n); sort by (centre, partner, image)
+    std::vector<int64_t> order(P);
+    for (int64_t q = 0; q < P; ++q) order[q] = q;
+    auto key = [&](int64_t q) { return std::make_tuple(owner[q], idx[q], pimg[q]); };
+    std::sort(order.begin(), order.end(), [&](int64_t x, int64_t y) { return key(x) < key(y); });
+    for (int64_t q = 1; q < P; ++q)
+        if (key(order[q]) == key(order[q - 1]))
+            return fail(ANN_ERR_ARG, "neighbour list: atom " + std::to_string(owner[order[q]]) + " lists partner " +
+                                         std::to_string(idx[order[q]]) + " twice with the same image");
+    for (int64_t q = 0; q < P; ++q) {
+        int n0, n1, n2;
+        unpack_img(pimg[q], n0, n1, n2);
+        const auto want = std::make_tuple(idx[q], owner[q], pack_img(-n0, -n1, -n2));
+        auto it = std::lower_bound(order.begin(), order.end(), want,
+                                   [&](int64_t x, const std::tuple<int, int, uint32_t>& w) { return key(x) < w; });
+        if (it == order.end() || key(*it) != want)
+            return fail(ANN_ERR_ARG, "neighbour list is not symmetric: entry " + std::to_string(q) + " (atom " +
+                                         std::to_string(owner[q]) + " -> " + std::to_string(idx[q]) +
+                                         ") has no reverse entry");
+        rev[q] = (int)*it;
+    }
+    NList& nl = ctx->nl;
+    hipStream_t st = ctx->stream;
+    int64_t c1 = nl.cap, c2 = nl.cap, c3 = nl.cap, c4 = nl.cap, c5 = nl.cap;
+    if (P > nl.cap) {
+        ANN_HIP(hipStreamSynchronize(st));
+        ANN_HIP(grow(nl.idx, c1, P));
+        ANN_HIP(grow(nl.img, c2, P));
+        ANN_HIP(grow(nl.rev, c3, P));
+        ANN_HIP(grow(nl.geo, c4, P));
+        ANN_HIP(grow(nl.nsp, c5, P));
+        nl.cap = std::min(std::min(c1, c2), std::min(std::min(c3, c4), c5));
+    }
+    ANN_HIP(hipMemcpyAsync(nl.off, off, sizeof(int) * (N + 1), hipMemcpyHostToDevice, st));
+    if (P > 0) {
+        ANN_HIP(hipMemcpyAsync(nl.idx, idx, sizeof(int) * P, hipMemcpyHostToDevice, st));
+        ANN_HIP(hipMemcpyAsync(nl.img, pimg.data(), sizeof(uint32_t) * P, hipMemcpyHostToDevice, st));
+        ANN_HIP(hipMemcpyAsync(nl.rev, rev.data(), sizeof(int) * P, hipMemcpyHostToDevice, st));
+        k_neigh_species<<<(unsigned)((P + 255) / 256), 256, 0, st>>>(P, nl.idx, ctx->d_species, nl.nsp);
+        ANN_HIP(hipGetLastError());
+    }
+    nl.npairs = P;
+    nl.maxn = maxn;
+    ANN_HIP(refresh_geometry(ctx));
+    ANN_HIP(hipStreamSynchronize(st));       // the host vectors go out of scope
+    (void)rc;
+    return ANN_OK;
 }
 
 // Pair geometry of every slot from the current positions and the list's image

@@ -58,7 +58,7 @@ EXPORTS = ["ann_last_error", "ann_create", "ann_destroy", "ann_set_stream", "ann
            "ann_autotune", "ann_get_tuning", "ann_set_tuning", "ann_input_stats",
            "ann_get_optimizer_state", "ann_set_optimizer_state", "ann_symm_fwd64", "ann_net_fwd64",
            "ann_set_precise_ratio", "ann_precise_count", "ann_force_reduce_noalloc", "ann_set_loss",
-           "ann_set_struc_weights"]
+           "ann_set_struc_weights", "ann_set_neighbors"]
 
 
 def lib():
@@ -205,6 +205,14 @@ class Engine:
 
     def neighbors(self, rc: float = 0.0) -> None:
         _check(lib().ann_neighbors(self.h, C.c_double(rc)))
+
+    def set_neighbors(self, off, idx, img, rc: float) -> None:
+        """A neighbour list of the caller's (ann_set_neighbors): host CSR."""
+        off = np.ascontiguousarray(off, dtype=np.int32)
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        img = np.ascontiguousarray(img, dtype=np.int32).reshape(-1)
+        _check(lib().ann_set_neighbors(self.h, off.ctypes.data_as(P_I), idx.ctypes.data_as(P_I),
+                                       img.ctypes.data_as(P_I), C.c_double(rc)))
 
     def nlist_size(self):
         P, m = C.c_int64(), C.c_int()

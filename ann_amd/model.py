@@ -124,7 +124,11 @@ class Model:
 
     def copy(self) -> "Model":
         return dataclasses.replace(
-            self,
+            self, masses=self.masses.copy(), evac=self.evac.copy(),
+            radial=[[tuple(np.copy(x) if isinstance(x, np.ndarray) else x for x in t) for t in row]
+                    for row in self.radial],
+            angular=[[tuple(np.copy(x) if isinstance(x, np.ndarray) else x for x in t) for t in row]
+                     for row in self.angular],
             nets=[dataclasses.replace(n, params=n.params.copy(), inpw=n.inpw.copy(),
                                       inpb=n.inpb.copy()) for n in self.nets])
 

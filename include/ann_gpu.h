@@ -117,6 +117,17 @@ int ann_set_scaling(ann_ctx* ctx, int species, const double* inpw, const double*
  * until the next ann_set_batch. */
 int ann_set_batch(ann_ctx* ctx, const ann_batch* batch);
 int ann_neighbors(ann_ctx* ctx, double rc);
+/* A neighbour list supplied by the caller, in place of ann_neighbors (the
+ * reference's verlet::List reuse, src/nnp/nnp.cpp:377-427, 522-587, 667-736, and
+ * LAMMPS-style callers with their own lists): host CSR off [N+1], partner
+ * index idx [P] (batch-global, in the centre's structure) and image vector
+ * img [P][3] per entry, r_i - r_j - R n; rows in any order, at most 512
+ * entries per atom, images within +-511 cells, no entry twice, no atom with
+ * itself at n = 0, and every (i -> j, n) matched by (j -> i, -n) (else
+ * ANN_ERR_ARG).  rc is the list's cutoff (entries beyond the model cutoff
+ * contribute nothing).  The pair geometry comes from the batch's current
+ * positions; synchronous. */
+int ann_set_neighbors(ann_ctx* ctx, const int* off, const int* idx, const int* img, double rc);
 /* synchronous: total pairs and the largest per-atom count */
 int ann_nlist_size(ann_ctx* ctx, int64_t* npairs, int* maxn);
 /* device-to-device copies of the list (any pointer may be NULL):

@@ -6,9 +6,17 @@
 //
 //   test_nnp_gpu roundtrip <in.nnp> <out.nnp>   read + write (no GPU)
 //   test_nnp_gpu force <pot.nnp> <seed>         GPU checks, prints errors, exit 0 on pass
+//   test_nnp_gpu dump <pot.nnp> <struc.txt> <out.txt> <skin> [<basis.txt> <name>]
+//       every entry point on one structure, written out for comparison with
+//       the oracle: NeighborList overloads at positions 0, then verlet::List
+//       overloads (list built at positions 0 with rc + skin, evaluated at
+//       positions 1).  With a basis file, NNP::read_basis replaces the basis
+//       of species <name> first.  struc.txt: N, the 9 entries of R (columns
+//       are lattice vectors), then N lines "type x0 y0 z0 x1 y1 z1".
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <fstream>
 #include <random>
 #include <string>
 
@@ -37,11 +45,78 @@ static double energy_of(NNP& nnp, Structure s, double rc) {
     return NNP::energy(nnp, s);
 }
 
+static void put_symm(FILE* o, const char* tag, const Structure& s) {
+    std::fprintf(o, "%s\n", tag);
+    for (int i = 0; i < s.nAtoms(); ++i) {
+        for (double g : s.symm(i)) std::fprintf(o, "%.17g ", g);
+        std::fprintf(o, "\n");
+    }
+}
+
+static void put_force(FILE* o, const char* tag, const Structure& s) {
+    std::fprintf(o, "%s\n", tag);
+    for (int i = 0; i < s.nAtoms(); ++i)
+        std::fprintf(o, "%.17g %.17g %.17g\n", s.force(i)[0], s.force(i)[1], s.force(i)[2]);
+}
+
+static int dump(NNP& nnp, int argc, char** argv) {
+    if (argc < 6) { std::fprintf(stderr, "usage: test_nnp_gpu dump pot struc out skin [basis name]\n"); return 2; }
+    if (argc >= 8) NNP::read_basis(argv[6], nnp, argv[7]);
+    std::ifstream in(argv[3]);
+    int N = 0;
+    in >> N;
+    std::array<double, 9> R;
+    for (auto& r : R) in >> r;
+    Structure s(N);
+    s.init(R);
+    std::vector<Vec3> x0(N), x1(N);
+    for (int i = 0; i < N; ++i) {
+        in >> s.type(i);
+        for (int x = 0; x < 3; ++x) in >> x0[i][x];
+        for (int x = 0; x < 3; ++x) in >> x1[i][x];
+        s.posn(i) = x0[i];
+    }
+    if (!in) { std::fprintf(stderr, "bad structure file\n"); return 2; }
+    FILE* o = std::fopen(argv[4], "w");
+    // NeighborList overloads at positions 0
+    NeighborList nl(s, nnp.rc());
+    NNP::init(nnp, s);
+    NNP::symm(nnp, s, nl);
+    put_symm(o, "G0", s);
+    std::fprintf(o, "E0\n%.17g\n", NNP::energy(nnp, s));
+    NNP::force(nnp, s, nl);
+    put_force(o, "Fforce0", s);
+    NNP::compute(nnp, s, nl);
+    std::fprintf(o, "pe0\n%.17g\n", s.pe());
+    put_force(o, "Fcompute0", s);
+    // verlet::List overloads: list at positions 0 (rc + skin), evaluation at positions 1
+    verlet::List vl;
+    vl.rc() = nnp.rc() + std::atof(argv[5]);
+    vl.build(s);
+    NNP::compute(nnp, s, vl);                 // takes the list over (positions 0)
+    std::fprintf(o, "pev0\n%.17g\n", s.pe());
+    for (int i = 0; i < N; ++i) s.posn(i) = x1[i];
+    NNP::symm(nnp, s, vl);                    // positions-only refresh
+    put_symm(o, "G1", s);
+    std::fprintf(o, "E1\n%.17g\n", NNP::energy(nnp, s));
+    NNP::force(nnp, s, vl);
+    put_force(o, "Fforce1", s);
+    NNP::compute(nnp, s, vl);
+    std::fprintf(o, "pe1\n%.17g\n", s.pe());
+    put_force(o, "Fcompute1", s);
+    int P = 0;
+    for (int i = 0; i < N; ++i) P += vl.size(i);
+    std::fprintf(o, "vpairs\n%d\n", P);
+    std::fclose(o);
+    return 0;
+}
+
 int main(int argc, char** argv) {
     if (argc < 4) { std::fprintf(stderr, "usage: test_nnp_gpu roundtrip|force ...\n"); return 2; }
     const std::string mode = argv[1];
     NNP nnp;
     NNP::read(argv[2], nnp);
+    if (mode == "dump") return dump(nnp, argc, argv);
     if (mode == "roundtrip") {
         NNP::write(argv[3], nnp);
         std::printf("types %d rc %f params %d\n", nnp.ntypes(), nnp.rc(), nnp.size());

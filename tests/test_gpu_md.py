@@ -173,3 +173,96 @@ def test_ef_graph_replays_match_eager():
         for a, b in zip(outs[0], outs[1]):
             assert torch.equal(a, b), step
     e2.ef_graph(False)
+
+
+def _caller_list(orc, b, rc, seed):
+    """The oracle's lists of every structure at rc, batch-global partner
+    indices, each atom's row in a shuffled order (a caller's own list)."""
+    rng = np.random.default_rng(seed)
+    offs, idxs, imgs = [0], [], []
+    for k in range(b.nstruc):
+        nl = orc.nlist(*b.struc(k), rc=rc)
+        a0 = int(b.atom_off[k])
+        for i in range(b.natoms[k]):
+            lo, hi = int(nl.off[i]), int(nl.off[i + 1])
+            perm = lo + rng.permutation(hi - lo)
+            idxs.append(nl.idx[perm] + a0)
+            imgs.append(nl.img[perm])
+            offs.append(offs[-1] + hi - lo)
+    return (np.array(offs, np.int32), np.concatenate(idxs).astype(np.int32),
+            np.concatenate(imgs).astype(np.int32))
+
+
+@pytest.mark.parametrize("form", [0, 1])
+def test_caller_list_matches_oracle(oracle_mod, form):
+    """ann_set_neighbors with a shuffled skin list over a two-structure batch:
+    energies and forces as with the library's own build, against the oracle."""
+    from ann_amd import configs
+    from ann_amd.gpu import Engine
+    model = configs.model_water()
+    b = configs.batch_water(2, seed=33, nmol=24, L=9.0)
+    orc = oracle_mod.Oracle(model)
+    off, idx, img = _caller_list(orc, b, model.rc + SKIN, 9)
+    print(f"caller list form {form}: {len(idx)} entries", flush=True)
+    eng = Engine(model, "cuda:0")
+    eng.set_batch(b)
+    eng.set_neighbors(off, idx, img, model.rc + SKIN)
+    P, maxn = eng.nlist_size()
+    assert P == len(idx)
+    E, F, V = eng.energy_forces(form)
+    Eo, Fo, Vo = orc.compute_batch(b)
+    de = float(np.max(np.abs(E.cpu().numpy() - Eo) / b.natoms))
+    rf = float(np.sqrt(((F.cpu().numpy() - Fo) ** 2).sum() / (Fo ** 2).sum()))
+    print(f"  dE/atom={de:.3e} force relRMS={rf:.3e}", flush=True)
+    assert de <= 1e-6 and rf <= 1e-5
+    # reverse slots pair every entry with its mirror
+    nl = eng.nlist()
+    own = np.repeat(np.arange(b.ntotal), np.diff(nl["off"]))
+    r = nl["rev"]
+    assert np.array_equal(nl["idx"][r], own) and np.array_equal(nl["img"][r], -nl["img"])
+
+
+def test_caller_list_errors(oracle_mod):
+    from ann_amd import configs
+    from ann_amd.gpu import AnnError, Engine
+    model = configs.model_water()
+    b = configs.batch_water(2, seed=34, nmol=16, L=8.0)
+    orc = oracle_mod.Oracle(model)
+    off, idx, img = _caller_list(orc, b, model.rc, 3)
+    eng = Engine(model, "cuda:0")
+    eng.set_batch(b)
+
+    def code(o, i, m):
+        with pytest.raises(AnnError) as ei:
+            eng.set_neighbors(o, i, m, model.rc)
+        print(f"  {ei.value}", flush=True)
+        return ei.value.code
+
+    # one entry dropped: its mirror has no reverse
+    o2 = off.copy()
+    o2[1:] -= 1
+    o2[0] = 0
+    assert code(o2, idx[1:], img[1:]) == -1
+    # one entry twice (and its mirror twice): duplicate
+    j, n = int(idx[0]), img[0]
+    rows = [(0, j, n), (j, 0, -n)]
+    o3, i3, m3 = [0], [], []
+    for a in range(b.ntotal):
+        lo, hi = off[a], off[a + 1]
+        i3 += list(idx[lo:hi]) + [r[1] for r in rows if r[0] == a]
+        m3 += list(img[lo:hi]) + [r[2] for r in rows if r[0] == a]
+        o3.append(len(i3))
+    assert code(np.array(o3), np.array(i3), np.array(m3)) == -1
+    # an atom with itself in its own cell
+    o4 = off.copy()
+    o4[1:] += 1
+    assert code(o4, np.concatenate([[0], idx]), np.concatenate([[[0, 0, 0]], img])) == -1
+    # a partner in the other structure
+    i5 = idx.copy()
+    i5[0] = b.ntotal - 1
+    assert code(off, i5, img) == -1
+    # the context is usable again after the errors
+    eng.set_neighbors(off, idx, img, model.rc)
+    E, F, V = eng.energy_forces()
+    Eo, Fo, Vo = orc.compute_batch(b)
+    assert float(np.max(np.abs(E.cpu().numpy() - Eo) / b.natoms)) <= 1e-6
