@@ -66,12 +66,13 @@ def build_shim() -> str:
     os.replace(tmp_lib, SHIM)
     # the C++ test driver of the drop-in classes (tests/cpp)
     tdir = os.path.join(ROOT, "tests", "cpp")
-    tsrc = os.path.join(tdir, "test_nnp_gpu.cpp")
-    if os.path.exists(tsrc):
-        _run(["g++", "-std=c++17", "-O2", "-D__HIP_PLATFORM_AMD__", "-I" + CPP, "-I" + os.path.join(ROOT, "include"),
-              "-I/opt/rocm/include", "-o", os.path.join(tdir, "test_nnp_gpu"), tsrc,
-              "-L" + HERE, "-Wl,-rpath," + HERE, "-Wl,-rpath,$ORIGIN/../../ann_amd", "-lann_nnp", "-lann_amd",
-              "-L/opt/rocm/lib", "-lamdhip64"])
+    for name in ("test_nnp_gpu", "test_classes_gpu"):
+        tsrc = os.path.join(tdir, name + ".cpp")
+        if os.path.exists(tsrc):
+            _run(["g++", "-std=c++17", "-O2", "-D__HIP_PLATFORM_AMD__", "-I" + CPP, "-I" + os.path.join(ROOT, "include"),
+                  "-I/opt/rocm/include", "-o", os.path.join(tdir, name), tsrc,
+                  "-L" + HERE, "-Wl,-rpath," + HERE, "-Wl,-rpath,$ORIGIN/../../ann_amd", "-lann_nnp", "-lann_amd",
+                  "-L/opt/rocm/lib", "-lamdhip64"])
     return SHIM
 
 

@@ -1,4 +1,4 @@
-// net_dev.h -- FP32 activation functions with first and second derivatives
+// net_dev.h -- FP32 and FP64 activation functions with first and second derivatives
 // (AFFPBP / AFFPBP2, src/ml/nn.cpp:149-780, every neuron of NN::Neuron;
 // softplus and logish keep the reference's truncated ln2, src/math/const.hpp:15).
 // Values and first derivatives follow the reference, including the neurons
@@ -18,146 +18,168 @@ enum { NEU_LINEAR = 1, NEU_SIGMOID = 2, NEU_TANH = 3, NEU_ISRU = 4, NEU_ARCTAN =
 // every neuron of NN::Neuron (src/ml/nn.hpp:83-107) except UNKNOWN and TEST
 __host__ inline bool neuron_supported(int n) { return n >= NEU_LINEAR && n <= NEU_ATISH; }
 
+// the math of one precision: FP32 uses the fast device intrinsics of the
+// network kernels, FP64 the full-precision library functions
+__device__ __forceinline__ float mexp(float x) { return __expf(x); }
+__device__ __forceinline__ double mexp(double x) { return exp(x); }
+__device__ __forceinline__ float mtanh(float x) { return tanhf(x); }
+__device__ __forceinline__ double mtanh(double x) { return tanh(x); }
+__device__ __forceinline__ float matan(float x) { return atanf(x); }
+__device__ __forceinline__ double matan(double x) { return atan(x); }
+__device__ __forceinline__ float mlog1p(float x) { return log1pf(x); }
+__device__ __forceinline__ double mlog1p(double x) { return log1p(x); }
+__device__ __forceinline__ float msqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double msqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float mrsqrt(float x) { return rsqrtf(x); }
+__device__ __forceinline__ double mrsqrt(double x) { return 1.0 / sqrt(x); }
+__device__ __forceinline__ float merf(float x) { return erff(x); }
+__device__ __forceinline__ double merf(double x) { return erf(x); }
+__device__ __forceinline__ float mabs(float x) { return fabsf(x); }
+__device__ __forceinline__ double mabs(double x) { return fabs(x); }
+__device__ __forceinline__ float mmax(float x, float y) { return fmaxf(x, y); }
+__device__ __forceinline__ double mmax(double x, double y) { return fmax(x, y); }
+
 // logistic sigmoid without overflow
-__device__ __forceinline__ float sigm(float u) {
-    const float e = __expf(-fabsf(u)), f = 1.0f / (1.0f + e);
-    return u >= 0.0f ? f : e * f;
+template <typename T>
+__device__ __forceinline__ T sigm(T u) {
+    const T e = mexp(-mabs(u)), f = T(1.0) / (T(1.0) + e);
+    return u >= T(0.0) ? f : e * f;
 }
 
-template <bool D2>
-__device__ __forceinline__ void act_f(int neuron, float c, float z, float& a, float& d, float& d2) {
+template <bool D2, typename T>
+__device__ __forceinline__ void act_f(int neuron, T c, T z, T& a, T& d, T& d2) {
     switch (neuron) {
         case NEU_SIGMOID: {
-            const float e = __expf(-c * fabsf(z)), f = 1.0f / (1.0f + e);
-            a = z >= 0.0f ? f : e * f;
+            const T e = mexp(-c * mabs(z)), f = T(1.0) / (T(1.0) + e);
+            a = z >= T(0.0) ? f : e * f;
             d = c * e * f * f;
-            if (D2) d2 = (z >= 0.0f ? -1.0f : 1.0f) * c * c * e * (1.0f - e) * f * f * f;
+            if (D2) d2 = (z >= T(0.0) ? -T(1.0) : T(1.0)) * c * c * e * (T(1.0) - e) * f * f * f;
         } break;
         case NEU_TANH: {
-            const float t = tanhf(c * z);
+            const T t = mtanh(c * z);
             a = t;
-            d = c * (1.0f - t * t);
-            if (D2) d2 = -2.0f * c * c * t * (1.0f - t * t);
+            d = c * (T(1.0) - t * t);
+            if (D2) d2 = -T(2.0) * c * c * t * (T(1.0) - t * t);
         } break;
         case NEU_ARCTAN: {
-            const float zz = c * z * 1.5707963267948966f, f = 1.0f / (1.0f + zz * zz);
-            a = 0.6366197723675814f * atanf(zz);
+            const T zz = c * z * T(1.5707963267948966), f = T(1.0) / (T(1.0) + zz * zz);
+            a = T(0.6366197723675814) * matan(zz);
             d = c * f;
-            if (D2) d2 = -2.0f * c * zz * f * f * (c * 1.5707963267948966f);  // true d2 (see oracle)
+            if (D2) d2 = -T(2.0) * c * zz * f * f * (c * T(1.5707963267948966));  // true d2 (see oracle)
         } break;
         case NEU_RELU:
-            a = z > 0.0f ? z : 0.0f;
-            d = z > 0.0f ? 1.0f : 0.0f;
-            if (D2) d2 = 0.0f;
+            a = z > T(0.0) ? z : T(0.0);
+            d = z > T(0.0) ? T(1.0) : T(0.0);
+            if (D2) d2 = T(0.0);
             break;
         case NEU_ELU:
-            if (z >= 0.0f) { a = z; d = 1.0f; if (D2) d2 = 0.0f; }
-            else { const float e = __expf(c * z); a = e - 1.0f; d = c * e; if (D2) d2 = c * c * e; }
+            if (z >= T(0.0)) { a = z; d = T(1.0); if (D2) d2 = T(0.0); }
+            else { const T e = mexp(c * z); a = e - T(1.0); d = c * e; if (D2) d2 = c * c * e; }
             break;
         case NEU_SWISH: {
-            const float zz = c * z;
-            const float e = __expf(-fabsf(zz)), den = 1.0f / (1.0f + e);
-            if (zz > 0.0f) {
+            const T zz = c * z;
+            const T e = mexp(-mabs(zz)), den = T(1.0) / (T(1.0) + e);
+            if (zz > T(0.0)) {
                 a = z * den;
-                d = (1.0f + e * (1.0f + zz)) * den * den;
-                if (D2) d2 = c * e * (2.0f - zz + e * (2.0f + zz)) * den * den * den;
+                d = (T(1.0) + e * (T(1.0) + zz)) * den * den;
+                if (D2) d2 = c * e * (T(2.0) - zz + e * (T(2.0) + zz)) * den * den * den;
             } else {
                 a = z * e * den;
-                d = e * (1.0f + zz + e) * den * den;
-                if (D2) d2 = c * e * (2.0f + zz + e * (2.0f - zz)) * den * den * den;
+                d = e * (T(1.0) + zz + e) * den * den;
+                if (D2) d2 = c * e * (T(2.0) + zz + e * (T(2.0) - zz)) * den * den * den;
             }
         } break;
         case NEU_SOFTPLUS: {
-            const float ln2 = 0.693147180559945f;
-            const float e = __expf(-c * fabsf(z)), den = 1.0f / (e + 1.0f);
-            a = (z >= 0.0f ? z : 0.0f) + (log1pf(e) - ln2) / c;
-            d = z >= 0.0f ? den : e * den;
+            const T ln2 = T(0.693147180559945);
+            const T e = mexp(-c * mabs(z)), den = T(1.0) / (e + T(1.0));
+            a = (z >= T(0.0) ? z : T(0.0)) + (mlog1p(e) - ln2) / c;
+            d = z >= T(0.0) ? den : e * den;
             if (D2) d2 = c * e * den * den;
         } break;
         case NEU_SQPLUS: {
-            const float zz = c * z, s = sqrtf(1.0f + zz * zz);
-            a = 0.5f / c * (zz - 1.0f + s);
-            d = 0.5f * (zz / s + 1.0f);
-            if (D2) d2 = 0.5f * c / (s * s * s);
+            const T zz = c * z, s = msqrt(T(1.0) + zz * zz);
+            a = T(0.5) / c * (zz - T(1.0) + s);
+            d = T(0.5) * (zz / s + T(1.0));
+            if (D2) d2 = T(0.5) * c / (s * s * s);
         } break;
         case NEU_ISRU: {
-            const float zz = c * z, isr = rsqrtf(1.0f + zz * zz), isr3 = isr * isr * isr;
+            const T zz = c * z, isr = mrsqrt(T(1.0) + zz * zz), isr3 = isr * isr * isr;
             a = zz * isr;
             d = c * isr3;
-            if (D2) d2 = -3.0f * c * c * zz * isr3 * isr * isr;
+            if (D2) d2 = -T(3.0) * c * c * zz * isr3 * isr * isr;
         } break;
         case NEU_TANHRE:
-            if (z >= 0.0f) { a = z; d = 1.0f; if (D2) d2 = 0.0f; }
+            if (z >= T(0.0)) { a = z; d = T(1.0); if (D2) d2 = T(0.0); }
             else {
-                const float t = tanhf(c * z);
+                const T t = mtanh(c * z);
                 a = t;
-                d = c * (1.0f - t * t);
-                if (D2) d2 = -2.0f * c * t * d;
+                d = c * (T(1.0) - t * t);
+                if (D2) d2 = -T(2.0) * c * t * d;
             }
             break;
         case NEU_SQRE:
-            if (z >= 0.0f) { a = z; d = 1.0f; if (D2) d2 = 0.0f; }
+            if (z >= T(0.0)) { a = z; d = T(1.0); if (D2) d2 = T(0.0); }
             else {
-                const float den = rsqrtf(1.0f + z * z), den3 = den * den * den;
+                const T den = mrsqrt(T(1.0) + z * z), den3 = den * den * den;
                 a = z * den;
                 d = den3;
-                if (D2) d2 = -3.0f * z * den3 * den * den;
+                if (D2) d2 = -T(3.0) * z * den3 * den * den;
             }
             break;
         case NEU_GELU: {   // 0.5 z (1 + erf(sqrt(pi)/2 c z))
-            const float zz = 0.886226925452758f * c * z, g = __expf(-zz * zz), ep1 = 1.0f + erff(zz);
-            a = 0.5f * z * ep1;
-            d = 0.5f * (ep1 + c * z * g);
-            if (D2) d2 = c * g * (1.0f - zz * zz);
+            const T zz = T(0.886226925452758) * c * z, g = mexp(-zz * zz), ep1 = T(1.0) + merf(zz);
+            a = T(0.5) * z * ep1;
+            d = T(0.5) * (ep1 + c * z * g);
+            if (D2) d2 = c * g * (T(1.0) - zz * zz);
         } break;
         case NEU_MISH: {   // z tanh(softplus(c z))
-            const float zz = c * z;
-            float f;
-            if (zz > 0.0f) {
-                const float e = __expf(-zz), e2 = e * e;
-                f = 1.0f - 2.0f * e2 / (1.0f + 2.0f * (e + e2));
+            const T zz = c * z;
+            T f;
+            if (zz > T(0.0)) {
+                const T e = mexp(-zz), e2 = e * e;
+                f = T(1.0) - T(2.0) * e2 / (T(1.0) + T(2.0) * (e + e2));
             } else {
-                const float e = __expf(zz), ff = e * (e + 2.0f);
-                f = ff / (ff + 2.0f);
+                const T e = mexp(zz), ff = e * (e + T(2.0));
+                f = ff / (ff + T(2.0));
             }
-            const float s = sigm(zz), fp = (1.0f - f * f) * s;
+            const T s = sigm(zz), fp = (T(1.0) - f * f) * s;
             a = z * f;
             d = f + zz * fp;
-            if (D2) d2 = 2.0f * c * fp + c * zz * fp * ((1.0f - s) - 2.0f * f * s);
+            if (D2) d2 = T(2.0) * c * fp + c * zz * fp * ((T(1.0) - s) - T(2.0) * f * s);
         } break;
         case NEU_PFLU: {
-            const float zz = c * z, z2 = zz * zz, q = rsqrtf(z2 + 1.0f), q3 = q * q * q;
-            a = 0.5f * z * (1.0f + zz * q);
-            d = 0.5f * (zz * (z2 + 2.0f) * q3 + 1.0f);
-            if (D2) d2 = -0.5f * c * (z2 - 2.0f) * q3 * q * q;
+            const T zz = c * z, z2 = zz * zz, q = mrsqrt(z2 + T(1.0)), q3 = q * q * q;
+            a = T(0.5) * z * (T(1.0) + zz * q);
+            d = T(0.5) * (zz * (z2 + T(2.0)) * q3 + T(1.0));
+            if (D2) d2 = -T(0.5) * c * (z2 - T(2.0)) * q3 * q * q;
         } break;
         case NEU_SERF: {   // z erf(softplus(z)) (no sharpness)
-            const float sp = fmaxf(z, 0.0f) + log1pf(__expf(-fabsf(z))), s = sigm(z);
-            const float g = 1.1283791670955126f * __expf(-sp * sp) * s, ef = erff(sp);
+            const T sp = mmax(z, T(0.0)) + mlog1p(mexp(-mabs(z))), s = sigm(z);
+            const T g = T(1.1283791670955126) * mexp(-sp * sp) * s, ef = merf(sp);
             a = z * ef;
             d = ef + z * g;
-            if (D2) d2 = 2.0f * g + z * g * ((1.0f - s) - 2.0f * sp * s);
+            if (D2) d2 = T(2.0) * g + z * g * ((T(1.0) - s) - T(2.0) * sp * s);
         } break;
         case NEU_LOGISH: {  // z log(1 + sigmoid(z)) / ln2 (no sharpness; the reference's truncated ln2)
-            const float li = 1.0f / 0.693147180559945f, s = sigm(z), L = log1pf(s);
-            const float h = s * (1.0f - s) / (1.0f + s);
+            const T li = T(1.0) / T(0.693147180559945), s = sigm(z), L = mlog1p(s);
+            const T h = s * (T(1.0) - s) / (T(1.0) + s);
             a = z * L * li;
             d = (L + z * h) * li;
             if (D2) {
-                const float hp = s * (1.0f - s) * ((1.0f - 2.0f * s) / (1.0f + s) - h / (1.0f + s));
-                d2 = (2.0f * h + z * hp) * li;
+                const T hp = s * (T(1.0) - s) * ((T(1.0) - T(2.0) * s) / (T(1.0) + s) - h / (T(1.0) + s));
+                d2 = (T(2.0) * h + z * hp) * li;
             }
         } break;
         case NEU_ATISH: {   // z (1 + 2/pi atan(pi z / 2)) / 2 (no sharpness)
-            const float zz = 1.5707963267948966f * z, q = 1.0f / (1.0f + zz * zz), at = atanf(zz);
-            a = z * 0.5f * (1.0f + 0.6366197723675814f * at);
-            d = 0.5f * (1.0f + z * q) + 0.3183098861837907f * at;
+            const T zz = T(1.5707963267948966) * z, q = T(1.0) / (T(1.0) + zz * zz), at = matan(zz);
+            a = z * T(0.5) * (T(1.0) + T(0.6366197723675814) * at);
+            d = T(0.5) * (T(1.0) + z * q) + T(0.3183098861837907) * at;
             if (D2) d2 = q * q;
         } break;
         default:  // linear
             a = z;
-            d = 1.0f;
-            if (D2) d2 = 0.0f;
+            d = T(1.0);
+            if (D2) d2 = T(0.0);
             break;
     }
 }

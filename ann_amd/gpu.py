@@ -58,7 +58,9 @@ EXPORTS = ["ann_last_error", "ann_create", "ann_destroy", "ann_set_stream", "ann
            "ann_autotune", "ann_get_tuning", "ann_set_tuning", "ann_input_stats",
            "ann_get_optimizer_state", "ann_set_optimizer_state", "ann_symm_fwd64", "ann_net_fwd64",
            "ann_set_precise_ratio", "ann_precise_count", "ann_force_reduce_noalloc", "ann_set_loss",
-           "ann_set_struc_weights", "ann_set_neighbors"]
+           "ann_set_struc_weights", "ann_set_neighbors", "ann_cutoff_eval", "ann_radial_eval",
+           "ann_angular_eval", "ann_mlp_create", "ann_mlp_set_params", "ann_mlp_set_scaling", "ann_mlp_size",
+           "ann_mlp_eval", "ann_mlp_grad", "ann_mlp_destroy"]
 
 
 def lib():
@@ -394,3 +396,104 @@ class Engine:
         ms, n = C.c_double(), C.c_int()
         _check(lib().ann_profile_get(self.h, name.encode(), C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+
+# ---------------------------------------------------------------- FP64 evaluators
+class RadialDesc(C.Structure):
+    _fields_ = [("cutoff", C.c_int), ("rc", C.c_double), ("name", C.c_int), ("n", C.c_int),
+                ("rs", P_D), ("eta", P_D)]
+
+
+class AngularDesc(C.Structure):
+    _fields_ = [("cutoff", C.c_int), ("rc", C.c_double), ("name", C.c_int), ("n", C.c_int),
+                ("eta", P_D), ("zeta", P_D), ("lam", P_I)]
+
+
+class MlpDesc(C.Structure):
+    _fields_ = [("nin", C.c_int), ("nlayer", C.c_int), ("nodes", P_I), ("neuron", C.c_int), ("c", C.c_double),
+                ("params", P_D), ("inpw", P_D), ("inpb", P_D), ("outw", C.c_double), ("outb", C.c_double)]
+
+
+def _f64(x, device=None):
+    t = torch.as_tensor(np.ascontiguousarray(x, dtype=np.float64))
+    return t.to(device) if device is not None else t
+
+
+def cutoff_eval(cutoff: int, rc: float, r: torch.Tensor):
+    """Cutoff::compute over r (FP64 device tensor): (f, g)."""
+    f, g = torch.empty_like(r), torch.empty_like(r)
+    _check(lib().ann_cutoff_eval(int(cutoff), C.c_double(rc), r.numel(), _ptr(r), _ptr(f), _ptr(g)))
+    return f, g
+
+
+def radial_eval(cutoff: int, rc: float, name: int, rs, eta, r: torch.Tensor, dEdG: Optional[torch.Tensor] = None):
+    """BasisR::symm / symmd / force over distances r: (G, dG/dr, amp)."""
+    rs = np.ascontiguousarray(rs, dtype=np.float64)
+    eta = np.ascontiguousarray(eta, dtype=np.float64)
+    d = RadialDesc(int(cutoff), float(rc), int(name), len(rs), rs.ctypes.data_as(P_D), eta.ctypes.data_as(P_D))
+    n = r.numel()
+    G = torch.empty((n, len(rs)), dtype=torch.float64, device=r.device)
+    dG = torch.empty_like(G)
+    amp = torch.empty(n, dtype=torch.float64, device=r.device) if dEdG is not None else None
+    _check(lib().ann_radial_eval(C.byref(d), n, _ptr(r), _ptr(G), _ptr(dG), _ptr(dEdG), _ptr(amp)))
+    return G, dG, amp
+
+
+def angular_eval(cutoff: int, rc: float, name: int, eta, zeta, lam, r: torch.Tensor, cos: torch.Tensor,
+                 dEdG: Optional[torch.Tensor] = None):
+    """BasisA::symm / symmd / force over triplets (r [n][2], cos [n]):
+    (G, dG [n][nf][3], phi, eta [n][2])."""
+    eta = np.ascontiguousarray(eta, dtype=np.float64)
+    zeta = np.ascontiguousarray(zeta, dtype=np.float64)
+    lam = np.ascontiguousarray(lam, dtype=np.int32)
+    d = AngularDesc(int(cutoff), float(rc), int(name), len(eta), eta.ctypes.data_as(P_D),
+                    zeta.ctypes.data_as(P_D), lam.ctypes.data_as(P_I))
+    n = cos.numel()
+    G = torch.empty((n, len(eta)), dtype=torch.float64, device=cos.device)
+    dG = torch.empty((n, len(eta), 3), dtype=torch.float64, device=cos.device)
+    phi = torch.empty(n, dtype=torch.float64, device=cos.device) if dEdG is not None else None
+    et = torch.empty((n, 2), dtype=torch.float64, device=cos.device) if dEdG is not None else None
+    _check(lib().ann_angular_eval(C.byref(d), n, _ptr(r), _ptr(cos), _ptr(G), _ptr(dG), _ptr(dEdG), _ptr(phi),
+                                  _ptr(et)))
+    return G, dG, phi, et
+
+
+class Mlp:
+    """One network on the FP64 evaluator (ann_mlp_*): NN::ANN fp / fpbp,
+    DODZ::dodi, Cost::grad and D2ODZDI products."""
+
+    def __init__(self, net, device: str = "cuda:0"):
+        self.device = torch.device(device)
+        torch.cuda.set_device(self.device)
+        nodes = np.ascontiguousarray(net.nodes, dtype=np.int32)
+        par = np.ascontiguousarray(net.params, dtype=np.float64)
+        iw = np.ascontiguousarray(net.inpw, dtype=np.float64)
+        ib = np.ascontiguousarray(net.inpb, dtype=np.float64)
+        d = MlpDesc(int(net.nin), len(nodes), nodes.ctypes.data_as(P_I), int(net.neuron), float(net.c),
+                    par.ctypes.data_as(P_D), iw.ctypes.data_as(P_D), ib.ctypes.data_as(P_D), float(net.outw),
+                    float(net.outb))
+        self.h = C.c_void_p()
+        _check(lib().ann_mlp_create(C.byref(d), C.byref(self.h)))
+        self.nin = int(net.nin)
+        self.nparam = int(lib().ann_mlp_size(self.h))
+
+    def __del__(self):
+        try:
+            lib().ann_mlp_destroy(self.h)
+        except Exception:
+            pass
+
+    def eval(self, x: torch.Tensor):
+        n = x.shape[0]
+        out = torch.empty(n, dtype=torch.float64, device=self.device)
+        dodi = torch.empty((n, self.nin), dtype=torch.float64, device=self.device)
+        _check(lib().ann_mlp_eval(self.h, n, _ptr(x), _ptr(out), _ptr(dodi)))
+        return out, dodi
+
+    def grad(self, x: torch.Tensor, dcdo: Optional[torch.Tensor] = None, v: Optional[torch.Tensor] = None,
+             per_row: bool = False):
+        n = x.shape[0]
+        shape = (n, self.nparam) if per_row else (self.nparam,)
+        g = torch.empty(shape, dtype=torch.float64, device=self.device)
+        _check(lib().ann_mlp_grad(self.h, n, _ptr(x), _ptr(dcdo), _ptr(v), int(per_row), _ptr(g)))
+        return g

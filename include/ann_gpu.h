@@ -297,6 +297,70 @@ int ann_profile_get(ann_ctx* ctx, const char* name, double* total_ms, int* count
  * timed region carries two event records per step instead of two per kernel */
 int ann_profile_only(ann_ctx* ctx, const char* name);
 
+/* ---- FP64 evaluators of the per-function classes ----
+ * No context.  npts points per call, device arrays, FP64 throughout,
+ * synchronous; output pointers may be null where not wanted.  Names of
+ * cutoffs and families as in ann_model_desc. */
+
+/* Cutoff::cutf / cutg / compute (src/nnp/cutoff.hpp:59-61,
+ * src/nnp/cutoff.cpp:62-143): f[p], g[p] = cutoff value and d/dr at r[p]. */
+int ann_cutoff_eval(int cutoff, double rc, int npts, const double* d_r, double* d_f, double* d_g);
+
+/* One radial basis (BasisR: name, rs, eta; src/nnp/basis_radial.hpp:40-88),
+ * at most 64 functions; parameter arrays on the HOST. */
+typedef struct ann_radial_desc {
+    int cutoff; double rc; int name; int n;
+    const double* rs; const double* eta;
+} ann_radial_desc;
+/* BasisR::symm (src/nnp/basis_radial.cpp:263-309) and BasisR::force
+ * (:316-376) per distance r[p]: G [npts][n], dG/dr [npts][n], and
+ * amp[p] = -sum_i dEdG[p][i] dG_i/dr (needs dEdG). */
+int ann_radial_eval(const ann_radial_desc* basis, int npts, const double* d_r, double* d_G, double* d_dGdr,
+                    const double* d_dEdG, double* d_amp);
+
+/* One angular basis (BasisA: name, eta, zeta, lambda;
+ * src/nnp/basis_angular.hpp:40-102), at most 64 functions; HOST arrays. */
+typedef struct ann_angular_desc {
+    int cutoff; double rc; int name; int n;
+    const double* eta; const double* zeta; const int* lambda;
+} ann_angular_desc;
+/* BasisA::symm (src/nnp/basis_angular.cpp:371-420), symmd (:259-364) and
+ * force (:430-563) per triplet p with legs r[p][0] = r_ij, r[p][1] = r_ik and
+ * cosine cos[p]: G [npts][n]; dG [npts][n][3] = (d/dcos, d/dr_ij, d/dr_ik);
+ * phi[p] = -sum_i dEdG dG_i/dcos and eta[p][2] = -sum_i dEdG dG_i/dr (need dEdG). */
+int ann_angular_eval(const ann_angular_desc* basis, int npts, const double* d_r, const double* d_cos, double* d_G,
+                     double* d_dG, const double* d_dEdG, double* d_phi, double* d_eta);
+
+/* One network (NN::ANN, src/ml/nn.hpp:244-370): nin inputs, nlayer layers of
+ * nodes[l] neurons (the last of size 1, linear), one neuron type with
+ * sharpness c, parameters in the reference packing (all biases, then each
+ * weight matrix column-major), input scaling inpw (G + inpb) (null: 1 and 0),
+ * output scaling outw a + outb.  HOST arrays, copied at creation.  Node state
+ * lives in 64 KiB of LDS per row: 8 (2 nin + 5 sum(nodes) + 6 max width) bytes. */
+typedef struct ann_mlp_desc {
+    int nin; int nlayer; const int* nodes; int neuron; double c;
+    const double* params; const double* inpw; const double* inpb;
+    double outw; double outb;
+} ann_mlp_desc;
+typedef struct ann_mlp ann_mlp;
+int ann_mlp_create(const ann_mlp_desc* desc, ann_mlp** out);
+/* operator<< of NN::ANN (src/ml/nn.cpp:867-885): n must equal ann_mlp_size */
+int ann_mlp_set_params(ann_mlp* mlp, const double* params, int n);
+int ann_mlp_set_scaling(ann_mlp* mlp, const double* inpw, const double* inpb, double outw, double outb);
+int ann_mlp_size(const ann_mlp* mlp);
+/* ANN::fp / fpbp (src/ml/nn.cpp:1343-1389) per row: out[p], and DODZ::dodi
+ * (do/di with the input scaling, src/ml/nn.cpp:1801-1810) in dodi [npts][nin]. */
+int ann_mlp_eval(ann_mlp* mlp, int npts, const double* d_in, double* d_out, double* d_dodi);
+/* Gradient w.r.t. the parameters of Phi_p = dcdo[p] o_p + sum_k v[p][k] do_p/di_k
+ * (dcdo null: 0; v null: 0).  v = 0 is Cost::grad (src/ml/nn.cpp:1712-1762);
+ * v = e_k with dcdo = 0 is column k of D2ODZDI::d2odpdi (:2012-2100), here the
+ * exact second derivative (the reference's explicit first-layer weight term
+ * omits the inpw factor, :2060-2066; the two agree when inpw = 1).
+ * per_row: grad [npts][nparam]; else grad [nparam] = sum over rows, in row order. */
+int ann_mlp_grad(ann_mlp* mlp, int npts, const double* d_in, const double* d_dcdo, const double* d_v, int per_row,
+                 double* d_grad);
+void ann_mlp_destroy(ann_mlp* mlp);
+
 #ifdef __cplusplus
 }
 #endif
