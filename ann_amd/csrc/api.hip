@@ -457,6 +457,7 @@ extern "C" int ann_create(const ann_model_desc* desc, ann_ctx** out) {
 extern "C" int ann_destroy(ann_ctx* ctx) {
     if (!ctx) return ANN_OK;
     hipDeviceSynchronize();
+    pair_free(ctx);
     void* ptrs[] = {ctx->dm, ctx->da.rad_rs, ctx->da.rad_eta, ctx->da.ang_eta, ctx->da.ang_zeta, ctx->da.ang_lam,
                     ctx->da.W, ctx->da.inpw, ctx->da.inpb, ctx->da.params, ctx->da.adam_m, ctx->da.adam_v,
                     ctx->d_struc_of, ctx->d_cellinfo, ctx->d_cellmat, ctx->d_bin_of, ctx->d_shift, ctx->d_bin_cnt,

@@ -149,6 +149,8 @@ struct Timer {
     int count = 0;
 };
 
+struct PairState;
+
 }  // namespace ann
 
 struct ann_ctx {
@@ -171,6 +173,7 @@ struct ann_ctx {
     double loss_delta = 1.0;
     const double* d_wEs = nullptr;    // per-structure energy / force weights (caller's device arrays) or null
     const double* d_wFs = nullptr;
+    ann::PairState* pair = nullptr;   // device copy of the last ann_pair_compute domain (pair.hip)
     // device model
     ann::DevModel* dm = nullptr;
     ann::DevArrays da;
@@ -348,6 +351,7 @@ hipError_t force_reduce_noalloc(ann_ctx* ctx, const float* P, const float* Eatom
 hipError_t refresh_geometry(ann_ctx* ctx);
 // the caller's host CSR list (off, idx, img [P][3]) for the current batch (ANN status)
 int set_neighbors_host(ann_ctx* ctx, const int* off, const int* idx, const int* img, double rc);
+void pair_free(ann_ctx* ctx);
 hipError_t wrap_positions(ann_ctx* ctx, double* pos);
 hipError_t md_kick_drift(ann_ctx* ctx, double dt, const double* mass, const double* F, double* vel, double* pos,
                          double lim2);

@@ -60,7 +60,7 @@ EXPORTS = ["ann_last_error", "ann_create", "ann_destroy", "ann_set_stream", "ann
            "ann_set_precise_ratio", "ann_precise_count", "ann_force_reduce_noalloc", "ann_set_loss",
            "ann_set_struc_weights", "ann_set_neighbors", "ann_cutoff_eval", "ann_radial_eval",
            "ann_angular_eval", "ann_mlp_create", "ann_mlp_set_params", "ann_mlp_set_scaling", "ann_mlp_size",
-           "ann_mlp_eval", "ann_mlp_grad", "ann_mlp_destroy"]
+           "ann_mlp_eval", "ann_mlp_grad", "ann_mlp_destroy", "ann_pair_compute"]
 
 
 def lib():
@@ -369,6 +369,32 @@ class Engine:
         _check(lib().ann_md_step(self.h, form, C.c_double(dt), _ptr(mass), _ptr(vel), _ptr(F), _ptr(E), _ptr(KE),
                                  _ptr(V), C.byref(moved) if check else None))
         return bool(moved.value)
+
+    # ------------------------------------------------------------ LAMMPS-style pair interface
+    def pair_compute(self, x, species, cell, owner, neigh, rc: float, nlocal: int, new_list: bool = True,
+                     form: int = FAST, per_atom: bool = True):
+        """ann_pair_compute: x [nall][3] (locals first, then ghosts), species
+        [nlocal], owner [nghost], neigh = list of nlocal integer arrays (the
+        full list, indices into [0, nall)).  Returns (E, F [nlocal][3],
+        eatom [nlocal], virial[6])."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        sp = np.ascontiguousarray(species, dtype=np.int32)
+        cell = np.ascontiguousarray(cell, dtype=np.float64)
+        own = np.ascontiguousarray(owner, dtype=np.int32)
+        nghost = len(x) - nlocal
+        rows = [np.ascontiguousarray(r, dtype=np.int32) for r in neigh] if neigh is not None else None
+        num = np.array([len(r) for r in rows], np.int32) if rows is not None else None
+        ptrs = (P_I * nlocal)(*[r.ctypes.data_as(P_I) for r in rows]) if rows is not None else None
+        f = np.zeros((nlocal, 3))
+        e = C.c_double(0.0)
+        ea = np.zeros(nlocal) if per_atom else None
+        vir = np.zeros(6)
+        _check(lib().ann_pair_compute(self.h, form, int(new_list), nlocal, nghost, x.ctypes.data_as(P_D),
+                                      sp.ctypes.data_as(P_I), cell.ctypes.data_as(P_D), own.ctypes.data_as(P_I),
+                                      num.ctypes.data_as(P_I) if num is not None else None, ptrs,
+                                      C.c_double(rc), f.ctypes.data_as(P_D), C.byref(e),
+                                      ea.ctypes.data_as(P_D) if ea is not None else None, vir.ctypes.data_as(P_D)))
+        return e.value, f, ea, vir
 
     # ------------------------------------------------------------ tuning
     def autotune(self, form: int = FAST) -> list:

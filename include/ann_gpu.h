@@ -297,6 +297,30 @@ int ann_profile_get(ann_ctx* ctx, const char* name, double* total_ms, int* count
  * timed region carries two event records per step instead of two per kernel */
 int ann_profile_only(ann_ctx* ctx, const char* name);
 
+/* ---- LAMMPS-style pair interface ----
+ * One periodic domain: nlocal owned atoms and nghost ghost copies (LAMMPS
+ * atom->nlocal / nghost), positions x [nlocal + nghost][3] and species
+ * [nlocal] on the HOST, the box as the cell (lattice vectors as columns),
+ * owner[g] = local index of the atom that ghost nlocal + g images (LAMMPS
+ * atom->map of its tag), and a FULL neighbour list of the local atoms,
+ * numneigh [nlocal] / firstneigh [nlocal] -> indices in [0, nlocal + nghost)
+ * (LAMMPS list->numneigh / firstneigh), with pairs up to rc (the neighbour
+ * cutoff, model cutoff + skin).  The reference names the pair style
+ * (README.md:411-416) without its source; semantics of its MD caller
+ * PotNNPE::compute (src/torch/pot_nnpe.cpp:166-287) with exact forces.
+ * Every ghost must be a lattice image of its owner (else ANN_ERR_ARG).  The
+ * forces on ghosts are folded back onto their owners: f[i] += F_i for the
+ * local atoms (ghost rows of f untouched, as after LAMMPS's reverse
+ * communication); *eng += E (vacuum energies included); eatom[i] += E_i and
+ * virial[6] += (xx, yy, zz, xy, xz, yz) of -dE/dstrain when not NULL.
+ * new_list = 0 reuses the list of the previous call (same atoms; LAMMPS
+ * neighbor->ago > 0) and moves only the local positions and the cell.
+ * Synchronous. */
+int ann_pair_compute(ann_ctx* ctx, int form, int new_list, int nlocal, int nghost, const double* x,
+                     const int* species, const double* cell, const int* owner, const int* numneigh,
+                     const int* const* firstneigh, double rc, double* f, double* eng, double* eatom,
+                     double* virial);
+
 /* ---- FP64 evaluators of the per-function classes ----
  * No context.  npts points per call, device arrays, FP64 throughout,
  * synchronous; output pointers may be null where not wanted.  Names of
