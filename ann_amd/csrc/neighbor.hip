@@ -14,7 +14,8 @@
 // candidate set of its search box (lanes over candidates, not over bins),
 // applies the exact FP64 inclusion test in the same operation order as the
 // oracle, compacts accepted keys (j << 30 | packed n) through a ballot into
-// LDS and sorts them with a bitonic network.  Integer work; the roofline is
+// LDS and sorts them (ranks in the staged search, a bitonic network in the
+// wide one).  Integer work; the roofline is
 // HBM/latency (positions in, 16 B + 13 B per pair out).
 #include <atomic>
 
@@ -207,85 +208,55 @@ __global__ __launch_bounds__(64) void k_search(
 // (every atom of the (2k+1)^3 bin images with its image offset) are copied
 // to LDS with one 32-byte load each; each wavefront then tests one centre
 // atom of the bin against them, compacts the accepted keys (j << 30 | packed
-// image) into its own LDS buffer and sorts them in registers with a bitonic
-// network over lane shuffles (no block barriers after the staging).  Bins
+// image) into its own LDS buffer and stores them sorted by rank (no block
+// barriers after the staging).  Bins
 // whose search box holds more than kStageCap candidates or kStageImages
 // images are flagged and left to k_search.
 constexpr int kStageCap = 1280;
 constexpr int kStageImages = 216;
-constexpr int kStageThreads = 256;
+constexpr int kStageThreads = 512;
+constexpr int kWaveKeys = 256;          // accepted keys per centre in the staged search (more: wide path)
+static_assert(kWaveKeys == 4 * kWave, "the staged sort covers up to four keys per lane");
 constexpr int kStageWaves = kStageThreads / kWave;
 constexpr int kStageIt = (kStageCap + kStageThreads - 1) / kStageThreads;
-constexpr int kShiftT = 3, kShiftW = 2 * kShiftT + 1, kShiftN = kShiftW * kShiftW * kShiftW;
 
-__device__ __forceinline__ void cswap_u64(uint64_t& a, uint64_t& b, bool asc) {
-    const uint64_t lo = a < b ? a : b, hi = a < b ? b : a;
-    a = asc ? lo : hi;
-    b = asc ? hi : lo;
-}
-
-// ascending bitonic sort of the 64 R keys held as v[r] = element lane + 64 r
+// Sorted store of n distinct keys by rank: every lane counts, for the up to R
+// keys it holds, the keys below them (broadcast LDS reads, no dependency
+// between iterations), and writes each key at its rank.
 template <int R>
-__device__ __forceinline__ void wave_sort_u64(uint64_t (&v)[R]) {
-    const int lane = threadIdx.x % kWave;
-    constexpr int n2 = kWave * R;
-#pragma unroll
-    for (int k = 2; k <= n2; k <<= 1) {
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            if (j < kWave) {
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const int e = lane + kWave * r;
-                    const uint64_t o = (uint64_t)__shfl_xor((long long)v[r], j);
-                    const bool asc = (e & k) == 0, lower = (lane & j) == 0;
-                    const uint64_t mn = v[r] < o ? v[r] : o, mx = v[r] < o ? o : v[r];
-                    v[r] = (asc == lower) ? mn : mx;
-                }
-            } else {
-                const int rj = j / kWave;
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    if ((r & rj) == 0) {
-                        const int e = lane + kWave * r;
-                        cswap_u64(v[r], v[r | rj], (e & k) == 0);
-                    }
-                }
-            }
-        }
-    }
-}
-
-template <int R>
-__device__ __forceinline__ void sort_and_store(const uint64_t* buf, int n, uint64_t* dst) {
+__device__ __forceinline__ void rank_and_store(const uint64_t* buf, int n, uint64_t* dst) {
     const int lane = threadIdx.x % kWave;
     uint64_t v[R];
+    int rk[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         const int e = lane + kWave * r;
         v[r] = e < n ? buf[e] : ~0ull;
+        rk[r] = 0;
     }
-    wave_sort_u64<R>(v);
+#pragma unroll 4
+    for (int m = 0; m < n; ++m) {
+        const uint64_t w = buf[m];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-        const int e = lane + kWave * r;
-        if (e < n) dst[e] = v[r];
+        for (int r = 0; r < R; ++r) rk[r] += w < v[r] ? 1 : 0;
     }
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+        if (lane + kWave * r < n) dst[rk[r]] = v[r];
 }
 
 __global__ __launch_bounds__(kStageThreads) void k_search_bins(
     const CellInfo* __restrict__ ci, const double* __restrict__ cells, const int* __restrict__ bin_struc,
-    const int* __restrict__ bin_start, const double4* __restrict__ rec, double rc2,
+    const int* __restrict__ bin_start, const double4* __restrict__ rec, const double* __restrict__ pos, double rc2,
     uint64_t* __restrict__ keys_out, int* __restrict__ cnt_out, int* __restrict__ flag, int* __restrict__ bin_ovf) {
     __shared__ int im_first[kStageImages];
     __shared__ int im_m[kStageImages];
     __shared__ int im_pre[kStageImages + 1];
-    __shared__ double cx[kStageCap], cy[kStageCap], cz[kStageCap];
+    __shared__ float4 cf[kStageCap];                        // FP32 image position x_j + R (m - shift_j)
     __shared__ uint64_t cjd[kStageCap];                     // j << 32 | packed (image - shift_j)
-    __shared__ uint64_t wkeys[kStageWaves][kMaxNeigh];
+    __shared__ uint64_t wkeys[kStageWaves][kWaveKeys];
     __shared__ int cen_bin[8], cen_pre[9], cen_img[8];
-    __shared__ double shv[3][kShiftN];                      // image shift vectors R^T n, |n_c| <= kShiftT
-    __shared__ int s_total, s_max;
+    __shared__ int s_total, s_max, s_xmax;
     const int g = xcd_block(blockIdx.x, gridDim.x), t = threadIdx.x;
     const int lane = t % kWave, wave = t / kWave;
     const int s = bin_struc[g];                             // structure of this super-bin
@@ -308,15 +279,8 @@ __global__ __launch_bounds__(kStageThreads) void k_search_bins(
                     acc += bin_start[bg + 1] - bin_start[bg];
                 }
         cen_pre[q] = acc;
-    }
-    for (int q = t; q < kShiftN; q += kStageThreads) {
-        // the same FP64 expression as pair_vec_v, so a table entry is bitwise its value
-#pragma clang fp contract(off)
-        const double m0 = q / (kShiftW * kShiftW) - kShiftT, m1 = (q / kShiftW) % kShiftW - kShiftT,
-                     m2 = q % kShiftW - kShiftT;
-        shv[0][q] = __dadd_rn(__dadd_rn(__dmul_rn(R[0], m0), __dmul_rn(R[1], m1)), __dmul_rn(R[2], m2));
-        shv[1][q] = __dadd_rn(__dadd_rn(__dmul_rn(R[3], m0), __dmul_rn(R[4], m1)), __dmul_rn(R[5], m2));
-        shv[2][q] = __dadd_rn(__dadd_rn(__dmul_rn(R[6], m0), __dmul_rn(R[7], m1)), __dmul_rn(R[8], m2));
+        s_max = 0;
+        s_xmax = 0;
     }
     const int ncb = n0b * n1b * n2b;
     const int e0 = n0b + 2 * c.k[0], e1 = n1b + 2 * c.k[1], e2 = n2b + 2 * c.k[2];
@@ -384,6 +348,7 @@ __global__ __launch_bounds__(kStageThreads) void k_search_bins(
 #pragma unroll
         for (int r = 0; r < kStageIt; ++r)
             if (src[r] >= 0) r4[r] = rec[src[r]];
+        float xm = 0.0f;
 #pragma unroll
         for (int r = 0; r < kStageIt; ++r) {
             const int q = t + r * kStageThreads;
@@ -392,22 +357,35 @@ __global__ __launch_bounds__(kStageThreads) void k_search_bins(
             int m0, m1, m2, s0, s1, s2;
             unpack_img((uint32_t)mq[r], m0, m1, m2);
             unpack_img((uint32_t)(js & 0xffffffffu), s0, s1, s2);
-            cx[q] = r4[r].x;
-            cy[q] = r4[r].y;
-            cz[q] = r4[r].z;
+            const double d0 = m0 - s0, d1 = m1 - s1, d2 = m2 - s2;
+            const float x = (float)(r4[r].x + (R[0] * d0 + R[1] * d1 + R[2] * d2));
+            const float y = (float)(r4[r].y + (R[3] * d0 + R[4] * d1 + R[5] * d2));
+            const float z = (float)(r4[r].z + (R[6] * d0 + R[7] * d1 + R[8] * d2));
+            cf[q] = make_float4(x, y, z, 0.0f);
+            xm = fmaxf(xm, fmaxf(fabsf(x), fmaxf(fabsf(y), fabsf(z))));
             cjd[q] = (js & 0xffffffff00000000ull) | (uint64_t)pack_img(m0 - s0, m1 - s1, m2 - s2);
         }
+        atomicMax(&s_xmax, __float_as_int(xm));             // non-negative floats order as integers
     }
     __syncthreads();
+    // FP32 pre-test.  With u = 2^-24 and every FP32 coordinate within
+    // X = max |x| of its FP64 value to u X, the FP32 squared distance of a
+    // pair within 1.1 rc is within  e = 2 sqrt3 rc D + 3 D^2 + 4 u rc^2,
+    // D = 2 u X + 1.2 u rc, of the FP64 value.  Four times e is the band:
+    // below rc^2 - band a pair is in, above rc^2 + band out, and the exact
+    // FP64 test of the oracle decides inside the band and near zero.
+    const double rc = sqrt(rc2), uu = 5.9604644775390625e-08;
+    const double D = 2.0 * uu * (double)__int_as_float(s_xmax) + 1.2 * uu * rc;
+    const float band = (float)(4.0 * (3.4641016151377544 * rc * D + 3.0 * D * D + 4.0 * uu * rc2) + 2.0 * uu * rc2);
+    const float rc2f = (float)rc2, lo_thr = rc2f - band, hi_thr = rc2f + band;
     uint64_t* wk = wkeys[wave];
     int wmax = 0;          // largest neighbour count of this wavefront's centres
-    if (t == 0) s_max = 0;
     for (int cq = wave; cq < ncen; cq += kStageWaves) {
         int cb = 0;
         while (cb + 1 < ncb && cq >= cen_pre[cb + 1]) ++cb;
         // the centre is staged too: its own bin's zero image, where (image - shift) = -shift
         const int qc = im_pre[cen_img[cb]] + (cq - cen_pre[cb]);
-        const double rix = cx[qc], riy = cy[qc], riz = cz[qc];
+        const float4 xi = cf[qc];
         const uint64_t jsi = cjd[qc];
         const int i = (int)(jsi >> 32);
         int si0, si1, si2;
@@ -418,49 +396,52 @@ __global__ __launch_bounds__(kStageThreads) void k_search_bins(
         int count = 0;
         for (int base = 0; base < total; base += kWave) {
             const int q = base + lane;
-            bool ok = false;
-            uint64_t key = 0;
-            if (q < total) {
-                const uint64_t jd = cjd[q];
-                const int j = (int)(jd >> 32);
-                int d0, d1, d2i;
-                unpack_img((uint32_t)(jd & 0xffffffffu), d0, d1, d2i);
-                const int n0 = si0 + d0, n1 = si1 + d1, n2 = si2 + d2i;
-                double x, y, z, d2;
-                if (abs(n0) <= kShiftT && abs(n1) <= kShiftT && abs(n2) <= kShiftT) {
-#pragma clang fp contract(off)
-                    const int ti = ((n0 + kShiftT) * kShiftW + (n1 + kShiftT)) * kShiftW + (n2 + kShiftT);
-                    x = __dsub_rn(__dsub_rn(rix, cx[q]), shv[0][ti]);
-                    y = __dsub_rn(__dsub_rn(riy, cy[q]), shv[1][ti]);
-                    z = __dsub_rn(__dsub_rn(riz, cz[q]), shv[2][ti]);
-                    d2 = __dadd_rn(__dadd_rn(__dmul_rn(x, x), __dmul_rn(y, y)), __dmul_rn(z, z));
-                } else {
-                    pair_vec_v(rix, riy, riz, cx[q], cy[q], cz[q], R, n0, n1, n2, x, y, z, d2);
-                }
-                ok = d2 > 0.0 && d2 < rc2 && !(j == i && n0 == 0 && n1 == 0 && n2 == 0);
-                key = ((uint64_t)j << 30) | (uint64_t)pack_img(n0, n1, n2);
+            bool ok = false, unsure = false;
+            if (q < total && q != qc) {
+                const float4 xq = cf[q];
+                const float dx = xi.x - xq.x, dy = xi.y - xq.y, dz = xi.z - xq.z;
+                const float d2f = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
+                ok = d2f < lo_thr && d2f > band;
+                unsure = d2f <= band || (d2f >= lo_thr && d2f <= hi_thr);
             }
+            const uint64_t jd = q < total ? cjd[q] : 0ull;
+            const int j = (int)(jd >> 32);
+            int d0, d1, d2i;
+            unpack_img((uint32_t)(jd & 0xffffffffu), d0, d1, d2i);
+            const int n0 = si0 + d0, n1 = si1 + d1, n2 = si2 + d2i;
+            if (__ballot(unsure)) {
+                if (unsure) {
+                    double x, y, z, d2;
+                    pair_vec(pos, i, j, R, n0, n1, n2, x, y, z, d2);
+                    ok = d2 > 0.0 && d2 < rc2;
+                }
+            }
+            const uint64_t key = ((uint64_t)j << 30) | (uint64_t)pack_img(n0, n1, n2);
             const uint64_t mask = __ballot(ok);
             const int before = __popcll(mask & ((1ull << lane) - 1ull));
-            if (ok && count + before < kMaxNeigh) wk[count + before] = key;
+            if (ok && count + before < kWaveKeys) wk[count + before] = key;
             count += __popcll(mask);
         }
-        const int n = count < kMaxNeigh ? count : kMaxNeigh;
+        if (count > kWaveKeys) {
+            // more neighbours than the staged buffer holds: the bin goes to the wide search
+            if (lane == 0) {
+                bin_ovf[cen_bin[cb]] = 1;
+                atomicOr(flag, 8);
+            }
+            continue;
+        }
+        const int n = count;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         uint64_t* dst = keys_out + (size_t)i * kMaxNeigh;
-        if (n <= kWave) sort_and_store<1>(wk, n, dst);
-        else if (n <= 2 * kWave) sort_and_store<2>(wk, n, dst);
-        else if (n <= 4 * kWave) sort_and_store<4>(wk, n, dst);
-        else sort_and_store<8>(wk, n, dst);
+        if (n <= kWave) rank_and_store<1>(wk, n, dst);
+        else if (n <= 2 * kWave) rank_and_store<2>(wk, n, dst);
+        else rank_and_store<4>(wk, n, dst);
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if (lane == 0) {
-            cnt_out[i] = n;
-            if (count > kMaxNeigh) atomicOr(flag, 1);
-        }
+        if (lane == 0) cnt_out[i] = n;
         wmax = max(wmax, count);
     }
     // one global atomic per block (a contended atomic per atom serialises the kernel)
@@ -474,9 +455,13 @@ __global__ void k_compact(int N, const double* __restrict__ pos, const int* __re
                           const double* __restrict__ cells, const int* __restrict__ species,
                           const uint64_t* __restrict__ keys, const int* __restrict__ off,
                           int* __restrict__ idx, uint32_t* __restrict__ img, int* __restrict__ rev,
-                          float4* __restrict__ geo, uint8_t* __restrict__ nsp, int cap) {
-    const int i = xcd_block(blockIdx.x, gridDim.x);
-    if (i >= N || off[N] > cap) return;     // a list that does not fit is compacted again after growing
+                          float4* __restrict__ geo, uint8_t* __restrict__ nsp, const int* __restrict__ order,
+                          int cap) {
+    // centres in bin order: the blocks of one XCD cover one region of space,
+    // so the partners they read stay in that XCD's L2
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    if (b >= N || off[N] > cap) return;     // a list that does not fit is compacted again after growing
+    const int i = order[b];
     const int o = off[i], n = off[i + 1] - o;
     const double* R = cells + 9 * sof[i];
     for (int t = threadIdx.x; t < n; t += blockDim.x) {
@@ -504,9 +489,12 @@ __global__ void k_compact(int N, const double* __restrict__ pos, const int* __re
 // so a slot without a partner (an asymmetric list, impossible for the
 // symmetric FP64 test) stays in range; the missing upper match is flagged.
 __global__ void k_rev(int N, const int* __restrict__ idx, const uint32_t* __restrict__ img,
-                      const int* __restrict__ off, int* __restrict__ rev, int* __restrict__ flag, int cap) {
-    const int i = xcd_block(blockIdx.x, gridDim.x);
-    if (i >= N || off[N] > cap) return;
+                      const int* __restrict__ off, int* __restrict__ rev, int* __restrict__ flag,
+                      const int* __restrict__ order, int cap) {
+    // centres in bin order (see k_compact)
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    if (b >= N || off[N] > cap) return;
+    const int i = order[b];
     const int o = off[i], n = off[i + 1] - o;
     for (int t = threadIdx.x; t < n; t += blockDim.x) {
         const int j = idx[o + t];
@@ -515,8 +503,8 @@ __global__ void k_rev(int N, const int* __restrict__ idx, const uint32_t* __rest
         unpack_img(p, n0, n1, n2);
         const uint32_t want = pack_img(-n0, -n1, -n2);
         if (j < i || (j == i && p < want)) continue;       // lower slot: written by its partner
-        const int b = off[j], e = off[j + 1];
-        int lo = b, hi = e;
+        const int b0 = off[j], e = off[j + 1];
+        int lo = b0, hi = e;
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
             if (idx[mid] < i) lo = mid + 1; else hi = mid;
@@ -755,12 +743,15 @@ static void launch_compact(ann_ctx* ctx, int cap) {
     const int N = ctx->natoms;
     hipStream_t st = ctx->stream;
     NList& nl = ctx->nl;
-    Launch L(ctx, "nb_compact");
-    k_compact<<<N, 128, 0, st>>>(N, ctx->d_pos, ctx->d_struc_of, ctx->d_cells, ctx->d_species, ctx->d_keys, nl.off,
-                                 nl.idx, nl.img, nl.rev, nl.geo, nl.nsp, cap);
+    {
+        Launch L(ctx, "nb_compact");
+        k_compact<<<N, 128, 0, st>>>(N, ctx->d_pos, ctx->d_struc_of, ctx->d_cells, ctx->d_species, ctx->d_keys,
+                                     nl.off, nl.idx, nl.img, nl.rev, nl.geo, nl.nsp, ctx->d_bin_atoms, cap);
+    }
     // a missing reverse slot (impossible for the symmetric FP64 test) is
     // reported by the next synchronising call (ann_nlist_size, next build)
-    k_rev<<<N, 128, 0, st>>>(N, nl.idx, nl.img, nl.off, nl.rev, ctx->d_flag + 2, cap);
+    Launch L(ctx, "nb_rev");
+    k_rev<<<N, 128, 0, st>>>(N, nl.idx, nl.img, nl.off, nl.rev, ctx->d_flag + 2, ctx->d_bin_atoms, cap);
 }
 
 hipError_t neighbors_build(ann_ctx* ctx, double rc) {
@@ -792,8 +783,8 @@ hipError_t neighbors_build(ann_ctx* ctx, double rc) {
     {
         Launch L(ctx, "nb_search");
         k_search_bins<<<ctx->grid_nsb, kStageThreads, 0, st>>>(dci, ctx->d_cells, ctx->d_bin_struc, ctx->d_bin_start,
-                                                      ctx->d_binrec, rc * rc, ctx->d_keys, ctx->d_cnt, ctx->d_flag,
-                                                      ctx->d_bin_ovf);
+                                                      ctx->d_binrec, ctx->d_pos, rc * rc, ctx->d_keys, ctx->d_cnt,
+                                                      ctx->d_flag, ctx->d_bin_ovf);
     }
     HIPCHK(hipGetLastError());
     NList& nl = ctx->nl;

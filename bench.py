@@ -52,9 +52,9 @@ def work_model(model, batch, nl):
       symm_tan (triplet tiles / PLAIN) = symm_fwd + symm_bwd
     networks: 4 MAC per atom and parameter-matrix entry (forward + input backward);
     second backward (net_bwd2) 8 MAC, parameter-gradient GEMMs (grad_reduce) 4 MAC;
-    HBM kernels in bytes: nb_search 24 B/atom + 8 B/slot, nb_compact (CSR fill and
-    reverse slots) 28 B/atom + 57 B/slot, force_gather 96 B/atom + 44 B/slot, struc_sums
-    76 B/atom."""
+    HBM kernels in bytes: nb_search 24 B/atom + 8 B/slot, nb_compact (CSR fill) 28 B/atom
+    + 37 B/slot, nb_rev (reverse slots) 20 B/slot, force_gather 96 B/atom + 44 B/slot,
+    struc_sums 76 B/atom."""
     off = nl["off"]
     nsp = batch.species[nl["idx"]]
     S = model.nspecies
@@ -103,7 +103,7 @@ def work_model(model, batch, nl):
             "symm_tan": fwd + bwd, "symm_tan_mom": tan_m, "net": 4.0 * mac,
             "net_bwd2": 8.0 * mac, "grad_reduce": 4.0 * mac,
             "pairs": P, "triplets": T,
-            "bytes": {"nb_search": 24.0 * N + 8.0 * P, "nb_compact": 28.0 * N + 57.0 * P,
+            "bytes": {"nb_search": 24.0 * N + 8.0 * P, "nb_compact": 28.0 * N + 37.0 * P, "nb_rev": 20.0 * P,
                       "force_gather": 96.0 * N + 44.0 * P, "struc_sums": 76.0 * N}}
 
 
@@ -535,7 +535,7 @@ def main():
         eng.profile(True)
         time_ef(eng, form, max(3, args.steps // 2), args.warmup, out)
         kern = {}
-        for k in ("nb_bin", "nb_search", "nb_compact", "symm_fwd_plain", "symm_fwd_fast", "symm_fwd_mom",
+        for k in ("nb_bin", "nb_search", "nb_compact", "nb_rev", "symm_fwd_plain", "symm_fwd_fast", "symm_fwd_mom",
                   "symm_bwd_plain", "symm_bwd_fast", "symm_bwd_mom", "net_plain", "net_mfma", "force_gather",
                   "struc_sums"):
             ms, n = eng.profile_get(k)

@@ -36,10 +36,22 @@ def _cases():
         # search boxes too large to stage in LDS (2500 candidates; 343 bin images)
         "tiny20": (configs.model_lips(), configs.small_random(25, natoms=20, a0=5.9)),
         "tiny1": (configs.model_lips(), configs.single([[0.3, 0.2, 0.1]], np.eye(3) * 2.5, [0])),
+        # pairs exactly at the cutoff (simple cubic, 3.0 A spacing, rc = 6.0): the FP64 test decides
+        "lattice216": (configs.model_ar256(), configs.single(_lattice(0.0, 0.0), np.eye(3) * 18.0,
+                                                              np.zeros(216, np.int32))),
+        # the same with 1e-9 A jitter, positions 1000 A away from the cell (large image shifts)
+        "lattice_far": (configs.model_ar256(), configs.single(_lattice(1e-9, 1000.0), np.eye(3) * 18.0,
+                                                               np.zeros(216, np.int32))),
     }
 
 
+def _lattice(jitter, shift):
+    g = np.array([[i, j, k] for i in range(6) for j in range(6) for k in range(6)], float) * 3.0
+    return g + np.random.default_rng(31).normal(0.0, jitter, g.shape) + shift if jitter else g + shift
+
+
 CASES = ["ar256", "water4", "uneven6", "lips3", "random20", "tiny20", "tiny1"]
+NL_CASES = CASES + ["lattice216", "lattice_far"]
 
 
 @pytest.fixture(scope="module")
@@ -67,7 +79,7 @@ def _check_nlist(eng, batch, orc, rc=None):
     assert base == len(nl["idx"])
 
 
-@pytest.mark.parametrize("name", CASES)
+@pytest.mark.parametrize("name", NL_CASES)
 def test_neighbor_list_bit_exact(cases, oracle_mod, name):
     model, batch = cases[name]
     print(f"case {name}", flush=True)
