@@ -468,7 +468,8 @@ extern "C" int ann_destroy(ann_ctx* ctx) {
                     ctx->d_chunks, ctx->d_chunk_off, ctx->d_part, ctx->d_pos_ref, ctx->d_keatom, ctx->d_md_flag,
                     ctx->d_bin_struc, ctx->d_binrec, ctx->d_adam_t, ctx->d_wsplit, ctx->d_T,
                     ctx->da.pitems, ctx->da.preflag, ctx->da.inpb64, ctx->da.inpb_net, ctx->da.rad_rs64,
-                    ctx->da.rad_eta64, ctx->da.ang_eta64, ctx->da.ang_zeta64};
+                    ctx->da.rad_eta64, ctx->da.ang_eta64, ctx->da.ang_zeta64, ctx->d_items, ctx->d_qoff,
+                    ctx->d_queue};
     for (void* p : ptrs)
         if (p) hipFree(p);
     if (ctx->h_rb) hipHostFree(ctx->h_rb);
@@ -620,6 +621,7 @@ extern "C" int ann_set_batch(ann_ctx* ctx, const ann_batch* b) {
                            ctx->stream));
     ANN_HIP(hipMemsetAsync(ctx->d_cnt, 0, sizeof(int) * (N + 1), ctx->stream));
     ANN_HIP(struc_index(ctx));
+    ANN_HIP(build_items(ctx));
     ANN_HIP(hipStreamSynchronize(ctx->stream));
     ctx->have_batch = true;
     ctx->batch_gen++;
@@ -893,7 +895,8 @@ extern "C" int ann_energy_forces(ann_ctx* ctx, int form, int with_evac, double* 
         (uint64_t)(uintptr_t)ctx->d_perm,
         (uint64_t)(uintptr_t)ctx->d_T, (uint64_t)(uintptr_t)ctx->d_chunks, (uint64_t)ctx->nchunks,
         (uint64_t)ctx->tune_apb[0], (uint64_t)ctx->tune_apb[1], (uint64_t)ctx->tune_apb[2],
-        (uint64_t)ctx->tune_angular, (uint64_t)ctx->wsplit_valid, ctx->batch_gen, (uint64_t)ctx->ntiles,
+        (uint64_t)ctx->tune_angular, (uint64_t)ctx->tune_queue, (uint64_t)ctx->wsplit_valid, ctx->batch_gen,
+        (uint64_t)ctx->ntiles,
         (uint64_t)(uintptr_t)ctx->d_tiles, ctx->scaling_gen, (uint64_t)ctx->npre};
     if (!ctx->ef_exec || key != ctx->ef_key) {
         if (ctx->ef_exec) { hipGraphExecDestroy(ctx->ef_exec); ctx->ef_exec = nullptr; }
@@ -1106,6 +1109,7 @@ static double time_ms(ann_ctx* ctx, hipError_t (*f)(ann_ctx*, int), int arg, int
 static hipError_t run_fwd(ann_ctx* c, int form) { return symm_fwd(c, form, c->d_G); }
 static hipError_t run_bwd(ann_ctx* c, int form) { return symm_bwd(c, form, c->d_dEdG, c->d_P); }
 static hipError_t run_tan(ann_ctx* c, int form) { return symm_tangent(c, form, c->d_u, c->d_v); }
+static hipError_t run_ef(ann_ctx* c, int form) { return ef_body(c, form, 1, nullptr, c->d_F, nullptr); }
 static hipError_t run_nb(ann_ctx* c, int) {
     c->grid_valid = false;      // the blocking policy is part of the cached grid
     return neighbors_build(c, c->nl_rc);
@@ -1136,6 +1140,20 @@ extern "C" int ann_autotune(ann_ctx* ctx, int form) {
             ctx->tune_apb[mode] = pick;
             if (mode < 2) best_total += best;
         }
+        // persistent work queues against grid dispatch, timed as whole E+F steps
+        // (the dispatch interacts with the stored moments and the network's
+        // timing, which kernel-by-kernel timings miss)
+        {
+            ctx->tune_queue = 0;
+            const double tg = time_ms(ctx, run_ef, form, 3, e);
+            ctx->tune_queue = 1;
+            const double tq = e == hipSuccess ? time_ms(ctx, run_ef, form, 3, e) : 0.0;
+            if (e != hipSuccess) return fail(ANN_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
+            ctx->tune_queue = tq < tg ? 1 : 0;
+            RET_HIP(symm_fwd(ctx, form, ctx->d_G), "symm_fwd");
+            best_total = time_ms(ctx, run_fwd, form, 3, e) + time_ms(ctx, run_bwd, form, 3, e);
+            if (e != hipSuccess) return fail(ANN_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
+        }
         // moment form against the triplet tiles (G + adjoint)
         ctx->tune_angular = 1;
         const double tt = time_ms(ctx, run_fwd, form, 3, e) + time_ms(ctx, run_bwd, form, 3, e);
@@ -1165,6 +1183,7 @@ extern "C" int ann_get_tuning(ann_ctx* ctx, int* t, int n) {
     if (!t || n < 5) return fail(ANN_ERR_ARG, "tuning needs 5 entries");
     t[0] = ctx->tune_apb[0]; t[1] = ctx->tune_apb[1]; t[2] = ctx->tune_apb[2];
     t[3] = ctx->tune_angular; t[4] = ctx->tune_sb;
+    if (n > 5) t[5] = ctx->tune_queue;
     return ANN_OK;
 }
 
@@ -1173,11 +1192,13 @@ extern "C" int ann_set_tuning(ann_ctx* ctx, const int* t, int n) {
     if (!t || n < 5) return fail(ANN_ERR_ARG, "tuning needs 5 entries");
     for (int q = 0; q < 3; ++q)
         if (t[q] != 1 && t[q] != 2 && t[q] != 4) return fail(ANN_ERR_ARG, "atoms per workgroup must be 1, 2 or 4");
-    if ((t[3] != 0 && t[3] != 1) || (t[4] != 0 && t[4] != 1)) return fail(ANN_ERR_ARG, "invalid tuning switch");
+    if ((t[3] != 0 && t[3] != 1) || (t[4] != 0 && t[4] != 1) || (n > 5 && t[5] != 0 && t[5] != 1))
+        return fail(ANN_ERR_ARG, "invalid tuning switch");
     for (int q = 0; q < 3; ++q) ctx->tune_apb[q] = t[q];
     ctx->tune_angular = t[3];
     if (ctx->tune_sb != t[4]) ctx->grid_valid = false;
     ctx->tune_sb = t[4];
+    if (n > 5) ctx->tune_queue = t[5];
     ctx->gen++;
     return ANN_OK;
 }

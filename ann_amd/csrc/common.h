@@ -252,6 +252,13 @@ struct ann_ctx {
     int tune_apb[3] = {1, 1, 1};      // atoms (wavefronts) per workgroup of the moment kernels, per mode
     int tune_angular = 0;             // 0: moment form where it applies, 1: triplet tiles
     int tune_sb = 0;                  // neighbour search blocks: 0 heuristic (2x2x2 bins where they fit), 1 single bins
+    int tune_queue = 0;               // moment kernels: 0 grid dispatch, 1 persistent work queues
+    // persistent dispatch: (first atom, count) items per XCD queue, built at ann_set_batch
+    int2* d_items = nullptr;
+    int* d_qoff = nullptr;
+    int* d_queue = nullptr;           // counters, one per queue on its own 128-byte line
+    int items_cap = 0;
+    int num_cu = 0;
     // training step captured as a HIP graph and replayed
     bool graph_on = false, capturing = false, train_warm = false;
     hipStream_t gstream = nullptr;    // private non-blocking stream the graph is captured on and replayed
@@ -352,6 +359,7 @@ hipError_t refresh_geometry(ann_ctx* ctx);
 // the caller's host CSR list (off, idx, img [P][3]) for the current batch (ANN status)
 int set_neighbors_host(ann_ctx* ctx, const int* off, const int* idx, const int* img, double rc);
 void pair_free(ann_ctx* ctx);
+hipError_t build_items(ann_ctx* ctx);
 hipError_t wrap_positions(ann_ctx* ctx, double* pos);
 hipError_t md_kick_drift(ann_ctx* ctx, double dt, const double* mass, const double* F, double* vel, double* pos,
                          double lim2);

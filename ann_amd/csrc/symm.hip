@@ -30,6 +30,9 @@
 //        two slot contributions are accumulated in registers (owner) and LDS
 //        (partner) -- no global atomics, deterministic.
 //  Both are FP32 VALU-bound (the roofline that prices them is the FP32 peak).
+#include <algorithm>
+#include <vector>
+
 #include "basis_dev.h"
 
 namespace ann {
@@ -55,6 +58,11 @@ struct SymmArgs {
     float* v;
     float* Tout;              // moment form, G: store each atom's moments T here (or null)
     const float* Tin;         // moment form, adjoint: moments of the same geometry (or null: recompute)
+    // persistent dispatch of the moment kernels (or null): per XCD a queue of
+    // (first atom, atom count) items, its counter at queue[32 q]
+    int* queue;
+    const int2* items;
+    const int* qoff;          // [kXcds + 1] item ranges of the queues
 };
 
 // Sum N per-lane values over the 64 lanes with a transposing butterfly: at
@@ -844,20 +852,16 @@ __constant__ MonoTable c_mono = MonoTable();
 
 __device__ __forceinline__ int pair_of(int b, int c) { return b <= c ? c * (c + 1) / 2 + b : b * (b + 1) / 2 + c; }
 
-// One wavefront per atom; a block holds blockDim.x / 64 atoms (the "atoms per
-// workgroup" tuning), each wavefront with its own LDS slice.
+// One wavefront per atom with its own LDS slice (smem4); no block barriers, so
+// the waves of a workgroup are independent.
 template <int MODE, int Z>
-__global__ __launch_bounds__(256) void k_moment(SymmArgs A) {
+__device__ __forceinline__ void moment_atom(const SymmArgs& A, const int i, float4* smem4) {
     constexpr int NM = nmono(Z);
     constexpr int ZP = Z + 1;
     constexpr int PWS = (3 * ZP * (MODE == 2 ? 2 : 1)) | 1;   // powers per neighbour, odd stride
     constexpr int U4 = kMaxUnionEta;
     constexpr int NG = kWave / NM;                           // lane groups over the neighbours
-    extern __shared__ float4 smem_all[];
-    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-    const int i = blockIdx.x * (blockDim.x / kWave) + wave;
-    if (i >= A.N) return;
-    float4* smem4 = smem_all + (size_t)wave * (A.smem_wave / 16);
+    const int lane = threadIdx.x % kWave;
     const DevModel& Mo = *A.dm;
     const int a = A.species[i], o = A.off[i];
     int n = A.off[i + 1] - o;
@@ -1230,6 +1234,44 @@ __global__ __launch_bounds__(256) void k_moment(SymmArgs A) {
     }
 }
 
+// Grid dispatch: a block holds blockDim.x / 64 atoms (the "atoms per
+// workgroup" tuning).  Persistent dispatch (A.queue set): a grid sized to the
+// resident capacity, every wavefront pulls (first atom, count) items off the
+// queue of its XCD (blocks go to XCD blockIdx.x % 8) until the queue is empty;
+// the items of a batch run largest structure first.
+template <int MODE, int Z>
+__global__ __launch_bounds__(256) void k_moment(SymmArgs A) {
+    extern __shared__ float4 smem_all[];
+    const int wave = threadIdx.x / kWave;
+    float4* smem4 = smem_all + (size_t)wave * (A.smem_wave / 16);
+    if (A.queue) {
+        const int lane = threadIdx.x % kWave;
+        const int q = blockIdx.x % kXcds;
+        const int beg = A.qoff[q], cnt = A.qoff[q + 1] - beg;
+        int* ctr = A.queue + 32 * q;
+        int k = 0;
+        if (lane == 0) k = atomicAdd(ctr, 1);
+        k = __shfl(k, 0);
+        while (k < cnt) {
+            // the next item is claimed before this one runs: the atomic's round
+            // trip overlaps the item's own loads
+            int nk = 0;
+            if (lane == 0) nk = atomicAdd(ctr, 1);
+            const int2 it = A.items[beg + k];
+            for (int a = it.x; a < it.x + it.y; ++a) moment_atom<MODE, Z>(A, a, smem4);
+            k = __shfl(nk, 0);
+        }
+        return;
+    }
+    const int i = blockIdx.x * (blockDim.x / kWave) + wave;
+    if (i >= A.N) return;
+    moment_atom<MODE, Z>(A, i, smem4);
+}
+
+__global__ void k_queue_reset(int* __restrict__ queue) {
+    if (threadIdx.x < kXcds) queue[32 * threadIdx.x] = 0;
+}
+
 static size_t moment_smem(const SymmArgs& A, int mode, int Z, int S, int NP) {
     const int NM = nmono(Z), ZP = Z + 1, U4 = kMaxUnionEta;
     const int PWS = (mode == 1 && A.Tin) ? 2 : (3 * ZP * (mode == 2 ? 2 : 1)) | 1;
@@ -1245,6 +1287,59 @@ static size_t moment_smem(const SymmArgs& A, int mode, int Z, int S, int NP) {
 }
 
 // ================================================================ host ====
+
+// Work items of the persistent dispatch: every structure cut into ranges of
+// up to kItemAtoms atoms (fewer when the batch would leave resident
+// wavefronts without items: about four items per wavefront the chip holds),
+// structures in decreasing size, items dealt round robin to the XCD queues so
+// that each queue gets an equal share of every size.
+constexpr int kItemAtoms = 8;
+
+hipError_t build_items(ann_ctx* ctx) {
+    const int S = ctx->nstruc;
+    hipError_t e;
+    if (!ctx->d_queue) {
+        if ((e = hipMalloc(&ctx->d_queue, sizeof(int) * 32 * kXcds)) != hipSuccess) return e;
+        if ((e = hipMalloc(&ctx->d_qoff, sizeof(int) * (kXcds + 1))) != hipSuccess) return e;
+        int dev = 0;
+        if ((e = hipGetDevice(&dev)) != hipSuccess) return e;
+        if ((e = hipDeviceGetAttribute(&ctx->num_cu, hipDeviceAttributeMultiprocessorCount, dev)) != hipSuccess)
+            return e;
+    }
+    const int resident = std::max(1, ctx->num_cu) * 32;            // wavefronts per CU the kernels can hold
+    const int len = std::max(1, std::min(kItemAtoms, ctx->atom_off[S] / (4 * resident)));
+    std::vector<int> order(S);
+    for (int s = 0; s < S; ++s) order[s] = s;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        return ctx->atom_off[a + 1] - ctx->atom_off[a] > ctx->atom_off[b + 1] - ctx->atom_off[b];
+    });
+    std::vector<int2> all;
+    for (int s : order)
+        for (int a = ctx->atom_off[s]; a < ctx->atom_off[s + 1]; a += len)
+            all.push_back(make_int2(a, std::min(len, ctx->atom_off[s + 1] - a)));
+    std::vector<int2> items;
+    items.reserve(all.size());
+    std::vector<int> qoff(kXcds + 1, 0);
+    for (int q = 0; q < kXcds; ++q) {
+        qoff[q] = (int)items.size();
+        for (size_t k = q; k < all.size(); k += kXcds) items.push_back(all[k]);
+    }
+    qoff[kXcds] = (int)items.size();
+    if ((int)items.size() > ctx->items_cap) {
+        if (ctx->d_items && (e = hipFree(ctx->d_items)) != hipSuccess) return e;
+        ctx->d_items = nullptr;
+        ctx->items_cap = 0;
+        if ((e = hipMalloc(&ctx->d_items, sizeof(int2) * items.size())) != hipSuccess) return e;
+        ctx->items_cap = (int)items.size();
+    }
+    if ((e = hipMemcpyAsync(ctx->d_items, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice,
+                            ctx->stream)) != hipSuccess)
+        return e;
+    if ((e = hipMemcpyAsync(ctx->d_qoff, qoff.data(), sizeof(int) * (kXcds + 1), hipMemcpyHostToDevice,
+                            ctx->stream)) != hipSuccess)
+        return e;
+    return hipStreamSynchronize(ctx->stream);     // the host vectors go out of scope
+}
 
 static SymmArgs make_args(ann_ctx* ctx) {
     SymmArgs A{};
@@ -1301,14 +1396,24 @@ static hipError_t run(ann_ctx* ctx, int form, SymmArgs A) {
         int apb = ctx->tune_apb[MODE];
         while (apb > 1 && sw * apb > 65536) apb /= 2;          // dynamic LDS of one workgroup
         A.smem_wave = (int)sw;
-        const unsigned grid = (unsigned)((A.N + apb - 1) / apb), block = 64u * apb;
+        unsigned grid = (unsigned)((A.N + apb - 1) / apb), block = 64u * apb;
         const size_t sm = sw * apb;
-        switch (M.moment_z) {
-            case 1: k_moment<MODE, 1><<<grid, block, sm, ctx->stream>>>(A); break;
-            case 2: k_moment<MODE, 2><<<grid, block, sm, ctx->stream>>>(A); break;
-            case 3: k_moment<MODE, 3><<<grid, block, sm, ctx->stream>>>(A); break;
-            default: k_moment<MODE, 4><<<grid, block, sm, ctx->stream>>>(A); break;
+        void (*kern)(SymmArgs) = M.moment_z == 1 ? k_moment<MODE, 1>
+                                 : M.moment_z == 2 ? k_moment<MODE, 2>
+                                 : M.moment_z == 3 ? k_moment<MODE, 3> : k_moment<MODE, 4>;
+        if (ctx->tune_queue == 1 && ctx->d_items) {
+            // persistent: enough workgroups for the most wavefronts a CU can hold
+            // (32); those that find no room start later, find the queues empty
+            // and exit.  A multiple of the XCD count, so every queue has workers.
+            const unsigned cap = (unsigned)std::max(1, ctx->num_cu) * (32u / (unsigned)apb);
+            grid = std::min(grid, cap);
+            grid = std::max(kXcds, (int)((grid + kXcds - 1) / kXcds * kXcds));
+            A.queue = ctx->d_queue;
+            A.items = ctx->d_items;
+            A.qoff = ctx->d_qoff;
+            k_queue_reset<<<1, 64, 0, ctx->stream>>>(ctx->d_queue);
         }
+        kern<<<grid, block, sm, ctx->stream>>>(A);
         if (MODE == 0) ctx->T_gen = A.Tout ? ctx->gen : ~0ull;
         return hipGetLastError();
     }

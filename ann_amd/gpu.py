@@ -402,13 +402,14 @@ class Engine:
         return self.tuning()
 
     def tuning(self) -> list:
-        t = (C.c_int * 5)()
-        _check(lib().ann_get_tuning(self.h, t, 5))
+        """[atoms per workgroup G, adjoint, tangent; triplet tiles; single-bin search; persistent queues]"""
+        t = (C.c_int * 6)()
+        _check(lib().ann_get_tuning(self.h, t, 6))
         return list(t)
 
     def set_tuning(self, t) -> None:
-        v = (C.c_int * 5)(*[int(x) for x in t])
-        _check(lib().ann_set_tuning(self.h, v, 5))
+        v = (C.c_int * len(t))(*[int(x) for x in t])
+        _check(lib().ann_set_tuning(self.h, v, len(t)))
 
     # ------------------------------------------------------------ timing
     def profile(self, enable: bool = True) -> None:

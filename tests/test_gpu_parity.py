@@ -204,7 +204,7 @@ def test_network_parity(cases, oracle_mod, name, form):
     assert np.all(dE[:, orc.ld:] == 0)
 
 
-@pytest.mark.parametrize("tuning", [[2, 2, 2, 0, 0], [4, 4, 4, 0, 1], [1, 1, 1, 1, 0]])
+@pytest.mark.parametrize("tuning", [[2, 2, 2, 0, 0], [4, 4, 4, 0, 1], [1, 1, 1, 1, 0], [2, 1, 1, 0, 0, 1]])
 @pytest.mark.parametrize("name", ["water4", "lips3"])
 def test_tuning_variants_keep_parity(cases, oracle_mod, name, tuning):
     """Every tuning choice (atoms per workgroup, moment/triplet kernels,
@@ -225,11 +225,32 @@ def test_tuning_variants_keep_parity(cases, oracle_mod, name, tuning):
     assert np.sqrt(((F.cpu().numpy() - Fo) ** 2).sum() / (Fo ** 2).sum()) <= 1e-5
 
 
+@pytest.mark.parametrize("name", ["uneven6", "water4"])
+def test_persistent_queues_match_grid_dispatch(cases, name):
+    """The persistent work-queue dispatch of the moment kernels gives the
+    grid dispatch's G, forces and adjoint bit for bit (each atom's wavefront
+    does the same work; only the order of atoms over the device differs)."""
+    import torch
+    from ann_amd.gpu import FAST
+    model, batch = cases[name]
+    out = []
+    for q in (0, 1):
+        eng = _engine(model, batch)
+        eng.set_tuning([1, 2, 1, 0, 0, q])
+        G = eng.symm(FAST)
+        E, F, V = eng.energy_forces(FAST)
+        out.append((G, E, F, V))
+        torch.cuda.synchronize()
+    for a, b in zip(*out):
+        assert torch.equal(a, b)
+
+
 def test_autotune_picks_valid_settings(cases, oracle_mod):
     model, batch = cases["water4"]
     eng = _engine(model, batch)
     t = eng.autotune(1)
     assert t[0] in (1, 2, 4) and t[1] in (1, 2, 4) and t[2] in (1, 2, 4) and t[3] in (0, 1) and t[4] in (0, 1)
+    assert t[5] in (0, 1)
     E, F, _ = eng.energy_forces(1)
     Eo, Fo, _ = oracle_mod.Oracle(model).compute_batch(batch, nthreads=4)
     assert np.sqrt(((F.cpu().numpy() - Fo) ** 2).sum() / (Fo ** 2).sum()) <= 1e-5
