@@ -1235,37 +1235,41 @@ __device__ __forceinline__ void moment_atom(const SymmArgs& A, const int i, floa
 }
 
 // Grid dispatch: a block holds blockDim.x / 64 atoms (the "atoms per
-// workgroup" tuning).  Persistent dispatch (A.queue set): a grid sized to the
-// resident capacity, every wavefront pulls (first atom, count) items off the
-// queue of its XCD (blocks go to XCD blockIdx.x % 8) until the queue is empty;
-// the items of a batch run largest structure first.
+// workgroup" tuning).
 template <int MODE, int Z>
 __global__ __launch_bounds__(256) void k_moment(SymmArgs A) {
     extern __shared__ float4 smem_all[];
     const int wave = threadIdx.x / kWave;
-    float4* smem4 = smem_all + (size_t)wave * (A.smem_wave / 16);
-    if (A.queue) {
-        const int lane = threadIdx.x % kWave;
-        const int q = blockIdx.x % kXcds;
-        const int beg = A.qoff[q], cnt = A.qoff[q + 1] - beg;
-        int* ctr = A.queue + 32 * q;
-        int k = 0;
-        if (lane == 0) k = atomicAdd(ctr, 1);
-        k = __shfl(k, 0);
-        while (k < cnt) {
-            // the next item is claimed before this one runs: the atomic's round
-            // trip overlaps the item's own loads
-            int nk = 0;
-            if (lane == 0) nk = atomicAdd(ctr, 1);
-            const int2 it = A.items[beg + k];
-            for (int a = it.x; a < it.x + it.y; ++a) moment_atom<MODE, Z>(A, a, smem4);
-            k = __shfl(nk, 0);
-        }
-        return;
-    }
     const int i = blockIdx.x * (blockDim.x / kWave) + wave;
     if (i >= A.N) return;
-    moment_atom<MODE, Z>(A, i, smem4);
+    moment_atom<MODE, Z>(A, i, smem_all + (size_t)wave * (A.smem_wave / 16));
+}
+
+// Persistent dispatch (a separate kernel, so that the grid kernel keeps its
+// own register allocation): a grid sized to the resident capacity, every
+// wavefront pulls (first atom, count) items off the queue of its XCD (blocks
+// go to XCD blockIdx.x % 8) until the queue is empty; the items of a batch
+// run largest structure first.
+template <int MODE, int Z>
+__global__ __launch_bounds__(256) void k_moment_queue(SymmArgs A) {
+    extern __shared__ float4 smem_all[];
+    const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+    float4* smem4 = smem_all + (size_t)wave * (A.smem_wave / 16);
+    const int q = blockIdx.x % kXcds;
+    const int beg = A.qoff[q], cnt = A.qoff[q + 1] - beg;
+    int* ctr = A.queue + 32 * q;
+    int k = 0;
+    if (lane == 0) k = atomicAdd(ctr, 1);
+    k = __shfl(k, 0);
+    while (k < cnt) {
+        // the next item is claimed before this one runs: the atomic's round
+        // trip overlaps the item's own loads
+        int nk = 0;
+        if (lane == 0) nk = atomicAdd(ctr, 1);
+        const int2 it = A.items[beg + k];
+        for (int a = it.x; a < it.x + it.y; ++a) moment_atom<MODE, Z>(A, a, smem4);
+        k = __shfl(nk, 0);
+    }
 }
 
 __global__ void k_queue_reset(int* __restrict__ queue) {
@@ -1398,10 +1402,13 @@ static hipError_t run(ann_ctx* ctx, int form, SymmArgs A) {
         A.smem_wave = (int)sw;
         unsigned grid = (unsigned)((A.N + apb - 1) / apb), block = 64u * apb;
         const size_t sm = sw * apb;
-        void (*kern)(SymmArgs) = M.moment_z == 1 ? k_moment<MODE, 1>
-                                 : M.moment_z == 2 ? k_moment<MODE, 2>
-                                 : M.moment_z == 3 ? k_moment<MODE, 3> : k_moment<MODE, 4>;
-        if (ctx->tune_queue == 1 && ctx->d_items) {
+        const bool queued = ctx->tune_queue == 1 && ctx->d_items;
+        void (*kern)(SymmArgs) =
+            queued ? (M.moment_z == 1 ? k_moment_queue<MODE, 1> : M.moment_z == 2 ? k_moment_queue<MODE, 2>
+                      : M.moment_z == 3 ? k_moment_queue<MODE, 3> : k_moment_queue<MODE, 4>)
+                   : (M.moment_z == 1 ? k_moment<MODE, 1> : M.moment_z == 2 ? k_moment<MODE, 2>
+                      : M.moment_z == 3 ? k_moment<MODE, 3> : k_moment<MODE, 4>);
+        if (queued) {
             // persistent: enough workgroups for the most wavefronts a CU can hold
             // (32); those that find no room start later, find the queues empty
             // and exit.  A multiple of the XCD count, so every queue has workers.
