@@ -248,6 +248,16 @@ def run_uneven(steps, warmup):
         dt = time_ef(eng, form, k, max(1, warmup // 2), out)
         res[name] = {"atoms_per_s": batch.ntotal / dt, "ms_per_step": dt * 1e3, "steps": k}
     res["fast_over_plain"] = res["fast"]["atoms_per_s"] / res["plain"]["atoms_per_s"]
+    # the moment kernels under grid dispatch and under the persistent (structure,
+    # atom-range) work queues, the rest of the tuning as chosen
+    res["moment_dispatch"] = {}
+    for q, name in ((0, "grid"), (1, "persistent_queues")):
+        t = list(tuning) + [0] * (6 - len(tuning))
+        t[5] = q
+        eng.set_tuning(t)
+        dt = time_ef(eng, FAST, steps, max(1, warmup // 2), out)
+        res["moment_dispatch"][name] = {"atoms_per_s": batch.ntotal / dt, "ms_per_step": dt * 1e3}
+    eng.set_tuning(tuning)
     # correctness at full size: the 4 smallest and 4 largest structures of the
     # batch against the FP64 oracle (FAST form, the timed configuration)
     E, F, _ = eng.energy_forces(FAST, out=out)
@@ -633,7 +643,8 @@ def main():
         "roofline": roof,
         "kernel_rooflines": roofs,
         "tuning": {"atoms_per_workgroup": {"G": tuning[0], "adjoint": tuning[1], "tangent": tuning[2]},
-                   "angular": "triplet" if tuning[3] else "moment", "search_blocks": "single" if tuning[4] else "2x2x2"},
+                   "angular": "triplet" if tuning[3] else "moment", "search_blocks": "single" if tuning[4] else "2x2x2",
+                   "moment_dispatch": "persistent queues" if len(tuning) > 5 and tuning[5] else "grid"},
         "train": train,
         "uneven_batch": uneven,
         "md": md,
