@@ -241,7 +241,9 @@ static int upload_model(ann_ctx* ctx, const ann_model_desc* d) {
                 if (kt == ks.end()) ks.push_back(key);
             }
             const int ng = (int)ge.size(), nk = (int)ks.size();
-            bool grid = ab.name == 1 && ng <= kGridG && nk <= kGridK && ng * nk == n && fast_shape_supported(ng, nk);
+            // every family runs the triplet tiles (GAUSS with per-neighbour factors,
+            // the others with the pair factor of r_j^2 + r_k^2); the moment form needs GAUSS
+            bool grid = ng <= kGridG && nk <= kGridK && ng * nk == n && fast_shape_supported(ng, nk);
             if (grid) {
                 std::vector<int> seen(ng * nk, 0);
                 for (int m = 0; m < n; ++m) {
@@ -297,6 +299,7 @@ static int upload_model(ann_ctx* ctx, const ann_model_desc* d) {
         for (int t = 0; t < S * NP; ++t) {
             const AngBlock& ab = M.ang[t];
             if (ab.n == 0) continue;
+            if (ab.name != 1) ok = false;                        // only GAUSS factorises
             for (int k = 0; k < ab.nk; ++k) {
                 if (ab.zint[k] < 1 || ab.zint[k] > kMomZ || std::fabs(ab.lamh[k]) > 0.5f) ok = false;
                 z = std::max(z, ab.zint[k]);

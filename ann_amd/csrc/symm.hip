@@ -395,10 +395,45 @@ __device__ __forceinline__ int fast_load(const SymmArgs& A, const DevModel& M, i
     return acc;
 }
 
+// Distance factor of width g for a pair of legs j, k: GAUSS blocks multiply the
+// per-neighbour factors e_g(r) = exp(-r^2/eta^2) fc(r) (and their r-derivatives)
+// computed once per neighbour; the other families depend on r_j^2 + r_k^2
+// through a factor that does not separate, evaluated here per pair:
+// D = fr(r_j^2 + r_k^2) fc_j fc_k, dD/dr_j = (2 r_j fr' fc_j + fr fc'_j) fc_k.
+template <bool GS, int NG>
+__device__ __forceinline__ void pair_factors(const AngBlock& B, const float* uie, const int (&gu)[NG], const FastSmem& sm,
+                                             int jj, int kk, bool deriv, float (&D)[NG], float (&Dj)[NG],
+                                             float (&Dk)[NG]) {
+    if (GS) {
+        const float4 ej4 = sm.ef[jj], ek4 = sm.ef[kk];
+        float4 dj4 = make_float4(0.f, 0.f, 0.f, 0.f), dk4 = dj4;
+        if (deriv) { dj4 = sm.efd[jj]; dk4 = sm.efd[kk]; }
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            const float ej = f4get(ej4, g), ek = f4get(ek4, g);
+            D[g] = ej * ek;
+            Dj[g] = deriv ? f4get(dj4, g) * ek : 0.0f;
+            Dk[g] = deriv ? ej * f4get(dk4, g) : 0.0f;
+        }
+    } else {
+        const float rj = sm.r[jj], rk = sm.r[kk];
+        const float2 fj = sm.fg[jj], fk = sm.fg[kk];
+        const float r2s = rj * rj + rk * rk, ff = fj.x * fk.x;
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            float fr, dfr;
+            ang_radial(B.name, uie[gu[g]], r2s, fr, dfr);
+            D[g] = fr * ff;
+            Dj[g] = deriv ? (2.0f * rj * dfr * fj.x + fr * fj.y) * fk.x : 0.0f;
+            Dk[g] = deriv ? (2.0f * rk * dfr * fk.x + fr * fk.y) * fj.x : 0.0f;
+        }
+    }
+}
+
 // ---- forward / tangent over one grid block (unordered pairs, flattened) ----
-template <int NG, int NK, int MODE>
-__device__ __forceinline__ void fast_block_flat(const AngBlock& B, const FastSmem& sm, int sb, int nb, int sc, int nc, bool tri,
-                                const float4* tv, float* out) {
+template <int NG, int NK, int MODE, bool GS>
+__device__ __forceinline__ void fast_block_flat(const AngBlock& B, const float* uie, const FastSmem& sm, int sb, int nb,
+                                                int sc, int nc, bool tri, const float4* tv, float* out) {
     const int lane = threadIdx.x % kWave;
     float lamh[NK], zeta[NK];
     int zi[NK], gu[NG];
@@ -431,8 +466,9 @@ __device__ __forceinline__ void fast_block_flat(const AngBlock& B, const FastSme
         jj += sb;
         kk += sc;
         const float4 gj = sm.g[jj], gk = sm.g[kk];
-        const float4 ej4 = sm.ef[jj], ek4 = sm.ef[kk];
         const float c = gj.x * gk.x + gj.y * gk.y + gj.z * gk.z;
+        float Dg[NG], Dj[NG], Dk[NG];
+        pair_factors<GS, NG>(B, uie, gu, sm, jj, kk, MODE != 0, Dg, Dj, Dk);
         if (MODE == 0) {
             float p[NK];
 #pragma unroll
@@ -442,13 +478,11 @@ __device__ __forceinline__ void fast_block_flat(const AngBlock& B, const FastSme
             }
 #pragma unroll
             for (int g = 0; g < NG; ++g) {
-                const float D = f4get(ej4, g) * f4get(ek4, g);
 #pragma unroll
-                for (int k = 0; k < NK; ++k) acc[g][k] += p[k] * D;
+                for (int k = 0; k < NK; ++k) acc[g][k] += p[k] * Dg[g];
             }
         } else {
             const float4 tj = tv[jj], tk = tv[kk];
-            const float4 dj4 = sm.efd[jj], dk4 = sm.efd[kk];
             const float cdot = ((gk.x * tj.x + gk.y * tj.y + gk.z * tj.z) - c * tj.w) * gj.w +
                                ((gj.x * tk.x + gj.y * tk.y + gj.z * tk.z) - c * tk.w) * gk.w;
             float p[NK], dp[NK];
@@ -461,9 +495,8 @@ __device__ __forceinline__ void fast_block_flat(const AngBlock& B, const FastSme
             }
 #pragma unroll
             for (int g = 0; g < NG; ++g) {
-                const float ej = f4get(ej4, g), ek = f4get(ek4, g);
-                const float rdot = f4get(dj4, g) * ek * tj.w + ej * f4get(dk4, g) * tk.w;
-                const float D = ej * ek * cdot;
+                const float rdot = Dj[g] * tj.w + Dk[g] * tk.w;
+                const float D = Dg[g] * cdot;
 #pragma unroll
                 for (int k = 0; k < NK; ++k) acc[g][k] += dp[k] * D + p[k] * rdot;
             }
@@ -492,9 +525,9 @@ __device__ __forceinline__ void fast_block_flat(const AngBlock& B, const FastSme
 }
 
 // ---- adjoint over one grid block: skewed owner/partner schedule ----
-template <int NG, int NK>
-__device__ __forceinline__ void fast_block_adj(const AngBlock& B, const FastSmem& sm, int sb, int nb, int sc, int nc, bool tri,
-                               const float* sd, float* sP) {
+template <int NG, int NK, bool GS>
+__device__ __forceinline__ void fast_block_adj(const AngBlock& B, const float* uie, const FastSmem& sm, int sb, int nb,
+                                               int sc, int nc, bool tri, const float* sd, float* sP) {
     const int lane = threadIdx.x % kWave;
     float lamh[NK], zeta[NK], dl[NG][NK];
     int zi[NK], gu[NG];
@@ -519,10 +552,12 @@ __device__ __forceinline__ void fast_block_adj(const AngBlock& B, const FastSmem
         const int q = act ? id / no : 0;
         const int J = so + j;
         const float4 gj = sm.g[J];
-        const float4 ej4 = sm.ef[J], dj4 = sm.efd[J];
-        float ej[NG], edj[NG];
+        float ej[NG], edj[NG];                  // GAUSS: the owner's per-neighbour factors, loaded once
+        if constexpr (GS) {
+            const float4 ej4 = sm.ef[J], dj4 = sm.efd[J];
 #pragma unroll
-        for (int g = 0; g < NG; ++g) { ej[g] = f4get(ej4, g); edj[g] = f4get(dj4, g); }
+            for (int g = 0; g < NG; ++g) { ej[g] = f4get(ej4, g); edj[g] = f4get(dj4, g); }
+        }
         float px = 0.0f, py = 0.0f, pz = 0.0f;
         int len, koff;
         if (tri) {
@@ -539,8 +574,20 @@ __device__ __forceinline__ void fast_block_adj(const AngBlock& B, const FastSmem
                 if (k >= np) k -= np;
                 const int K = spp + k;
                 const float4 gk = sm.g[K];
-                const float4 ek4 = sm.ef[K], dk4 = sm.efd[K];
                 const float c = gj.x * gk.x + gj.y * gk.y + gj.z * gk.z;
+                float Dg[NG], Dj[NG], Dk[NG];
+                if constexpr (GS) {
+                    const float4 ek4 = sm.ef[K], dk4 = sm.efd[K];
+#pragma unroll
+                    for (int g = 0; g < NG; ++g) {
+                        const float ek = f4get(ek4, g);
+                        Dg[g] = ej[g] * ek;
+                        Dj[g] = edj[g] * ek;
+                        Dk[g] = ej[g] * f4get(dk4, g);
+                    }
+                } else {
+                    pair_factors<GS, NG>(B, uie, gu, sm, J, K, true, Dg, Dj, Dk);
+                }
                 float p[NK], dp[NK];
 #pragma unroll
                 for (int kk = 0; kk < NK; ++kk) {
@@ -558,10 +605,9 @@ __device__ __forceinline__ void fast_block_adj(const AngBlock& B, const FastSmem
                         Sg += dl[g][kk] * p[kk];
                         Tg += dl[g][kk] * dp[kk];
                     }
-                    const float ek = f4get(ek4, g), edk = f4get(dk4, g);
-                    Ac += Tg * ej[g] * ek;
-                    Bj += Sg * edj[g] * ek;
-                    Bk += Sg * ej[g] * edk;
+                    Ac += Tg * Dg[g];
+                    Bj += Sg * Dj[g];
+                    Bk += Sg * Dk[g];
                 }
                 const float aj = Ac * gj.w, ak = Ac * gk.w;
                 px += aj * (gk.x - c * gj.x) + Bj * gj.x;
@@ -785,8 +831,14 @@ __global__ __launch_bounds__(64) void k_fast(SymmArgs A) {
             const int sc = sm.seg[c], nc = sm.seg[c + 1] - sc;
             const bool tri = (b == c);
             if (tri ? nb < 2 : (nb == 0 || nc == 0)) continue;
-            if (MODE == 1) fast_block_adj<NG, NK>(B, sm, sb, nb, sc, nc, tri, rowbuf, sP);
-            else fast_block_flat<NG, NK, MODE == 0 ? 0 : 1>(B, sm, sb, nb, sc, nc, tri, tv, rowbuf);
+            const float* uie = M.uieta2[a];
+            if (B.name == ANG_GAUSS) {
+                if (MODE == 1) fast_block_adj<NG, NK, true>(B, uie, sm, sb, nb, sc, nc, tri, rowbuf, sP);
+                else fast_block_flat<NG, NK, MODE == 0 ? 0 : 1, true>(B, uie, sm, sb, nb, sc, nc, tri, tv, rowbuf);
+            } else {
+                if (MODE == 1) fast_block_adj<NG, NK, false>(B, uie, sm, sb, nb, sc, nc, tri, rowbuf, sP);
+                else fast_block_flat<NG, NK, MODE == 0 ? 0 : 1, false>(B, uie, sm, sb, nb, sc, nc, tri, tv, rowbuf);
+            }
             __syncthreads();
         }
     }

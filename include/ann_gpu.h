@@ -45,7 +45,8 @@ enum {
     ANN_FORM_FAST = 1,   /* angular sums as per-neighbour tensor moments where every
                             angular block is GAUSS with integer zeta <= 4 (exact
                             factorisation, O(n) per atom), otherwise species-blocked
-                            triplet tiles; pair-separable exponentials, atomic-free
+                            triplet tiles (GAUSS: pair-separable exponentials; other
+                            families: a distance factor per pair of legs); atomic-free
                             adjoint, network layers on BF16 MFMA with split hi+lo
                             products */
     ANN_FORM_TRIPLET = 2 /* FAST with the triplet tiles everywhere (no moment form) */
