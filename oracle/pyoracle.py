@@ -37,7 +37,8 @@ class OrcStruc(C.Structure):
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        # ORACLE_LIB selects another build of the same sources (the sanitizer build)
+        path = os.environ.get("ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             raise RuntimeError(f"oracle library missing: {path} (run __graft_entry__.build())")
         L = C.CDLL(path)

@@ -5,6 +5,7 @@
 // points (energy = compute + vacuum energies, force = compute, sum F = 0).
 //
 //   test_nnp_gpu roundtrip <in.nnp> <out.nnp>   read + write (no GPU)
+//   test_nnp_gpu basis <in.nnp> <basis.txt> <name> <out.nnp>   read_basis + write (no GPU)
 //   test_nnp_gpu force <pot.nnp> <seed>         GPU checks, prints errors, exit 0 on pass
 //   test_nnp_gpu dump <pot.nnp> <struc.txt> <out.txt> <skin> [<basis.txt> <name>]
 //       every entry point on one structure, written out for comparison with
@@ -117,6 +118,13 @@ int main(int argc, char** argv) {
     NNP nnp;
     NNP::read(argv[2], nnp);
     if (mode == "dump") return dump(nnp, argc, argv);
+    if (mode == "basis") {        // read_basis + write, no GPU call: basis <pot> <basis> <name> <out>
+        if (argc < 6) return 2;
+        NNP::read_basis(argv[3], nnp, argv[4]);
+        NNP::write(argv[5], nnp);
+        std::printf("types %d params %d\n", nnp.ntypes(), nnp.size());
+        return 0;
+    }
     if (mode == "roundtrip") {
         NNP::write(argv[3], nnp);
         std::printf("types %d rc %f params %d\n", nnp.ntypes(), nnp.rc(), nnp.size());
