@@ -14,6 +14,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 _REF = None
+_REF_RNG = None
 
 P_D = C.POINTER(C.c_double)
 P_I = C.POINTER(C.c_int)
@@ -67,6 +68,23 @@ def ref_lib():
             getattr(L, f).restype = C.c_double
         _REF = L
     return _REF
+
+
+def ref_rng_lib():
+    """The reference's random distributions (src/rand/rng.cpp) and std::mt19937
+    compiled into oracle/_ref (or None)."""
+    global _REF_RNG
+    if _REF_RNG is None:
+        path = os.path.join(_HERE, "_ref", "libref_rng.so")
+        if not os.path.exists(path):
+            return None
+        L = C.CDLL(path)
+        L.ref_rng_draw.argtypes = [C.c_int, C.c_double, C.c_uint, C.c_int, C.c_void_p]
+        L.ref_mt_raw.argtypes = [C.c_uint, C.c_int, C.c_void_p]
+        L.ref_uniform.argtypes = [C.c_uint, C.c_double, C.c_double, C.c_int, C.c_void_p]
+        L.ref_taylor_draws.argtypes = [C.c_int, C.c_double, C.c_uint, C.c_int, C.c_void_p]
+        _REF_RNG = L
+    return _REF_RNG
 
 
 def _p(a, t):
