@@ -63,6 +63,8 @@ static int drain_timers(ann_ctx* ctx) {
 
 hipError_t struc_index(ann_ctx* ctx);
 bool fast_shape_supported(int ng, int nk);
+bool vec_net_supported(const ann_ctx* ctx);
+bool mfma_net_supported(const DevModel& M);
 hipError_t train_step_impl(ann_ctx* ctx, int form, const double* Eref, const double* Fref, double wE, double wF,
                            double gamma, double* loss, double* grad);
 
@@ -383,6 +385,32 @@ static int upload_model(ann_ctx* ctx, const ann_model_desc* d) {
         }
         ctx->wsplit_n = acc;
         ctx->wsplit_valid = false;
+        // tables of the vector-ALU network (layout in net_dev.h): the weight
+        // rows of the hidden layers, their biases, the output neuron
+        int cacc = 0, cmax = 0;
+        for (int a = 0; a < S; ++a) {
+            NetDesc& D = M.net[a];
+            D.wc0 = cacc;
+            for (int l = 0; l < kMaxLayers; ++l) D.wcf[l] = D.wcs[l] = D.wcb[l] = 0;
+            for (int l = 0; l < D.nlayer - 1; ++l) {
+                const int ni = l == 0 ? D.nin : D.nodes[l - 1], no = D.nodes[l];
+                D.wcs[l] = vec_row_stride(no);
+                D.wcf[l] = cacc;
+                cacc += vec_pad4(ni * D.wcs[l]);
+            }
+            for (int l = 0; l < D.nlayer - 1; ++l) {
+                D.wcb[l] = cacc;
+                cacc += vec_pad4(D.nodes[l]);
+            }
+            const int nlast = D.nlayer > 1 ? D.nodes[D.nlayer - 2] : D.nin;
+            D.wco = cacc;
+            cacc += (nlast + 3) / 4 * 4 + 4;
+            D.wcn = cacc - D.wc0;
+            cmax = std::max(cmax, D.wcn);
+        }
+        ctx->wchunk_n = cacc;
+        ctx->wchunk_max = cmax;
+        ctx->wchunk_valid = false;
     }
     // device copies
     ANN_HIP(hipMalloc(&ctx->dm, sizeof(DevModel)));
@@ -399,6 +427,7 @@ static int upload_model(ann_ctx* ctx, const ann_model_desc* d) {
     ANN_HIP(up(ctx->da.ang_lam, alam));
     ANN_HIP(up(ctx->da.W, W));
     ANN_HIP(hipMalloc(&ctx->d_wsplit, sizeof(uint4) * std::max(ctx->wsplit_n, 1)));
+    ANN_HIP(hipMalloc(&ctx->d_wchunk, sizeof(float) * std::max(ctx->wchunk_n, 4)));
     ANN_HIP(up(ctx->da.inpw, iw));
     ANN_HIP(up(ctx->da.inpb, ib));
     ANN_HIP(up(ctx->da.inpb_net, ib));
@@ -469,7 +498,7 @@ extern "C" int ann_destroy(ann_ctx* ctx) {
                     ctx->d_Eatom, ctx->d_P, ctx->d_u, ctx->d_v, ctx->d_F, ctx->d_E, ctx->d_gE, ctx->d_atomvir,
                     ctx->d_scr, ctx->d_gpart, ctx->d_rank, ctx->d_spcount, ctx->d_perm, ctx->d_tiles, ctx->d_spend, ctx->d_grad_tiles, ctx->d_gsplit,
                     ctx->d_chunks, ctx->d_chunk_off, ctx->d_part, ctx->d_pos_ref, ctx->d_keatom, ctx->d_md_flag,
-                    ctx->d_bin_struc, ctx->d_binrec, ctx->d_adam_t, ctx->d_wsplit, ctx->d_T,
+                    ctx->d_bin_struc, ctx->d_binrec, ctx->d_adam_t, ctx->d_wsplit, ctx->d_wchunk, ctx->d_T,
                     ctx->da.pitems, ctx->da.preflag, ctx->da.inpb64, ctx->da.inpb_net, ctx->da.rad_rs64,
                     ctx->da.rad_eta64, ctx->da.ang_eta64, ctx->da.ang_zeta64, ctx->d_items, ctx->d_qoff,
                     ctx->d_queue};
@@ -898,7 +927,7 @@ extern "C" int ann_energy_forces(ann_ctx* ctx, int form, int with_evac, double* 
         (uint64_t)(uintptr_t)ctx->d_perm,
         (uint64_t)(uintptr_t)ctx->d_T, (uint64_t)(uintptr_t)ctx->d_chunks, (uint64_t)ctx->nchunks,
         (uint64_t)ctx->tune_apb[0], (uint64_t)ctx->tune_apb[1], (uint64_t)ctx->tune_apb[2],
-        (uint64_t)ctx->tune_angular, (uint64_t)ctx->tune_queue, (uint64_t)ctx->wsplit_valid, ctx->batch_gen,
+        (uint64_t)ctx->tune_angular, (uint64_t)ctx->tune_queue, (uint64_t)ctx->tune_net, (uint64_t)ctx->wsplit_valid, (uint64_t)ctx->wchunk_valid, ctx->batch_gen,
         (uint64_t)ctx->ntiles,
         (uint64_t)(uintptr_t)ctx->d_tiles, ctx->scaling_gen, (uint64_t)ctx->npre};
     if (!ctx->ef_exec || key != ctx->ef_key) {
@@ -1163,6 +1192,17 @@ extern "C" int ann_autotune(ann_ctx* ctx, int form) {
         if (e != hipSuccess) return fail(ANN_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
         ctx->tune_angular = tt < best_total ? 1 : 0;
     }
+    // network kernel (TRIPLET, FAST): vector-ALU tiles against the matrix cores,
+    // timed as whole E+F steps
+    if (form != ANN_FORM_PLAIN && vec_net_supported(ctx) && mfma_net_supported(M)) {
+        double t[2];
+        for (int q = 0; q < 2; ++q) {
+            ctx->tune_net = q;
+            t[q] = time_ms(ctx, run_ef, form, 3, e);
+            if (e != hipSuccess) return fail(ANN_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
+        }
+        ctx->tune_net = t[1] < t[0] ? 1 : 0;
+    }
     // neighbour search blocking
     {
         double t[2];
@@ -1187,6 +1227,7 @@ extern "C" int ann_get_tuning(ann_ctx* ctx, int* t, int n) {
     t[0] = ctx->tune_apb[0]; t[1] = ctx->tune_apb[1]; t[2] = ctx->tune_apb[2];
     t[3] = ctx->tune_angular; t[4] = ctx->tune_sb;
     if (n > 5) t[5] = ctx->tune_queue;
+    if (n > 6) t[6] = ctx->tune_net;
     return ANN_OK;
 }
 
@@ -1195,13 +1236,15 @@ extern "C" int ann_set_tuning(ann_ctx* ctx, const int* t, int n) {
     if (!t || n < 5) return fail(ANN_ERR_ARG, "tuning needs 5 entries");
     for (int q = 0; q < 3; ++q)
         if (t[q] != 1 && t[q] != 2 && t[q] != 4) return fail(ANN_ERR_ARG, "atoms per workgroup must be 1, 2 or 4");
-    if ((t[3] != 0 && t[3] != 1) || (t[4] != 0 && t[4] != 1) || (n > 5 && t[5] != 0 && t[5] != 1))
+    if ((t[3] != 0 && t[3] != 1) || (t[4] != 0 && t[4] != 1) || (n > 5 && t[5] != 0 && t[5] != 1) ||
+        (n > 6 && t[6] != 0 && t[6] != 1))
         return fail(ANN_ERR_ARG, "invalid tuning switch");
     for (int q = 0; q < 3; ++q) ctx->tune_apb[q] = t[q];
     ctx->tune_angular = t[3];
     if (ctx->tune_sb != t[4]) ctx->grid_valid = false;
     ctx->tune_sb = t[4];
     if (n > 5) ctx->tune_queue = t[5];
+    if (n > 6) ctx->tune_net = t[6];
     ctx->gen++;
     return ANN_OK;
 }

@@ -67,6 +67,12 @@ struct NetDesc {
     int inoff;                                 // into inpw / inpb
     int wsf[kMaxLayers], wsb[kMaxLayers];      // hidden layer l: BF16x3 operand tables of W_l and W_l^T
                                                // (16-byte units into the split-weight array)
+    int wc0;                                   // start of this species' vector-ALU tables (floats)
+    int wcf[kMaxLayers];                       // hidden layer l: rows of W_l (floats into the table array)
+    int wcs[kMaxLayers];                       // hidden layer l: row stride
+    int wcb[kMaxLayers];                       // hidden layer l: its biases
+    int wco;                                   // output neuron: weights (padded to 4), then its bias
+    int wcn;                                   // floats of this species' tables (a multiple of 4)
     float c, outw, outb;
     double evac;
 };
@@ -227,6 +233,10 @@ struct ann_ctx {
     uint4* d_wsplit = nullptr;        // hidden-layer weights as MFMA operand tables (three BF16 terms)
     int wsplit_n = 0;                 // 16-byte entries
     bool wsplit_valid = false;        // the tables match the FP32 working copy of the weights
+    float* d_wchunk = nullptr;        // network parameters in the vector-ALU kernel's layout
+    int wchunk_n = 0;                 // floats
+    int wchunk_max = 0;               // floats of the largest species
+    bool wchunk_valid = false;        // those tables match the FP32 working copy of the weights
     int* h_rb = nullptr;              // pinned host words written by kernels for the host to read after a sync
     int rb_seq = 0;                   // completion marker of the last neighbour-build readback
     int* d_rb = nullptr;              // the same words as seen from the device
@@ -253,6 +263,7 @@ struct ann_ctx {
     int tune_angular = 0;             // 0: moment form where it applies, 1: triplet tiles
     int tune_sb = 0;                  // neighbour search blocks: 0 heuristic (2x2x2 bins where they fit), 1 single bins
     int tune_queue = 0;               // moment kernels: 0 grid dispatch, 1 persistent work queues
+    int tune_net = 0;                 // networks (TRIPLET, FAST): 0 vector-ALU tiles where they fit, 1 matrix cores
     // persistent dispatch: (first atom, count) items per XCD queue, built at ann_set_batch
     int2* d_items = nullptr;
     int* d_qoff = nullptr;

@@ -170,9 +170,16 @@ hipError_t net_fwd_plain(ann_ctx* ctx, const float* G, float* Eatom, float* dEdG
 
 hipError_t net_fwd_mfma(ann_ctx* ctx, const float* G, float* Eatom, float* dEdG, const float* inpb);
 bool mfma_net_supported(const DevModel& M);
+hipError_t net_fwd_vec(ann_ctx* ctx, const float* G, float* Eatom, float* dEdG, const float* inpb);
+bool vec_net_supported(const ann_ctx* ctx);
 
+// TRIPLET and FAST: the vector-ALU tile kernel where the species' weights fit
+// its LDS budget (unless the tuning asks for the matrix cores), else the
+// matrix-core kernel; PLAIN: one GEMV chain per atom
 hipError_t net_fwd(ann_ctx* ctx, int form, const float* G, float* Eatom, float* dEdG, bool centred) {
     const float* inpb = centred ? ctx->da.inpb_net : ctx->da.inpb;
+    if (form != ANN_FORM_PLAIN && ctx->tune_net == 0 && vec_net_supported(ctx))
+        return net_fwd_vec(ctx, G, Eatom, dEdG, inpb);
     if (form != ANN_FORM_PLAIN && mfma_net_supported(ctx->hm)) return net_fwd_mfma(ctx, G, Eatom, dEdG, inpb);
     return net_fwd_plain(ctx, G, Eatom, dEdG, inpb);
 }

@@ -15,6 +15,14 @@ enum { NEU_LINEAR = 1, NEU_SIGMOID = 2, NEU_TANH = 3, NEU_ISRU = 4, NEU_ARCTAN =
        NEU_TANHRE = 8, NEU_SQRE = 9, NEU_SWISH = 10, NEU_GELU = 11, NEU_MISH = 12, NEU_PFLU = 13, NEU_SERF = 14,
        NEU_LOGISH = 15, NEU_SOFTPLUS = 16, NEU_SQPLUS = 17, NEU_ATISH = 18 };
 
+// Layout of a hidden layer's weights for the vector-ALU network (net_vec.hip):
+// rows k = 0 .. ni-1 of W[k][j], j contiguous, with an odd row stride, so that
+// lanes over the neurons (forward: one row) and lanes over the inputs
+// (backward: one column) both read distinct LDS banks.
+__host__ __device__ inline int vec_row_stride(int no) { return no | 1; }
+// floats of a region of n, rounded to whole 16-byte units
+__host__ __device__ inline int vec_pad4(int n) { return (n + 3) / 4 * 4; }
+
 // every neuron of NN::Neuron (src/ml/nn.hpp:83-107) except UNKNOWN and TEST
 __host__ inline bool neuron_supported(int n) { return n >= NEU_LINEAR && n <= NEU_ATISH; }
 
@@ -182,6 +190,41 @@ __device__ __forceinline__ void act_f(int neuron, T c, T z, T& a, T& d, T& d2) {
             if (D2) d2 = T(0.0);
             break;
     }
+}
+
+// act_f over M values with the neuron's switch taken once: each case holds the
+// M evaluations of one neuron, unrolled.  Inlining the per-value switch M
+// times instead makes the hot loop of a network kernel tens of kilobytes long
+// (past the instruction cache) and branches through the case chain per value.
+template <bool D2, int M>
+__device__ __forceinline__ void act_many(int neuron, float c, const float (&z)[M], float (&a)[M], float (&d)[M],
+                                         float (&d2)[M]) {
+#define ANN_ACT_CASE(NEU)                                                        \
+    case NEU:                                                                    \
+        _Pragma("unroll") for (int q = 0; q < M; ++q) act_f<D2>(NEU, c, z[q], a[q], d[q], d2[q]); \
+        break;
+    switch (neuron) {
+        ANN_ACT_CASE(NEU_SIGMOID)
+        ANN_ACT_CASE(NEU_TANH)
+        ANN_ACT_CASE(NEU_ISRU)
+        ANN_ACT_CASE(NEU_ARCTAN)
+        ANN_ACT_CASE(NEU_RELU)
+        ANN_ACT_CASE(NEU_ELU)
+        ANN_ACT_CASE(NEU_TANHRE)
+        ANN_ACT_CASE(NEU_SQRE)
+        ANN_ACT_CASE(NEU_SWISH)
+        ANN_ACT_CASE(NEU_GELU)
+        ANN_ACT_CASE(NEU_MISH)
+        ANN_ACT_CASE(NEU_PFLU)
+        ANN_ACT_CASE(NEU_SERF)
+        ANN_ACT_CASE(NEU_LOGISH)
+        ANN_ACT_CASE(NEU_SOFTPLUS)
+        ANN_ACT_CASE(NEU_SQPLUS)
+        ANN_ACT_CASE(NEU_ATISH)
+        default:
+            ANN_ACT_CASE(NEU_LINEAR)
+    }
+#undef ANN_ACT_CASE
 }
 
 }  // namespace ann

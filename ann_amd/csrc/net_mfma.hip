@@ -19,6 +19,7 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 hipError_t species_perm(ann_ctx* ctx);
+hipError_t chunk_weights(ann_ctx* ctx);
 
 constexpr int kTile = 16;
 constexpr int kMaxK = 128;     // padded input width supported (n_in <= 128)
@@ -146,7 +147,10 @@ hipError_t split_weights(ann_ctx* ctx, int* tick) {
     k_wsplit<<<grid, 256, 0, ctx->stream>>>(ctx->dm, ctx->da.W, ctx->d_wsplit, tick);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) ctx->wsplit_valid = true;
-    return e;
+    // the vector-ALU network's tables follow every change of the weights (a
+    // graph that captures a parameter update rebuilds both on every replay)
+    ctx->wchunk_valid = false;
+    return e == hipSuccess ? chunk_weights(ctx) : e;
 }
 
 // two wavefronts per 16-atom single-species tile: the column tiles of every
@@ -203,12 +207,15 @@ __global__ __launch_bounds__(128) void k_net_mfma(MfmaArgs A) {
             if ((ct & 1) != wv) continue;
             const f32x4 acc = accs[ct];
             const int n = ct * 16 + col;
+            const float bn = n < no ? bl[n] : 0.0f;
+            float z[4], av[4], dv[4], d2[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) z[i] = acc[i] + bn;
+            act_many<false, 4>(D.neuron, D.c, z, av, dv, d2);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                float av = 0.f, dv = 0.f, d2;
-                if (n < no) act_f<false>(D.neuron, D.c, acc[i] + bl[n], av, dv, d2);
-                out[ct][i] = av;
-                sdz[l][rb + i][n < kMaxWidth ? n : kMaxWidth] = dv;
+                out[ct][i] = n < no ? av[i] : 0.0f;
+                sdz[l][rb + i][n < kMaxWidth ? n : kMaxWidth] = n < no ? dv[i] : 0.0f;
             }
         }
         __syncthreads();

@@ -328,15 +328,19 @@ __global__ __launch_bounds__(128) void k_bwd2_mfma(Bwd2Args A, const int* __rest
             if (ct * 16 >= no) break;
             const int n = ct * 16 + c16;
             const f32x4 az = AZ[h], ad = AD[h];
+            const float bn = n < no ? bl[n] : 0.0f;
+            float z[4], av[4], dv[4], d2[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) z[i] = az[i] + bn;
+            act_many<true, 4>(D.neuron, D.c, z, av, dv, d2);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                float av = 0.f, dv = 0.f, d2 = 0.f;
-                if (n < no) act_f<true>(D.neuron, D.c, az[i] + bl[n], av, dv, d2);
-                oa[h][i] = av;
-                od[h][i] = dv * ad[i];
+                if (n >= no) av[i] = dv[i] = d2[i] = 0.0f;
+                oa[h][i] = av[i];
+                od[h][i] = dv[i] * ad[i];
                 if (n < no) {
-                    sdz[(l * kB2Tile + rb + i) * WP + n] = dv;
-                    sqz[(l * kB2Tile + rb + i) * WP + n] = d2 * ad[i];
+                    sdz[(l * kB2Tile + rb + i) * WP + n] = dv[i];
+                    sqz[(l * kB2Tile + rb + i) * WP + n] = d2[i] * ad[i];
                 }
             }
         }

@@ -402,9 +402,10 @@ class Engine:
         return self.tuning()
 
     def tuning(self) -> list:
-        """[atoms per workgroup G, adjoint, tangent; triplet tiles; single-bin search; persistent queues]"""
-        t = (C.c_int * 6)()
-        _check(lib().ann_get_tuning(self.h, t, 6))
+        """[atoms per workgroup G, adjoint, tangent; triplet tiles; single-bin search; persistent queues;
+        matrix-core network]"""
+        t = (C.c_int * 7)()
+        _check(lib().ann_get_tuning(self.h, t, 7))
         return list(t)
 
     def set_tuning(self, t) -> None:

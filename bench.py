@@ -204,7 +204,7 @@ def run_train(steps, warmup, form, graph=False, kernels=False):
             eng.train_step(Eref, Fref, 1.0, 0.1, 1e-3, form, out=out)
         kern = {}
         for k in ("symm_fwd_mom", "symm_bwd_mom", "symm_tan_mom", "symm_fwd_fast", "symm_bwd_fast", "symm_tan_fast",
-                  "net_mfma", "net_bwd2", "grad_reduce", "loss", "adam", "force_gather", "struc_sums"):
+                  "net_vec", "net_mfma", "net_bwd2", "grad_reduce", "loss", "adam", "force_gather", "struc_sums"):
             ms, n = eng.profile_get(k)
             if n:
                 kern[k] = {"avg_us": ms / n * 1e3, "launches": n}
@@ -546,8 +546,8 @@ def main():
         time_ef(eng, form, max(3, args.steps // 2), args.warmup, out)
         kern = {}
         for k in ("nb_bin", "nb_search", "nb_compact", "nb_rev", "symm_fwd_plain", "symm_fwd_fast", "symm_fwd_mom",
-                  "symm_bwd_plain", "symm_bwd_fast", "symm_bwd_mom", "net_plain", "net_mfma", "force_gather",
-                  "struc_sums"):
+                  "symm_bwd_plain", "symm_bwd_fast", "symm_bwd_mom", "net_plain", "net_vec", "net_mfma",
+                  "force_gather", "struc_sums"):
             ms, n = eng.profile_get(k)
             if n:
                 kern[k] = {"avg_us": ms / n * 1e3, "launches": n}
@@ -566,6 +566,22 @@ def main():
         res[name] = {"ms_per_step": dt * 1e3, "atoms_per_s": batch.ntotal / dt, "kernels": kern}
         E, F, _ = eng.energy_forces(form)
         forces[name] = (E.cpu().numpy(), F.cpu().numpy())
+    # the two network kernels on the same E+F steps: vector-ALU tiles against
+    # matrix-core tiles (the autotuner keeps the faster)
+    net_kernels = {}
+    for q, key in ((0, "net_vec"), (1, "net_mfma")):
+        t = list(tuning) + [0] * (7 - len(tuning))
+        t[6] = q
+        eng.set_tuning(t)
+        eng.profile(True)
+        eng.profile_only(key)
+        time_ef(eng, FAST, 5, 1, out)
+        ms, n = eng.profile_get(key)
+        eng.profile(False)
+        eng.profile_only("")
+        if n:
+            net_kernels[key] = {"avg_us": ms / n * 1e3, "launches": n}
+    eng.set_tuning(tuning)
     # dominant kernel of the FAST form, its duration measured inside the timed region
     dom, dom_us = live["fast"]
     roofs = {name: kernel_rooflines(res[name]["kernels"], flops) for name in res}
@@ -644,7 +660,9 @@ def main():
         "kernel_rooflines": roofs,
         "tuning": {"atoms_per_workgroup": {"G": tuning[0], "adjoint": tuning[1], "tangent": tuning[2]},
                    "angular": "triplet" if tuning[3] else "moment", "search_blocks": "single" if tuning[4] else "2x2x2",
-                   "moment_dispatch": "persistent queues" if len(tuning) > 5 and tuning[5] else "grid"},
+                   "moment_dispatch": "persistent queues" if len(tuning) > 5 and tuning[5] else "grid",
+                   "network": "matrix cores" if len(tuning) > 6 and tuning[6] else "vector ALU"},
+        "network_kernels": net_kernels,
         "train": train,
         "uneven_batch": uneven,
         "md": md,

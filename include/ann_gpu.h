@@ -282,11 +282,14 @@ int ann_input_stats(ann_ctx* ctx, const float* d_G, int* count, double* stats, i
  * for G, adjoint and tangent; moment form against triplet tiles; grid
  * dispatch of the moment kernels against persistent work queues (per XCD,
  * (structure, atom-range) items, largest structure first); neighbour search
- * in 2x2x2-bin or single-bin blocks.  Results do not depend on the choice
- * beyond FP32 summation order.  Tuning vector (5 or 6 ints): atoms per
- * workgroup for G, adjoint, tangent; angular kernels (0 moment, 1 triplet);
- * search blocks (0 heuristic, 1 single bins); moment-kernel dispatch (0 grid,
- * 1 persistent queues).  No counterpart in the reference. */
+ * in 2x2x2-bin or single-bin blocks; the networks (TRIPLET and FAST) as
+ * vector-ALU tiles of 16 atoms against matrix-core tiles.  Results do not
+ * depend on the choice beyond FP32 summation order.  Tuning vector (5, 6 or 7
+ * ints): atoms per workgroup for G, adjoint, tangent; angular kernels
+ * (0 moment, 1 triplet); search blocks (0 heuristic, 1 single bins);
+ * moment-kernel dispatch (0 grid, 1 persistent queues); network kernel
+ * (0 vector ALU where the species' weights fit its 64 KiB of LDS, else matrix
+ * cores; 1 matrix cores).  No counterpart in the reference. */
 int ann_autotune(ann_ctx* ctx, int form);
 int ann_get_tuning(ann_ctx* ctx, int* tuning, int n);
 int ann_set_tuning(ann_ctx* ctx, const int* tuning, int n);
