@@ -491,12 +491,14 @@ __global__ void k_compact(int N, const double* __restrict__ pos, const int* __re
 __global__ void k_rev(int N, const int* __restrict__ idx, const uint32_t* __restrict__ img,
                       const int* __restrict__ off, int* __restrict__ rev, int* __restrict__ flag,
                       const int* __restrict__ order, int cap) {
-    // centres in bin order (see k_compact)
-    const int b = xcd_block(blockIdx.x, gridDim.x);
+    // one wavefront per centre, the centres of a workgroup consecutive in bin
+    // order (see k_compact): their partner rows overlap, so the searches of a
+    // workgroup share the lines they bring into its CU's cache
+    const int b = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
     if (b >= N || off[N] > cap) return;
     const int i = order[b];
     const int o = off[i], n = off[i + 1] - o;
-    for (int t = threadIdx.x; t < n; t += blockDim.x) {
+    for (int t = threadIdx.x % kWave; t < n; t += kWave) {
         const int j = idx[o + t];
         const uint32_t p = img[o + t];
         int n0, n1, n2;
@@ -751,7 +753,9 @@ static void launch_compact(ann_ctx* ctx, int cap) {
     // a missing reverse slot (impossible for the symmetric FP64 test) is
     // reported by the next synchronising call (ann_nlist_size, next build)
     Launch L(ctx, "nb_rev");
-    k_rev<<<N, 128, 0, st>>>(N, nl.idx, nl.img, nl.off, nl.rev, ctx->d_flag + 2, ctx->d_bin_atoms, cap);
+    constexpr int kRevRows = 4;                 // centres (wavefronts) per workgroup
+    k_rev<<<(N + kRevRows - 1) / kRevRows, kRevRows * kWave, 0, st>>>(N, nl.idx, nl.img, nl.off, nl.rev, ctx->d_flag + 2,
+                                                                      ctx->d_bin_atoms, cap);
 }
 
 hipError_t neighbors_build(ann_ctx* ctx, double rc) {
