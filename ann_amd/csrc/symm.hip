@@ -876,6 +876,17 @@ __global__ __launch_bounds__(64) void k_fast(SymmArgs A) {
 // C(4,2) = 6 for zeta <= 4; larger zeta keeps the triplet kernels.
 
 __host__ __device__ constexpr int nmono(int Z) { return (Z + 1) * (Z + 2) * (Z + 3) / 6; }
+// neighbours per chunk of power rows (G and adjoint: 64, which frees LDS for
+// more resident wavefronts; tangent: every neighbour at once, measured faster
+// with its two-level rows), and the floats of the region that holds them (it
+// also holds the cutoff pairs, two floats per neighbour, before the powers are
+// needed; with stored moments only those)
+constexpr int kPwRows = 64;
+__host__ __device__ inline int pw_chunk(int mode, int maxn) { return mode == 2 ? maxn : kPwRows; }
+__host__ __device__ inline int pw_floats(int mode, int maxn, int pws, bool haveT) {
+    const int rows = pw_chunk(mode, maxn);
+    return haveT ? 2 * maxn : (rows * pws > 2 * maxn ? rows * pws : 2 * maxn);
+}
 __host__ __device__ constexpr int mono_start(int m) { return m * (m + 1) * (m + 2) / 6; }
 __host__ __device__ constexpr int ifact(int n) { return n <= 1 ? 1 : n * ifact(n - 1); }
 // position of x^a y^b z^c in the monomial order below
@@ -932,8 +943,8 @@ __device__ __forceinline__ void moment_atom(const SymmArgs& A, const int i, floa
     // adjoint with stored moments: no power rows (the cutoff pairs still use their space)
     const bool haveT = MODE == 1 && A.Tin != nullptr;
     const int TS = S * U4 * NM;                                 // moments per atom
-    float* pw = sP + (MODE == 1 ? 3 * maxn : 0);                // [maxn][PWS]
-    float* T = pw + (size_t)maxn * (haveT ? 2 : PWS);           // [S][U4][NM]
+    float* pw = sP + (MODE == 1 ? 3 * maxn : 0);                // [pw_chunk][PWS] (one chunk of neighbours)
+    float* T = pw + pw_floats(MODE, maxn, PWS, haveT);         // [S][U4][NM]
     float* dT = T + S * U4 * NM;                                // [S][U4][NM] (mode 2)
     float* D = dT + (MODE == 2 ? S * U4 * NM : 0);              // [S][U4]
     float* dD = D + S * U4;                                     // [S][U4] (mode 2: dD, mode 1: dE/dD)
@@ -984,28 +995,33 @@ __device__ __forceinline__ void moment_atom(const SymmArgs& A, const int i, floa
         for (int q = kPreT * kWave + lane; q < TS; q += kWave) T[q] = A.Tin[(size_t)i * TS + q];
     }
     // powers of the unit vector (and, for the tangent, their derivatives along
-    // the projected direction (t - r^ (r^.t)) / r)
-    for (int J = lane; J < (haveT ? 0 : n); J += kWave) {
-        const float4 gq = sm.g[J];
-        float* row = pw + (size_t)J * PWS;
-        const float v3[3] = {gq.x, gq.y, gq.z};
-        float d3[3] = {0.f, 0.f, 0.f};
-        if (MODE == 2) {
-            const float4 t4 = tv[J];
-            d3[0] = (t4.x - gq.x * t4.w) * gq.w;
-            d3[1] = (t4.y - gq.y * t4.w) * gq.w;
-            d3[2] = (t4.z - gq.z * t4.w) * gq.w;
-        }
+    // the projected direction (t - r^ (r^.t)) / r) of neighbours c0 .. c1-1,
+    // at most pw_chunk, into the chunk rows (the moment loop below walks each
+    // species segment in such chunks, so for G and the adjoint the rows take a
+    // fixed slice of LDS instead of one row per neighbour)
+    auto power_rows = [&](int c0, int c1) {
+        for (int J = c0 + lane; J < c1; J += kWave) {
+            const float4 gq = sm.g[J];
+            float* row = pw + (size_t)(J - c0) * PWS;
+            const float v3[3] = {gq.x, gq.y, gq.z};
+            float d3[3] = {0.f, 0.f, 0.f};
+            if (MODE == 2) {
+                const float4 t4 = tv[J];
+                d3[0] = (t4.x - gq.x * t4.w) * gq.w;
+                d3[1] = (t4.y - gq.y * t4.w) * gq.w;
+                d3[2] = (t4.z - gq.z * t4.w) * gq.w;
+            }
 #pragma unroll
-        for (int x = 0; x < 3; ++x) {
-            float p = 1.0f;
+            for (int x = 0; x < 3; ++x) {
+                float p = 1.0f;
 #pragma unroll
-            for (int k = 0; k < ZP; ++k) {
-                if (MODE == 2) row[3 * ZP + x * ZP + k] = k == 0 ? 0.0f : (float)k * p * d3[x];
-                row[x * ZP + k] = k == 0 ? 1.0f : (p *= v3[x]);
+                for (int k = 0; k < ZP; ++k) {
+                    if (MODE == 2) row[3 * ZP + x * ZP + k] = k == 0 ? 0.0f : (float)k * p * d3[x];
+                    row[x * ZP + k] = k == 0 ? 1.0f : (p *= v3[x]);
+                }
             }
         }
-    }
+    };
     wave_sync();
     // moments T_b[u][alpha]: lane = (alpha, group); groups stride the neighbours.
     // G with zeta up to 4 (35 monomials, one group would leave 29 lanes idle):
@@ -1023,35 +1039,42 @@ __device__ __forceinline__ void moment_atom(const SymmArgs& A, const int i, floa
             float acc[U4], dacc[U4];
 #pragma unroll
             for (int u = 0; u < U4; ++u) { acc[u] = 0.0f; dacc[u] = 0.0f; }
-            if (act) {
-                // one neighbour of the segment: T[u][alpha] += e_u(r) r^^alpha
-                auto step = [&](const float* row, int J) {
-                    const float px = row[ea], py = row[ZP + eb], pz = row[2 * ZP + ec];
-                    const float mono = px * py * pz;
-                    const float4 e4 = sm.ef[J];
-                    acc[0] += e4.x * mono; acc[1] += e4.y * mono; acc[2] += e4.z * mono; acc[3] += e4.w * mono;
-                    if (MODE == 2) {
-                        const float dmono = row[3 * ZP + ea] * py * pz + px * row[4 * ZP + eb] * pz +
-                                            px * py * row[5 * ZP + ec];
-                        const float4 f4 = sm.efd[J];
-                        const float wm = tv[J].w * mono;
-                        dacc[0] += f4.x * wm + e4.x * dmono;
-                        dacc[1] += f4.y * wm + e4.y * dmono;
-                        dacc[2] += f4.z * wm + e4.z * dmono;
-                        dacc[3] += f4.w * wm + e4.w * dmono;
-                    }
-                };
-                int J = s0 + grp;
-                // four neighbours per trip: their power rows sit at fixed offsets
-                // from one address, so the loads use immediate offsets and overlap
-                for (; J + 3 * NGL < s1; J += 4 * NGL) {
-                    const float* row = pw + (size_t)J * PWS;
-                    step(row, J);
-                    step(row + NGL * PWS, J + NGL);
-                    step(row + 2 * NGL * PWS, J + 2 * NGL);
-                    step(row + 3 * NGL * PWS, J + 3 * NGL);
+            // one neighbour of the segment: T[u][alpha] += e_u(r) r^^alpha
+            auto step = [&](const float* row, int J) {
+                const float px = row[ea], py = row[ZP + eb], pz = row[2 * ZP + ec];
+                const float mono = px * py * pz;
+                const float4 e4 = sm.ef[J];
+                acc[0] += e4.x * mono; acc[1] += e4.y * mono; acc[2] += e4.z * mono; acc[3] += e4.w * mono;
+                if (MODE == 2) {
+                    const float dmono = row[3 * ZP + ea] * py * pz + px * row[4 * ZP + eb] * pz +
+                                        px * py * row[5 * ZP + ec];
+                    const float4 f4 = sm.efd[J];
+                    const float wm = tv[J].w * mono;
+                    dacc[0] += f4.x * wm + e4.x * dmono;
+                    dacc[1] += f4.y * wm + e4.y * dmono;
+                    dacc[2] += f4.z * wm + e4.z * dmono;
+                    dacc[3] += f4.w * wm + e4.w * dmono;
                 }
-                for (; J < s1; J += NGL) step(pw + (size_t)J * PWS, J);
+            };
+            const int chunk = pw_chunk(MODE, maxn);
+            for (int c0 = s0; c0 < s1; c0 += chunk) {
+                const int c1 = min(c0 + chunk, s1);
+                wave_sync();                                    // the previous chunk's rows are read
+                power_rows(c0, c1);
+                wave_sync();
+                if (act) {
+                    int J = c0 + grp;
+                    // four neighbours per trip: their power rows sit at fixed offsets
+                    // from one address, so the loads use immediate offsets and overlap
+                    for (; J + 3 * NGL < c1; J += 4 * NGL) {
+                        const float* row = pw + (size_t)(J - c0) * PWS;
+                        step(row, J);
+                        step(row + NGL * PWS, J + NGL);
+                        step(row + 2 * NGL * PWS, J + 2 * NGL);
+                        step(row + 3 * NGL * PWS, J + 3 * NGL);
+                    }
+                    for (; J < c1; J += NGL) step(pw + (size_t)(J - c0) * PWS, J);
+                }
             }
             if (SPLIT) {
 #pragma unroll
@@ -1334,7 +1357,8 @@ static size_t moment_smem(const SymmArgs& A, int mode, int Z, int S, int NP) {
     const int NG = kWave / NM;
     size_t b = 16 * (size_t)A.maxn * (2 + (mode == 0 ? 0 : 1) + (mode == 2 ? 1 : 0));
     b += 4 * 2 * (size_t)A.maxn + 4 * 16;
-    size_t f = (size_t)A.ldg + (mode == 1 ? 3 * (size_t)A.maxn : 0) + (size_t)A.maxn * PWS;
+    size_t f = (size_t)A.ldg + (mode == 1 ? 3 * (size_t)A.maxn : 0) +
+               (size_t)pw_floats(mode, A.maxn, PWS, mode == 1 && A.Tin);
     f += (size_t)S * U4 * NM * (mode == 2 ? 2 : 1) + 2 * (size_t)S * U4 + (size_t)NP * U4 * ZP;
     f += (mode == 1 ? (size_t)S * NM * U4 : 0);
     f += (NG > 1 ? (size_t)NG * NM * U4 * (mode == 2 ? 2 : 1) : 0);
