@@ -377,7 +377,7 @@ __device__ __forceinline__ int fast_load(const SymmArgs& A, const DevModel& M, i
                 tv[pos] = make_float4(tx, ty, tz, (gq.x * ir) * tx + (gq.y * ir) * ty + (gq.z * ir) * tz);
             }
             sm.r[pos] = r;
-            sm.canon[pos] = t;
+            if (sm.canon) sm.canon[pos] = t;                     // slot map (kernels that need it)
             float f, gcut;
             cutoff_fg(M.cutoff, M.rc, M.rci, M.pirc, r, f, gcut);
             sm.fg[pos] = make_float2(f, gcut);
@@ -876,13 +876,13 @@ __global__ __launch_bounds__(64) void k_fast(SymmArgs A) {
 // C(4,2) = 6 for zeta <= 4; larger zeta keeps the triplet kernels.
 
 __host__ __device__ constexpr int nmono(int Z) { return (Z + 1) * (Z + 2) * (Z + 3) / 6; }
-// neighbours per chunk of power rows (G and adjoint: 64, which frees LDS for
-// more resident wavefronts; tangent: every neighbour at once, measured faster
-// with its two-level rows), and the floats of the region that holds them (it
+// neighbours per chunk of power rows (G: 32 and adjoint: 64, which frees LDS
+// for more resident wavefronts; tangent: every neighbour at once, measured
+// faster with its two-level rows), and the floats of the region that holds them (it
 // also holds the cutoff pairs, two floats per neighbour, before the powers are
 // needed; with stored moments only those)
 constexpr int kPwRows = 64;
-__host__ __device__ inline int pw_chunk(int mode, int maxn) { return mode == 2 ? maxn : kPwRows; }
+__host__ __device__ inline int pw_chunk(int mode, int maxn) { return mode == 2 ? maxn : (mode == 0 ? kPwRows / 2 : kPwRows); }
 __host__ __device__ inline int pw_floats(int mode, int maxn, int pws, bool haveT) {
     const int rows = pw_chunk(mode, maxn);
     return haveT ? 2 * maxn : (rows * pws > 2 * maxn ? rows * pws : 2 * maxn);
@@ -935,8 +935,9 @@ __device__ __forceinline__ void moment_atom(const SymmArgs& A, const int i, floa
     sm.efd = sm.ef + maxn;
     float4* tv = sm.efd + (MODE == 0 ? 0 : maxn);
     sm.r = reinterpret_cast<float*>(tv + (MODE == 2 ? maxn : 0));
-    sm.canon = reinterpret_cast<int*>(sm.r + maxn);
-    sm.seg = sm.canon + maxn;
+    int* const after_r = reinterpret_cast<int*>(sm.r + maxn);
+    sm.canon = MODE == 1 ? after_r : nullptr;                   // the adjoint's slot map
+    sm.seg = after_r + (MODE == 1 ? maxn : 0);
     sm.extra = reinterpret_cast<float*>(sm.seg + 16);
     float* rowbuf = sm.extra;                                   // [ldg]
     float* sP = rowbuf + ldg;                                   // [maxn][3] (mode 1)
@@ -1320,6 +1321,17 @@ __global__ __launch_bounds__(256) void k_moment(SymmArgs A) {
     moment_atom<MODE, Z>(A, i, smem_all + (size_t)wave * (A.smem_wave / 16));
 }
 
+// G: the same with registers held to five wavefronts per SIMD (it fits in 96
+// VGPRs without spilling; its LDS per wavefront allows as many)
+template <int Z>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_moment_g(SymmArgs A) {
+    extern __shared__ float4 smem_all[];
+    const int wave = threadIdx.x / kWave;
+    const int i = blockIdx.x * (blockDim.x / kWave) + wave;
+    if (i >= A.N) return;
+    moment_atom<0, Z>(A, i, smem_all + (size_t)wave * (A.smem_wave / 16));
+}
+
 // Persistent dispatch (a separate kernel, so that the grid kernel keeps its
 // own register allocation): a grid sized to the resident capacity, every
 // wavefront pulls (first atom, count) items off the queue of its XCD (blocks
@@ -1356,7 +1368,7 @@ static size_t moment_smem(const SymmArgs& A, int mode, int Z, int S, int NP) {
     const int PWS = (mode == 1 && A.Tin) ? 2 : (3 * ZP * (mode == 2 ? 2 : 1)) | 1;
     const int NG = kWave / NM;
     size_t b = 16 * (size_t)A.maxn * (2 + (mode == 0 ? 0 : 1) + (mode == 2 ? 1 : 0));
-    b += 4 * 2 * (size_t)A.maxn + 4 * 16;
+    b += 4 * (mode == 1 ? 2 : 1) * (size_t)A.maxn + 4 * 16;                         // r, canon (adjoint), seg
     size_t f = (size_t)A.ldg + (mode == 1 ? 3 * (size_t)A.maxn : 0) +
                (size_t)pw_floats(mode, A.maxn, PWS, mode == 1 && A.Tin);
     f += (size_t)S * U4 * NM * (mode == 2 ? 2 : 1) + 2 * (size_t)S * U4 + (size_t)NP * U4 * ZP;
@@ -1482,6 +1494,8 @@ static hipError_t run(ann_ctx* ctx, int form, SymmArgs A) {
         void (*kern)(SymmArgs) =
             queued ? (M.moment_z == 1 ? k_moment_queue<MODE, 1> : M.moment_z == 2 ? k_moment_queue<MODE, 2>
                       : M.moment_z == 3 ? k_moment_queue<MODE, 3> : k_moment_queue<MODE, 4>)
+                   : MODE == 0 ? (M.moment_z == 1 ? k_moment_g<1> : M.moment_z == 2 ? k_moment_g<2>
+                                  : M.moment_z == 3 ? k_moment_g<3> : k_moment_g<4>)
                    : (M.moment_z == 1 ? k_moment<MODE, 1> : M.moment_z == 2 ? k_moment<MODE, 2>
                       : M.moment_z == 3 ? k_moment<MODE, 3> : k_moment<MODE, 4>);
         if (queued) {
