@@ -882,6 +882,8 @@ __host__ __device__ constexpr int nmono(int Z) { return (Z + 1) * (Z + 2) * (Z +
 // also holds the cutoff pairs, two floats per neighbour, before the powers are
 // needed; with stored moments only those)
 constexpr int kPwRows = 64;
+// adjoint: dE/dT entries per lane held in registers while they replace the moments
+constexpr int kUbRegs = 8;
 __host__ __device__ inline int pw_chunk(int mode, int maxn) { return mode == 2 ? maxn : (mode == 0 ? kPwRows / 2 : kPwRows); }
 __host__ __device__ inline int pw_floats(int mode, int maxn, int pws, bool haveT) {
     const int rows = pw_chunk(mode, maxn);
@@ -950,8 +952,10 @@ __device__ __forceinline__ void moment_atom(const SymmArgs& A, const int i, floa
     float* D = dT + (MODE == 2 ? S * U4 * NM : 0);              // [S][U4]
     float* dD = D + S * U4;                                     // [S][U4] (mode 2: dD, mode 1: dE/dD)
     float* Am = dD + S * U4;                                    // [NP][U4][ZP]
-    float* Ub = Am + NP * U4 * ZP;                              // [S][NM][U4] (mode 1)
-    float* part = Ub + (MODE == 1 ? S * NM * U4 : 0);           // [NG][NM][U4] x (1 or 2)
+    // adjoint: dE/dT overwrites the moments when it fits a few registers per lane
+    const bool ub_in_T = S * NM * U4 <= kUbRegs * kWave;
+    float* Ub = ub_in_T ? T : Am + NP * U4 * ZP;               // [S][NM][U4] (mode 1)
+    float* part = Am + NP * U4 * ZP + (MODE == 1 && !ub_in_T ? S * NM * U4 : 0);   // [NG][NM][U4] x (1 or 2)
     float* wq = part + (NG > 1 ? NG * NM * U4 * (MODE == 2 ? 2 : 1) : 0);   // [NM] monomial weights
     sm.fg = reinterpret_cast<float2*>(pw);   // cutoffs live in the power rows until the radial part is done
 
@@ -1246,14 +1250,31 @@ __device__ __forceinline__ void moment_atom(const SymmArgs& A, const int i, floa
         return;
     }
     // adjoint: dE/dT_b[alpha][u] and dE/dD_b[u]
-    for (int it = lane; it < S * NM * U4; it += kWave) {
+    auto dEdT = [&](int it) {
         const int b = it / (NM * U4), al = (it / U4) % NM, u = it % U4;
         int m = 0;   // degree of monomial al
 #pragma unroll
         for (int mm = 1; mm <= Z; ++mm) m += al >= mono_start(mm) ? 1 : 0;
         float s = 0.0f;
         for (int c = 0; c < S; ++c) s += Am[(pair_of(b, c) * U4 + u) * ZP + m] * T[(c * U4 + u) * NM + al];
-        Ub[it] = wq[al] * s;
+        return wq[al] * s;
+    };
+    if (ub_in_T) {
+        // every lane's entries in registers, then written over the moments
+        float ub[kUbRegs];
+#pragma unroll
+        for (int h = 0; h < kUbRegs; ++h) {
+            const int it = lane + h * kWave;
+            ub[h] = it < S * NM * U4 ? dEdT(it) : 0.0f;
+        }
+        wave_sync();
+#pragma unroll
+        for (int h = 0; h < kUbRegs; ++h) {
+            const int it = lane + h * kWave;
+            if (it < S * NM * U4) Ub[it] = ub[h];
+        }
+    } else {
+        for (int it = lane; it < S * NM * U4; it += kWave) Ub[it] = dEdT(it);
     }
     if (lane < S * U4) {
         const int b = lane / U4, u = lane % U4;
@@ -1372,7 +1393,7 @@ static size_t moment_smem(const SymmArgs& A, int mode, int Z, int S, int NP) {
     size_t f = (size_t)A.ldg + (mode == 1 ? 3 * (size_t)A.maxn : 0) +
                (size_t)pw_floats(mode, A.maxn, PWS, mode == 1 && A.Tin);
     f += (size_t)S * U4 * NM * (mode == 2 ? 2 : 1) + 2 * (size_t)S * U4 + (size_t)NP * U4 * ZP;
-    f += (mode == 1 ? (size_t)S * NM * U4 : 0);
+    f += (mode == 1 && S * NM * U4 > kUbRegs * kWave ? (size_t)S * NM * U4 : 0);   // dE/dT beside the moments
     f += (NG > 1 ? (size_t)NG * NM * U4 * (mode == 2 ? 2 : 1) : 0);
     f += NM;
     return b + 4 * f;
