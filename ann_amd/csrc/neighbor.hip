@@ -546,6 +546,7 @@ static hipError_t grow(T*& p, int64_t& cap, int64_t want) {
 // first wavefront, a carry across tiles.  One launch instead of the two of a
 // decoupled device scan, and the host-visible words written by the same kernel.
 constexpr int kScanThreads = 1024;
+constexpr int kScanPre = 4;             // tiles of 4 elements per thread loaded up front
 constexpr int kScanMax = 1 << 16;
 
 __global__ void k_clear2(int* __restrict__ a, int na, int* __restrict__ b, int nb) {
@@ -566,12 +567,26 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_small(const int* __restri
     const int t = threadIdx.x, lane = t % kWave, w = t / kWave;
     if (t == 0) s_carry = 0;
     __syncthreads();
-    for (int base = 0; base < n; base += 4 * kScanThreads) {
+    // the first kScanPre tiles are loaded before any is scanned (one memory
+    // latency for up to 16384 elements); later tiles load in their own trip
+    int pre[kScanPre][4];
+#pragma unroll
+    for (int p = 0; p < kScanPre; ++p)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int e = p * 4 * kScanThreads + 4 * t + q;
+            pre[p][q] = e < n ? in[e] : 0;
+        }
+    for (int base = 0, tile = 0; base < n; base += 4 * kScanThreads, ++tile) {
         int v[4], sum = 0;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int e = base + 4 * t + q;
-            v[q] = e < n ? in[e] : 0;
+            int x = 0;
+#pragma unroll
+            for (int p = 0; p < kScanPre; ++p)
+                if (p == tile) x = pre[p][q];
+            v[q] = tile < kScanPre ? x : (e < n ? in[e] : 0);
             sum += v[q];
         }
         int inc = sum;   // inclusive scan over the wavefront
