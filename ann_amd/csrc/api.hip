@@ -408,6 +408,19 @@ static int upload_model(ann_ctx* ctx, const ann_model_desc* d) {
             D.wcn = cacc - D.wc0;
             cmax = std::max(cmax, D.wcn);
         }
+        // transposed rows W_l^T[j][k] of the hidden layers (odd stride)
+        int tacc = 0;
+        for (int a = 0; a < S; ++a) {
+            NetDesc& D = M.net[a];
+            for (int l = 0; l < kMaxLayers; ++l) D.wtf[l] = D.wts[l] = 0;
+            for (int l = 0; l < D.nlayer - 1; ++l) {
+                const int ni = l == 0 ? D.nin : D.nodes[l - 1], no = D.nodes[l];
+                D.wts[l] = vec_row_stride(ni);
+                D.wtf[l] = tacc;
+                tacc += vec_pad4(no * D.wts[l]);
+            }
+        }
+        ctx->wtrans_n = tacc;
         ctx->wchunk_n = cacc;
         ctx->wchunk_max = cmax;
         ctx->wchunk_valid = false;
@@ -428,6 +441,7 @@ static int upload_model(ann_ctx* ctx, const ann_model_desc* d) {
     ANN_HIP(up(ctx->da.W, W));
     ANN_HIP(hipMalloc(&ctx->d_wsplit, sizeof(uint4) * std::max(ctx->wsplit_n, 1)));
     ANN_HIP(hipMalloc(&ctx->d_wchunk, sizeof(float) * std::max(ctx->wchunk_n, 4)));
+    ANN_HIP(hipMalloc(&ctx->d_wtrans, sizeof(float) * std::max(ctx->wtrans_n, 4)));
     ANN_HIP(up(ctx->da.inpw, iw));
     ANN_HIP(up(ctx->da.inpb, ib));
     ANN_HIP(up(ctx->da.inpb_net, ib));
@@ -498,7 +512,7 @@ extern "C" int ann_destroy(ann_ctx* ctx) {
                     ctx->d_Eatom, ctx->d_P, ctx->d_u, ctx->d_v, ctx->d_F, ctx->d_E, ctx->d_gE, ctx->d_atomvir,
                     ctx->d_scr, ctx->d_gpart, ctx->d_rank, ctx->d_spcount, ctx->d_perm, ctx->d_tiles, ctx->d_spend, ctx->d_grad_tiles, ctx->d_gsplit,
                     ctx->d_chunks, ctx->d_chunk_off, ctx->d_part, ctx->d_pos_ref, ctx->d_keatom, ctx->d_md_flag,
-                    ctx->d_bin_struc, ctx->d_binrec, ctx->d_adam_t, ctx->d_wsplit, ctx->d_wchunk, ctx->d_T,
+                    ctx->d_bin_struc, ctx->d_binrec, ctx->d_adam_t, ctx->d_wsplit, ctx->d_wchunk, ctx->d_wtrans, ctx->d_T,
                     ctx->da.pitems, ctx->da.preflag, ctx->da.inpb64, ctx->da.inpb_net, ctx->da.rad_rs64,
                     ctx->da.rad_eta64, ctx->da.ang_eta64, ctx->da.ang_zeta64, ctx->d_items, ctx->d_qoff,
                     ctx->d_queue};
@@ -892,6 +906,10 @@ extern "C" int ann_force_reduce_noalloc(ann_ctx* ctx, const float* P, const floa
 
 static hipError_t ef_body(ann_ctx* ctx, int form, int with_evac, double* E, double* F, double* V) {
     hipError_t e;
+    if ((F || V) && ctx->tune_fused == 1 && fused_supported(ctx, form)) {
+        if ((e = symm_fused(ctx, ctx->da.inpb_net, ctx->d_Eatom, ctx->d_P)) != hipSuccess) return e;
+        return force_reduce(ctx, ctx->d_P, ctx->d_Eatom, with_evac, F, E, V);
+    }
     if ((e = network_rows(ctx, form, ctx->d_G)) != hipSuccess) return e;
     if ((e = net_fwd(ctx, form, ctx->d_G, ctx->d_Eatom, F || V ? ctx->d_dEdG : nullptr, true)) != hipSuccess) return e;
     if (F || V) {
@@ -927,7 +945,8 @@ extern "C" int ann_energy_forces(ann_ctx* ctx, int form, int with_evac, double* 
         (uint64_t)(uintptr_t)ctx->d_perm,
         (uint64_t)(uintptr_t)ctx->d_T, (uint64_t)(uintptr_t)ctx->d_chunks, (uint64_t)ctx->nchunks,
         (uint64_t)ctx->tune_apb[0], (uint64_t)ctx->tune_apb[1], (uint64_t)ctx->tune_apb[2],
-        (uint64_t)ctx->tune_angular, (uint64_t)ctx->tune_queue, (uint64_t)ctx->tune_net, (uint64_t)ctx->wsplit_valid, (uint64_t)ctx->wchunk_valid, ctx->batch_gen,
+        (uint64_t)ctx->tune_angular, (uint64_t)ctx->tune_queue, (uint64_t)ctx->tune_net, (uint64_t)ctx->tune_fused,
+        (uint64_t)ctx->wsplit_valid, (uint64_t)ctx->wchunk_valid, ctx->batch_gen,
         (uint64_t)ctx->ntiles,
         (uint64_t)(uintptr_t)ctx->d_tiles, ctx->scaling_gen, (uint64_t)ctx->npre};
     if (!ctx->ef_exec || key != ctx->ef_key) {
@@ -1192,6 +1211,17 @@ extern "C" int ann_autotune(ann_ctx* ctx, int form) {
         if (e != hipSuccess) return fail(ANN_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
         ctx->tune_angular = tt < best_total ? 1 : 0;
     }
+    // fused E+F (FAST, moment form) against the separate kernels, whole E+F steps
+    ctx->tune_fused = 0;
+    if (fused_supported(ctx, form)) {
+        double t[2];
+        for (int q = 0; q < 2; ++q) {
+            ctx->tune_fused = q;
+            t[q] = time_ms(ctx, run_ef, form, 3, e);
+            if (e != hipSuccess) return fail(ANN_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
+        }
+        ctx->tune_fused = t[1] < t[0] ? 1 : 0;
+    }
     // network kernel (TRIPLET, FAST): vector-ALU tiles against the matrix cores,
     // timed as whole E+F steps
     if (form != ANN_FORM_PLAIN && vec_net_supported(ctx) && mfma_net_supported(M)) {
@@ -1228,6 +1258,7 @@ extern "C" int ann_get_tuning(ann_ctx* ctx, int* t, int n) {
     t[3] = ctx->tune_angular; t[4] = ctx->tune_sb;
     if (n > 5) t[5] = ctx->tune_queue;
     if (n > 6) t[6] = ctx->tune_net;
+    if (n > 7) t[7] = ctx->tune_fused;
     return ANN_OK;
 }
 
@@ -1237,7 +1268,7 @@ extern "C" int ann_set_tuning(ann_ctx* ctx, const int* t, int n) {
     for (int q = 0; q < 3; ++q)
         if (t[q] != 1 && t[q] != 2 && t[q] != 4) return fail(ANN_ERR_ARG, "atoms per workgroup must be 1, 2 or 4");
     if ((t[3] != 0 && t[3] != 1) || (t[4] != 0 && t[4] != 1) || (n > 5 && t[5] != 0 && t[5] != 1) ||
-        (n > 6 && t[6] != 0 && t[6] != 1))
+        (n > 6 && t[6] != 0 && t[6] != 1) || (n > 7 && t[7] != 0 && t[7] != 1))
         return fail(ANN_ERR_ARG, "invalid tuning switch");
     for (int q = 0; q < 3; ++q) ctx->tune_apb[q] = t[q];
     ctx->tune_angular = t[3];
@@ -1245,6 +1276,7 @@ extern "C" int ann_set_tuning(ann_ctx* ctx, const int* t, int n) {
     ctx->tune_sb = t[4];
     if (n > 5) ctx->tune_queue = t[5];
     if (n > 6) ctx->tune_net = t[6];
+    if (n > 7) ctx->tune_fused = t[7];
     ctx->gen++;
     return ANN_OK;
 }

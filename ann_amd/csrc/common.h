@@ -73,6 +73,8 @@ struct NetDesc {
     int wcb[kMaxLayers];                       // hidden layer l: its biases
     int wco;                                   // output neuron: weights (padded to 4), then its bias
     int wcn;                                   // floats of this species' tables (a multiple of 4)
+    int wtf[kMaxLayers];                       // hidden layer l: rows of W_l^T (floats into the transposed array)
+    int wts[kMaxLayers];                       // hidden layer l: row stride of W_l^T
     float c, outw, outb;
     double evac;
 };
@@ -234,6 +236,8 @@ struct ann_ctx {
     int wsplit_n = 0;                 // 16-byte entries
     bool wsplit_valid = false;        // the tables match the FP32 working copy of the weights
     float* d_wchunk = nullptr;        // network parameters in the vector-ALU kernel's layout
+    float* d_wtrans = nullptr;        // hidden-layer weights transposed (backward rows of the fused kernel)
+    int wtrans_n = 0;                 // floats
     int wchunk_n = 0;                 // floats
     int wchunk_max = 0;               // floats of the largest species
     bool wchunk_valid = false;        // those tables match the FP32 working copy of the weights
@@ -264,6 +268,7 @@ struct ann_ctx {
     int tune_sb = 0;                  // neighbour search blocks: 0 heuristic (2x2x2 bins where they fit), 1 single bins
     int tune_queue = 0;               // moment kernels: 0 grid dispatch, 1 persistent work queues
     int tune_net = 0;                 // networks (TRIPLET, FAST): 0 vector-ALU tiles where they fit, 1 matrix cores
+    int tune_fused = 0;               // FAST E+F in the moment form: 0 separate G / network / adjoint kernels, 1 one fused kernel
     // persistent dispatch: (first atom, count) items per XCD queue, built at ann_set_batch
     int2* d_items = nullptr;
     int* d_qoff = nullptr;
@@ -352,6 +357,9 @@ hipError_t widen_rows(ann_ctx* ctx, const float* G, double* G64);
 hipError_t centre_rows(ann_ctx* ctx, const double* G64, float* Gc);
 hipError_t symm_bwd(ann_ctx* ctx, int form, const float* dEdG, float* P);
 hipError_t symm_tangent(ann_ctx* ctx, int form, const float* u, float* v);
+bool fused_supported(const ann_ctx* ctx, int form);
+hipError_t symm_fused(ann_ctx* ctx, const float* inpb, float* Eatom, float* P);   // G + network + adjoint
+hipError_t chunk_weights(ann_ctx* ctx);
 // centred: G rows hold G + inpb at the FP64-evaluated inputs (precise_inputs);
 // the network then applies inpb_net instead of inpb
 hipError_t net_fwd(ann_ctx* ctx, int form, const float* G, float* Eatom, float* dEdG, bool centred = false);

@@ -73,11 +73,34 @@ __global__ __launch_bounds__(256) void k_wchunk(const DevModel* __restrict__ dm,
     tab[e] = v;
 }
 
+// Transposed weight rows W_l^T[j][k] (blockIdx.y = species * kMaxLayers +
+// layer), one thread per float including the padding.
+__global__ __launch_bounds__(256) void k_wtrans(const DevModel* __restrict__ dm, const float* __restrict__ W,
+                                               float* __restrict__ tab) {
+    const int a = blockIdx.y / kMaxLayers, l = blockIdx.y % kMaxLayers;
+    if (a >= dm->S) return;
+    const NetDesc& D = dm->net[a];
+    if (l >= D.nlayer - 1) return;
+    const int ni = l == 0 ? D.nin : D.nodes[l - 1], no = D.nodes[l], ps = D.wts[l];
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= vec_pad4(no * ps)) return;
+    const int j = e / ps, k = e % ps;
+    tab[D.wtf[l] + e] = (j < no && k < ni) ? W[D.woff[l] + k * no + j] : 0.0f;
+}
+
 hipError_t chunk_weights(ann_ctx* ctx) {
     const DevModel& M = ctx->hm;
     dim3 grid((std::max(ctx->wchunk_max, 1) + 255) / 256, std::max(M.S, 1));
     k_wchunk<<<grid, 256, 0, ctx->stream>>>(ctx->dm, ctx->da.W, ctx->d_wchunk);
     hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    int most = 1;
+    for (int a = 0; a < M.S; ++a)
+        for (int l = 0; l < M.net[a].nlayer - 1; ++l)
+            most = std::max(most, vec_pad4(M.net[a].nodes[l] * M.net[a].wts[l]));
+    dim3 gt((most + 255) / 256, std::max(M.S, 1) * kMaxLayers);
+    k_wtrans<<<gt, 256, 0, ctx->stream>>>(ctx->dm, ctx->da.W, ctx->d_wtrans);
+    e = hipGetLastError();
     if (e == hipSuccess) ctx->wchunk_valid = true;
     return e;
 }

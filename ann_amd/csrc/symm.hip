@@ -34,6 +34,7 @@
 #include <vector>
 
 #include "basis_dev.h"
+#include "net_dev.h"
 
 namespace ann {
 
@@ -58,6 +59,12 @@ struct SymmArgs {
     float* v;
     float* Tout;              // moment form, G: store each atom's moments T here (or null)
     const float* Tin;         // moment form, adjoint: moments of the same geometry (or null: recompute)
+    // fused E+F (mode 3): the network's parameter tables and inputs, the atom energies
+    const float* wchunk;      // forward weight rows, biases, output neuron (net_dev.h layout)
+    const float* wtrans;      // transposed weight rows (backward)
+    const float* inpw;
+    const float* inpb;
+    float* Eatom;
     // persistent dispatch of the moment kernels (or null): per XCD a queue of
     // (first atom, atom count) items, its counter at queue[32 q]
     int* queue;
@@ -884,9 +891,15 @@ __host__ __device__ constexpr int nmono(int Z) { return (Z + 1) * (Z + 2) * (Z +
 constexpr int kPwRows = 64;
 // adjoint: dE/dT entries per lane held in registers while they replace the moments
 constexpr int kUbRegs = 8;
-__host__ __device__ inline int pw_chunk(int mode, int maxn) { return mode == 2 ? maxn : (mode == 0 ? kPwRows / 2 : kPwRows); }
+__host__ __device__ inline int pw_chunk(int mode, int maxn) {
+    return mode == 2 ? maxn : (mode == 0 || mode == 3 ? kPwRows / 2 : kPwRows);
+}
 __host__ __device__ inline int pw_floats(int mode, int maxn, int pws, bool haveT) {
     const int rows = pw_chunk(mode, maxn);
+    if (mode == 3) {                                // fused: also the adjoint vectors and network vectors
+        const int m = rows * pws > 3 * maxn ? rows * pws : 3 * maxn;
+        return m > 3 * kMaxWidth ? m : 3 * kMaxWidth;
+    }
     return haveT ? 2 * maxn : (rows * pws > 2 * maxn ? rows * pws : 2 * maxn);
 }
 __host__ __device__ constexpr int mono_start(int m) { return m * (m + 1) * (m + 2) / 6; }
@@ -917,6 +930,114 @@ __constant__ MonoTable c_mono = MonoTable();
 
 __device__ __forceinline__ int pair_of(int b, int c) { return b <= c ? c * (c + 1) / 2 + b : b * (b + 1) / 2 + c; }
 
+// groups of lanes sharing a reduction of n outputs (n <= 16: four groups of
+// 16 lanes, <= 32: two of 32, else one of 64)
+__device__ __forceinline__ int lane_groups(int n) { return n <= 16 ? 4 : (n <= 32 ? 2 : 1); }
+
+// The atom's network on its own wavefront (fused E+F; ANN::fpbp + DODZ::grad,
+// src/ml/nn.cpp:1367-1389, 1801-1830): rowbuf holds the G row on entry and
+// dE/dG on exit (zero past the species' inputs); the atom energy goes to
+// A.Eatom.  Lanes over a layer's outputs, its reduction split over up to four
+// lane groups joined by butterflies; the layer's inputs are broadcast LDS
+// reads, the weights coalesced reads of the row tables (forward: rows of W,
+// backward: rows of W^T, net_dev.h layout).  nb: [3][kMaxWidth] LDS floats.
+__device__ __forceinline__ void fused_net(const SymmArgs& A, const NetDesc& D, int i, float* rowbuf, float* nb) {
+    const int lane = threadIdx.x % kWave;
+    const int L = D.nlayer, nin = D.nin, ldg = A.ldg;
+    const float* iw = A.inpw + D.inoff;
+    const float* ib = A.inpb + D.inoff;
+    for (int k = lane; k < nin; k += kWave) rowbuf[k] = iw[k] * (rowbuf[k] + ib[k]);
+    wave_sync();
+    const float* x = rowbuf;
+    int ni = nin;
+    float sp[kMaxLayers - 1];                               // sigma' of the lane's neuron per hidden layer
+#pragma unroll
+    for (int l = 0; l < kMaxLayers - 1; ++l) sp[l] = 0.0f;
+#pragma unroll
+    for (int l = 0; l < kMaxLayers - 1; ++l) {
+        if (l >= L - 1) break;
+        const int no = D.nodes[l], ps = D.wcs[l];
+        const float* W = A.wchunk + D.wcf[l];
+        const float* b = A.wchunk + D.wcb[l];
+        const int G = lane_groups(no), gw = kWave / G;
+        const int j = lane % gw, grp = lane / gw, jj = min(j, no - 1);
+        const int kper = (ni + G - 1) / G, k0 = grp * kper, k1 = min(ni, k0 + kper);
+        float acc = 0.0f;
+#pragma unroll 4
+        for (int k = k0; k < k1; ++k) acc += W[k * ps + jj] * x[k];
+        for (int m = gw; m < kWave; m <<= 1) acc += __shfl_xor(acc, m);
+        float z[1] = {acc + b[jj]}, av[1], dv[1], d2[1];
+        act_many<false, 1>(D.neuron, D.c, z, av, dv, d2);
+        sp[l] = dv[0];
+        float* h = nb + (l & 1) * kMaxWidth;
+        if (grp == 0 && j < no) h[j] = av[0];
+        wave_sync();
+        x = h;
+        ni = no;
+    }
+    // linear output neuron
+    const float* WL = A.wchunk + D.wco;                     // weights, then (at a multiple of 4) the bias
+    float part = 0.0f;
+    for (int m = lane; m < ni; m += kWave) part += WL[m] * x[m];
+    part = wave_sum(part);
+    if (lane == 0) A.Eatom[i] = D.outb + D.outw * (WL[vec_pad4(ni)] + part);
+    wave_sync();                                            // every lane is done with x
+    if (L == 1) {
+        for (int k = lane; k < ldg; k += kWave) rowbuf[k] = k < nin ? D.outw * WL[k] * iw[k] : 0.0f;
+        wave_sync();
+        return;
+    }
+    // g of the last hidden layer (its lanes hold sigma' for neuron lane % gw)
+    float* dc = nb + 2 * kMaxWidth;
+    float* dn = nb;
+    {
+        const int G = lane_groups(ni), gw = kWave / G, j = lane % gw;
+        float spl = 0.0f;
+#pragma unroll
+        for (int q = 0; q < kMaxLayers - 1; ++q)
+            if (q == L - 2) spl = sp[q];
+        if (lane < gw && j < ni) dc[j] = D.outw * WL[j] * spl;
+    }
+    wave_sync();
+    // hidden layers: g_{l-1}[k] = sigma'_{l-1}[k] sum_j W_l[k][j] g_l[j], lanes over k
+#pragma unroll
+    for (int l = kMaxLayers - 2; l >= 1; --l) {
+        if (l > L - 2) continue;
+        const int no = D.nodes[l], nil = D.nodes[l - 1], pst = D.wts[l];
+        const float* WT = A.wtrans + D.wtf[l];
+        const int G = lane_groups(nil), gw = kWave / G;
+        const int k = lane % gw, grp = lane / gw, kk = min(k, nil - 1);
+        const int jper = (no + G - 1) / G, j0 = grp * jper, j1 = min(no, j0 + jper);
+        float acc = 0.0f;
+#pragma unroll 4
+        for (int j = j0; j < j1; ++j) acc += WT[j * pst + kk] * dc[j];
+        for (int m = gw; m < kWave; m <<= 1) acc += __shfl_xor(acc, m);
+        if (grp == 0 && k < nil) dn[k] = acc * sp[l - 1];
+        wave_sync();
+        float* t = dc;
+        dc = dn;
+        dn = t;
+    }
+    // input layer: dE/dG[k] = inpw[k] sum_j W_0[k][j] g_0[j], blocks of up to 64 inputs
+    {
+        const int no = D.nodes[0], pst = D.wts[0];
+        const float* WT = A.wtrans + D.wtf[0];
+        for (int kb = 0; kb < nin; kb += kWave) {
+            const int nk = min(kWave, nin - kb);
+            const int G = lane_groups(nk), gw = kWave / G;
+            const int k = kb + lane % gw, grp = lane / gw, kk = min(k, nin - 1);
+            const int jper = (no + G - 1) / G, j0 = grp * jper, j1 = min(no, j0 + jper);
+            float acc = 0.0f;
+#pragma unroll 4
+            for (int j = j0; j < j1; ++j) acc += WT[j * pst + kk] * dc[j];
+            for (int m = gw; m < kWave; m <<= 1) acc += __shfl_xor(acc, m);
+            if (grp == 0 && lane % gw < nk) rowbuf[k] = acc * iw[k];
+        }
+        for (int k = nin + lane; k < ldg; k += kWave) rowbuf[k] = 0.0f;
+        wave_sync();
+    }
+}
+
 // One wavefront per atom with its own LDS slice (smem4); no block barriers, so
 // the waves of a workgroup are independent.
 template <int MODE, int Z>
@@ -926,6 +1047,9 @@ __device__ __forceinline__ void moment_atom(const SymmArgs& A, const int i, floa
     constexpr int PWS = (3 * ZP * (MODE == 2 ? 2 : 1)) | 1;   // powers per neighbour, odd stride
     constexpr int U4 = kMaxUnionEta;
     constexpr int NG = kWave / NM;                           // lane groups over the neighbours
+    constexpr bool FUSED = MODE == 3;                        // G, network and adjoint of the atom
+    constexpr bool ADJ = MODE == 1 || FUSED;                 // per-slot adjoint vectors written
+    constexpr int GM = FUSED ? 0 : MODE;                     // mode of the basis phase
     const int lane = threadIdx.x % kWave;
     const DevModel& Mo = *A.dm;
     const int a = A.species[i], o = A.off[i];
@@ -938,15 +1062,18 @@ __device__ __forceinline__ void moment_atom(const SymmArgs& A, const int i, floa
     float4* tv = sm.efd + (MODE == 0 ? 0 : maxn);
     sm.r = reinterpret_cast<float*>(tv + (MODE == 2 ? maxn : 0));
     int* const after_r = reinterpret_cast<int*>(sm.r + maxn);
-    sm.canon = MODE == 1 ? after_r : nullptr;                   // the adjoint's slot map
-    sm.seg = after_r + (MODE == 1 ? maxn : 0);
+    sm.canon = ADJ ? after_r : nullptr;                         // the adjoint's slot map
+    sm.seg = after_r + (ADJ ? maxn : 0);
     sm.extra = reinterpret_cast<float*>(sm.seg + 16);
     float* rowbuf = sm.extra;                                   // [ldg]
-    float* sP = rowbuf + ldg;                                   // [maxn][3] (mode 1)
+    // [maxn][3] adjoint vectors (mode 1; fused: they share the power rows'
+    // region, whose rows are dead once the moments are summed)
+    float* sP = rowbuf + ldg + (FUSED ? 2 * maxn : 0);
     // adjoint with stored moments: no power rows (the cutoff pairs still use their space)
     const bool haveT = MODE == 1 && A.Tin != nullptr;
     const int TS = S * U4 * NM;                                 // moments per atom
-    float* pw = sP + (MODE == 1 ? 3 * maxn : 0);                // [pw_chunk][PWS] (one chunk of neighbours)
+    float* fgr = rowbuf + ldg;                                  // fused: [maxn] cutoff pairs of their own
+    float* pw = FUSED ? sP : sP + (MODE == 1 ? 3 * maxn : 0);  // [pw_chunk][PWS] (one chunk of neighbours)
     float* T = pw + pw_floats(MODE, maxn, PWS, haveT);         // [S][U4][NM]
     float* dT = T + S * U4 * NM;                                // [S][U4][NM] (mode 2)
     float* D = dT + (MODE == 2 ? S * U4 * NM : 0);              // [S][U4]
@@ -955,9 +1082,12 @@ __device__ __forceinline__ void moment_atom(const SymmArgs& A, const int i, floa
     // adjoint: dE/dT overwrites the moments when it fits a few registers per lane
     const bool ub_in_T = S * NM * U4 <= kUbRegs * kWave;
     float* Ub = ub_in_T ? T : Am + NP * U4 * ZP;               // [S][NM][U4] (mode 1)
-    float* part = Am + NP * U4 * ZP + (MODE == 1 && !ub_in_T ? S * NM * U4 : 0);   // [NG][NM][U4] x (1 or 2)
+    float* part = Am + NP * U4 * ZP + (ADJ && !ub_in_T ? S * NM * U4 : 0);   // [NG][NM][U4] x (1 or 2)
     float* wq = part + (NG > 1 ? NG * NM * U4 * (MODE == 2 ? 2 : 1) : 0);   // [NM] monomial weights
-    sm.fg = reinterpret_cast<float2*>(pw);   // cutoffs live in the power rows until the radial part is done
+    float* nb = FUSED ? pw : wq + NM;                           // fused: [3][kMaxWidth] network vectors (power rows' region)
+    // cutoffs live in the power rows until the radial part is done (fused: the
+    // adjoint's radial part runs after the moments, so they have their own rows)
+    sm.fg = reinterpret_cast<float2*>(FUSED ? fgr : pw);
 
     // adjoint: the dE/dG row and the stored moments are loaded before the neighbours so
     // that their latency overlaps the neighbour load (registers, written to LDS below)
@@ -975,7 +1105,7 @@ __device__ __forceinline__ void moment_atom(const SymmArgs& A, const int i, floa
             pre_t[h] = (haveT && q < TS) ? A.Tin[(size_t)i * TS + q] : 0.0f;
         }
     }
-    n = fast_load(A, Mo, i, a, o, n, sm, MODE != 0, MODE == 1, MODE == 2 ? tv : nullptr);
+    n = fast_load(A, Mo, i, a, o, n, sm, MODE != 0, ADJ, MODE == 2 ? tv : nullptr);
     if (lane < NM) wq[lane] = c_mono.w[lane];
     if (MODE == 1) {
 #pragma unroll
@@ -989,7 +1119,7 @@ __device__ __forceinline__ void moment_atom(const SymmArgs& A, const int i, floa
         for (int m = lane; m < ldg; m += kWave) rowbuf[m] = 0.0f;
     }
     wave_sync();
-    fast_radial<MODE>(A, Mo, a, sm, tv, rowbuf, sP, rowbuf);
+    fast_radial<GM>(A, Mo, a, sm, tv, rowbuf, sP, rowbuf);
     wave_sync();
     if (haveT) {
 #pragma unroll
@@ -1032,7 +1162,7 @@ __device__ __forceinline__ void moment_atom(const SymmArgs& A, const int i, floa
     // G with zeta up to 4 (35 monomials, one group would leave 29 lanes idle):
     // monomials 1, x, y, z are summed with the diagonal sums below and the other
     // 31 run as two groups of 32 lanes, joined by one cross-half exchange.
-    constexpr bool SPLIT = MODE == 0 && NM == 35;
+    constexpr bool SPLIT = GM == 0 && NM == 35;
     constexpr int NGL = SPLIT ? 2 : NG;                        // neighbour groups of the loop
     if (!haveT) {
         const int al = SPLIT ? min(4 + lane % 32, NM - 1) : lane % NM;
@@ -1169,8 +1299,9 @@ __device__ __forceinline__ void moment_atom(const SymmArgs& A, const int i, floa
     if (MODE == 0 && A.Tout) {
         for (int q = lane; q < TS; q += kWave) A.Tout[(size_t)i * TS + q] = T[q];
     }
-    // block contractions (modes 0, 2) or their adjoint seeds (mode 1)
-    if (MODE == 1) {
+    // block contractions (modes 0, 2, 3) or their adjoint seeds (mode 1; mode 3
+    // after its network)
+    auto adjoint_seeds = [&]() {
         // lanes over (pair, width, power) of every block at once; each lane loads
         // all of its output indices and coefficients before using any, so the
         // block descriptors cost a few rounds of memory latency, not one per term
@@ -1196,7 +1327,8 @@ __device__ __forceinline__ void moment_atom(const SymmArgs& A, const int i, floa
             }
             Am[it] = s;
         }
-    }
+    };
+    if (MODE == 1) adjoint_seeds();
     for (int it = lane; MODE != 1 && it < NP * U4 * ZP; it += kWave) {
         const int p = it / (U4 * ZP), u = (it / ZP) % U4, m = it % ZP;
         int c = 0;
@@ -1218,9 +1350,9 @@ __device__ __forceinline__ void moment_atom(const SymmArgs& A, const int i, floa
             for (int j = 0; j < QD; ++j) {
                 const int q = min(q0 + j, NM - 1);
                 const float w = q0 + j < q1 ? wq[q] : 0.0f;
-                s += MODE == 0 ? w * Tb[q] * Tc[q] : w * (dTb[q] * Tc[q] + Tb[q] * dTc[q]);
+                s += GM == 0 ? w * Tb[q] * Tc[q] : w * (dTb[q] * Tc[q] + Tb[q] * dTc[q]);
             }
-            if (b == c) s = 0.5f * (s - (MODE == 0 ? D[b * U4 + u] : dD[b * U4 + u]));
+            if (b == c) s = 0.5f * (s - (GM == 0 ? D[b * U4 + u] : dD[b * U4 + u]));
         }
         Am[it] = s;
     }
@@ -1245,9 +1377,22 @@ __device__ __forceinline__ void moment_atom(const SymmArgs& A, const int i, floa
             }
         }
         wave_sync();
-        float* out = (MODE == 0 ? A.G : A.v) + (size_t)i * ldg;
-        for (int m = lane; m < ldg; m += kWave) out[m] = rowbuf[m];
-        return;
+        if (!FUSED) {
+            float* out = (MODE == 0 ? A.G : A.v) + (size_t)i * ldg;
+            for (int m = lane; m < ldg; m += kWave) out[m] = rowbuf[m];
+            return;
+        }
+    }
+    if (FUSED) {
+        // the atom's G row in rowbuf: its energy and dE/dG (in place), then the
+        // radial adjoint and the angular seeds as the adjoint kernel forms them
+        fused_net(A, Mo.net[a], i, rowbuf, nb);
+        for (int t = lane; t < 3 * n; t += kWave) sP[t] = 0.0f;
+        wave_sync();
+        fast_radial<1>(A, Mo, a, sm, tv, rowbuf, sP, rowbuf);
+        wave_sync();
+        adjoint_seeds();
+        wave_sync();
     }
     // adjoint: dE/dT_b[alpha][u] and dE/dD_b[u]
     auto dEdT = [&](int it) {
@@ -1384,16 +1529,18 @@ __global__ void k_queue_reset(int* __restrict__ queue) {
     if (threadIdx.x < kXcds) queue[32 * threadIdx.x] = 0;
 }
 
+// LDS bytes of one wavefront of moment_atom<mode> (the layout at its top)
 static size_t moment_smem(const SymmArgs& A, int mode, int Z, int S, int NP) {
     const int NM = nmono(Z), ZP = Z + 1, U4 = kMaxUnionEta;
+    const bool adj = mode == 1 || mode == 3;
     const int PWS = (mode == 1 && A.Tin) ? 2 : (3 * ZP * (mode == 2 ? 2 : 1)) | 1;
     const int NG = kWave / NM;
     size_t b = 16 * (size_t)A.maxn * (2 + (mode == 0 ? 0 : 1) + (mode == 2 ? 1 : 0));
-    b += 4 * (mode == 1 ? 2 : 1) * (size_t)A.maxn + 4 * 16;                         // r, canon (adjoint), seg
-    size_t f = (size_t)A.ldg + (mode == 1 ? 3 * (size_t)A.maxn : 0) +
+    b += 4 * (adj ? 2 : 1) * (size_t)A.maxn + 4 * 16;                              // r, canon (adjoint), seg
+    size_t f = (size_t)A.ldg + (mode == 1 ? 3 * (size_t)A.maxn : 0) + (mode == 3 ? 2 * (size_t)A.maxn : 0) +
                (size_t)pw_floats(mode, A.maxn, PWS, mode == 1 && A.Tin);
     f += (size_t)S * U4 * NM * (mode == 2 ? 2 : 1) + 2 * (size_t)S * U4 + (size_t)NP * U4 * ZP;
-    f += (mode == 1 && S * NM * U4 > kUbRegs * kWave ? (size_t)S * NM * U4 : 0);   // dE/dT beside the moments
+    f += (adj && S * NM * U4 > kUbRegs * kWave ? (size_t)S * NM * U4 : 0);         // dE/dT beside the moments
     f += (NG > 1 ? (size_t)NG * NM * U4 * (mode == 2 ? 2 : 1) : 0);
     f += NM;
     return b + 4 * f;
@@ -1562,6 +1709,49 @@ hipError_t symm_bwd(ann_ctx* ctx, int form, const float* dEdG, float* P) {
     A.dEdG = dEdG;
     A.P = P;
     return run<1>(ctx, form, A);
+}
+
+// Fused E+F: G, the network and the force adjoint of each atom in one
+// moment-form kernel (moment_atom<3>), for FAST in the moment form with FP32
+// inputs and networks of at most 128 inputs and 64 neurons per layer.
+bool fused_supported(const ann_ctx* ctx, int form) {
+    const DevModel& M = ctx->hm;
+    if (form != ANN_FORM_FAST || !M.all_grid || M.fast_ng <= 0 || M.moment_z <= 0 || ctx->tune_angular != 0 ||
+        ctx->npre != 0 || ctx->natoms == 0)
+        return false;
+    for (int a = 0; a < M.S; ++a) {
+        const NetDesc& D = M.net[a];
+        if (D.nin > 2 * kWave) return false;
+        for (int l = 0; l < D.nlayer - 1; ++l)
+            if (D.nodes[l] > kMaxWidth) return false;
+    }
+    const SymmArgs A = make_args(const_cast<ann_ctx*>(ctx));   // reads only
+    return moment_smem(A, 3, M.moment_z, M.S, M.NP) <= 65536;
+}
+
+hipError_t symm_fused(ann_ctx* ctx, const float* inpb, float* Eatom, float* P) {
+    SymmArgs A = make_args(ctx);
+    if (A.N == 0) return hipSuccess;
+    hipError_t e;
+    if (!ctx->wchunk_valid && (e = chunk_weights(ctx)) != hipSuccess) return e;
+    A.P = P;
+    A.Eatom = Eatom;
+    A.wchunk = ctx->d_wchunk;
+    A.wtrans = ctx->d_wtrans;
+    A.inpw = ctx->da.inpw;
+    A.inpb = inpb;
+    const DevModel& M = ctx->hm;
+    Launch L(ctx, "ef_fused");
+    const size_t sw = (moment_smem(A, 3, M.moment_z, M.S, M.NP) + 15) / 16 * 16;
+    int apb = ctx->tune_apb[1];                             // the adjoint's atoms per workgroup
+    while (apb > 1 && sw * apb > 65536) apb /= 2;
+    A.smem_wave = (int)sw;
+    const unsigned grid = (unsigned)((A.N + apb - 1) / apb), block = 64u * apb;
+    void (*kern)(SymmArgs) = M.moment_z == 1 ? k_moment<3, 1> : M.moment_z == 2 ? k_moment<3, 2>
+                             : M.moment_z == 3 ? k_moment<3, 3> : k_moment<3, 4>;
+    kern<<<grid, block, sw * apb, ctx->stream>>>(A);
+    ctx->T_gen = ~0ull;                                     // no stored moments
+    return hipGetLastError();
 }
 
 hipError_t symm_tangent(ann_ctx* ctx, int form, const float* u, float* v) {

@@ -403,9 +403,9 @@ class Engine:
 
     def tuning(self) -> list:
         """[atoms per workgroup G, adjoint, tangent; triplet tiles; single-bin search; persistent queues;
-        matrix-core network]"""
-        t = (C.c_int * 7)()
-        _check(lib().ann_get_tuning(self.h, t, 7))
+        matrix-core network; fused E+F kernel]"""
+        t = (C.c_int * 8)()
+        _check(lib().ann_get_tuning(self.h, t, 8))
         return list(t)
 
     def set_tuning(self, t) -> None:

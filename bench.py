@@ -101,6 +101,8 @@ def work_model(model, batch, nl):
     N = batch.ntotal
     return {"symm_fwd": fwd, "symm_bwd": bwd, "symm_fwd_mom": fwd_m, "symm_bwd_mom": bwd_m,
             "symm_tan": fwd + bwd, "symm_tan_mom": tan_m, "net": 4.0 * mac,
+            # fused E+F: G, network (forward + dE/dG) and adjoint of each atom in one kernel
+            "ef_fused": fwd_m + bwd_m + 4.0 * mac,
             "net_bwd2": 8.0 * mac, "grad_reduce": 4.0 * mac,
             "pairs": P, "triplets": T,
             "bytes": {"nb_search": 24.0 * N + 8.0 * P, "nb_compact": 28.0 * N + 37.0 * P, "nb_rev": 20.0 * P,
@@ -120,6 +122,10 @@ def kernel_rooflines(kern, work):
             fl = work[base + "_mom"] if k.endswith("_mom") else work[base]
             ach = fl / sec / 1e12
             out[k] = {"bound": "valu", "work": fl, "unit": "TFLOP/s", "achieved": ach, "peak": FP32_PEAK_TFLOPS,
+                      "frac": ach / FP32_PEAK_TFLOPS}
+        elif k == "ef_fused":
+            ach = work[k] / sec / 1e12
+            out[k] = {"bound": "valu", "work": work[k], "unit": "TFLOP/s", "achieved": ach, "peak": FP32_PEAK_TFLOPS,
                       "frac": ach / FP32_PEAK_TFLOPS}
         elif k in ("net_bwd2", "grad_reduce"):
             # FP32 matrix-core products (v_mfma_f32_16x16x4_f32); peak = the FP32 figure
@@ -252,8 +258,9 @@ def run_uneven(steps, warmup):
     # atom-range) work queues, the rest of the tuning as chosen
     res["moment_dispatch"] = {}
     for q, name in ((0, "grid"), (1, "persistent_queues")):
-        t = list(tuning) + [0] * (6 - len(tuning))
+        t = list(tuning) + [0] * (8 - len(tuning))
         t[5] = q
+        t[7] = 0                                  # the queues apply to the separate moment kernels
         eng.set_tuning(t)
         dt = time_ef(eng, FAST, steps, max(1, warmup // 2), out)
         res["moment_dispatch"][name] = {"atoms_per_s": batch.ntotal / dt, "ms_per_step": dt * 1e3}
@@ -547,7 +554,7 @@ def main():
         kern = {}
         for k in ("nb_bin", "nb_search", "nb_compact", "nb_rev", "symm_fwd_plain", "symm_fwd_fast", "symm_fwd_mom",
                   "symm_bwd_plain", "symm_bwd_fast", "symm_bwd_mom", "net_plain", "net_vec", "net_mfma",
-                  "force_gather", "struc_sums"):
+                  "ef_fused", "force_gather", "struc_sums"):
             ms, n = eng.profile_get(k)
             if n:
                 kern[k] = {"avg_us": ms / n * 1e3, "launches": n}
@@ -566,12 +573,21 @@ def main():
         res[name] = {"ms_per_step": dt * 1e3, "atoms_per_s": batch.ntotal / dt, "kernels": kern}
         E, F, _ = eng.energy_forces(form)
         forces[name] = (E.cpu().numpy(), F.cpu().numpy())
-    # the two network kernels on the same E+F steps: vector-ALU tiles against
-    # matrix-core tiles (the autotuner keeps the faster)
+    # the E+F step with the fused kernel against the separate G / network /
+    # adjoint kernels (the autotuner keeps the faster)
+    ef_kernels = {}
+    for q, key in ((0, "separate"), (1, "fused")):
+        t = list(tuning) + [0] * (8 - len(tuning))
+        t[7] = q
+        eng.set_tuning(t)
+        ef_kernels[key] = {"ms_per_step": time_ef(eng, FAST, max(3, args.steps // 2), 2, out) * 1e3}
+    # the two network kernels on the same E+F steps (separate kernels): vector-ALU
+    # tiles against matrix-core tiles (the autotuner keeps the faster)
     net_kernels = {}
     for q, key in ((0, "net_vec"), (1, "net_mfma")):
-        t = list(tuning) + [0] * (7 - len(tuning))
+        t = list(tuning) + [0] * (8 - len(tuning))
         t[6] = q
+        t[7] = 0
         eng.set_tuning(t)
         eng.profile(True)
         eng.profile_only(key)
@@ -661,7 +677,9 @@ def main():
         "tuning": {"atoms_per_workgroup": {"G": tuning[0], "adjoint": tuning[1], "tangent": tuning[2]},
                    "angular": "triplet" if tuning[3] else "moment", "search_blocks": "single" if tuning[4] else "2x2x2",
                    "moment_dispatch": "persistent queues" if len(tuning) > 5 and tuning[5] else "grid",
-                   "network": "matrix cores" if len(tuning) > 6 and tuning[6] else "vector ALU"},
+                   "network": "matrix cores" if len(tuning) > 6 and tuning[6] else "vector ALU",
+                   "ef_kernels": "fused" if len(tuning) > 7 and tuning[7] else "separate"},
+        "ef_kernels": ef_kernels,
         "network_kernels": net_kernels,
         "train": train,
         "uneven_batch": uneven,

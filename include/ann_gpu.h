@@ -283,13 +283,16 @@ int ann_input_stats(ann_ctx* ctx, const float* d_G, int* count, double* stats, i
  * dispatch of the moment kernels against persistent work queues (per XCD,
  * (structure, atom-range) items, largest structure first); neighbour search
  * in 2x2x2-bin or single-bin blocks; the networks (TRIPLET and FAST) as
- * vector-ALU tiles of 16 atoms against matrix-core tiles.  Results do not
- * depend on the choice beyond FP32 summation order.  Tuning vector (5, 6 or 7
- * ints): atoms per workgroup for G, adjoint, tangent; angular kernels
- * (0 moment, 1 triplet); search blocks (0 heuristic, 1 single bins);
- * moment-kernel dispatch (0 grid, 1 persistent queues); network kernel
- * (0 vector ALU where the species' weights fit its 64 KiB of LDS, else matrix
- * cores; 1 matrix cores).  No counterpart in the reference. */
+ * vector-ALU tiles of 16 atoms against matrix-core tiles; FAST E+F in the
+ * moment form as one fused kernel (G, network and force adjoint per atom)
+ * against separate kernels.  Results do not depend on the choice beyond FP32
+ * summation order.  Tuning vector (5 to 8 ints): atoms per workgroup for G,
+ * adjoint (also the fused kernel), tangent; angular kernels (0 moment,
+ * 1 triplet); search blocks (0 heuristic, 1 single bins); moment-kernel
+ * dispatch (0 grid, 1 persistent queues); network kernel (0 vector ALU where
+ * the species' weights fit its 64 KiB of LDS, else matrix cores; 1 matrix
+ * cores); E+F kernels (0 separate, 1 fused where it applies).  No counterpart
+ * in the reference. */
 int ann_autotune(ann_ctx* ctx, int form);
 int ann_get_tuning(ann_ctx* ctx, int* tuning, int n);
 int ann_set_tuning(ann_ctx* ctx, const int* tuning, int n);

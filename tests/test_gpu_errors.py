@@ -82,6 +82,7 @@ def test_calls_out_of_order_and_bad_arguments():
     assert _code(lambda: eng.set_tuning([3, 1, 1, 0, 0]))[0] == ARG
     assert _code(lambda: eng.set_tuning([1, 1, 1, 0, 0, 2]))[0] == ARG
     assert _code(lambda: eng.set_tuning([1, 1, 1, 0, 0, 0, 3]))[0] == ARG
+    assert _code(lambda: eng.set_tuning([1, 1, 1, 0, 0, 0, 0, 2]))[0] == ARG
     assert _code(lambda: eng.set_loss("MSE", -1.0))[0] == ARG
     assert _code(lambda: _check(lib().ann_set_loss(eng.h, 9, C.c_double(1.0), 1)))[0] == ARG
     assert _code(lambda: eng.set_scaling(5, np.ones(80), np.zeros(80)))[0] == ARG

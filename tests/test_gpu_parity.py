@@ -205,12 +205,14 @@ def test_network_parity(cases, oracle_mod, name, form):
 
 
 @pytest.mark.parametrize("tuning", [[2, 2, 2, 0, 0], [4, 4, 4, 0, 1], [1, 1, 1, 1, 0], [2, 1, 1, 0, 0, 1],
-                                    [2, 1, 1, 0, 0, 0, 1], [1, 1, 1, 1, 0, 0, 1]])
+                                    [2, 1, 1, 0, 0, 0, 1], [1, 1, 1, 1, 0, 0, 1], [2, 2, 1, 0, 0, 0, 0, 1],
+                                    [4, 4, 1, 0, 1, 0, 1, 1]])
 @pytest.mark.parametrize("name", ["water4", "lips3"])
 def test_tuning_variants_keep_parity(cases, oracle_mod, name, tuning):
     """Every tuning choice (atoms per workgroup, moment/triplet kernels,
-    search blocking, work queues, matrix-core against vector-ALU networks)
-    gives the same list bit for bit and E/F within the bars."""
+    search blocking, work queues, matrix-core against vector-ALU networks,
+    the fused E+F kernel) gives the same list bit for bit and E/F within the
+    bars."""
     model, batch = cases[name]
     from ann_amd.gpu import Engine
     eng = Engine(model, "cuda:0")
@@ -252,7 +254,7 @@ def test_autotune_picks_valid_settings(cases, oracle_mod):
     eng = _engine(model, batch)
     t = eng.autotune(1)
     assert t[0] in (1, 2, 4) and t[1] in (1, 2, 4) and t[2] in (1, 2, 4) and t[3] in (0, 1) and t[4] in (0, 1)
-    assert t[5] in (0, 1) and t[6] in (0, 1)
+    assert t[5] in (0, 1) and t[6] in (0, 1) and t[7] in (0, 1)
     E, F, _ = eng.energy_forces(1)
     Eo, Fo, _ = oracle_mod.Oracle(model).compute_batch(batch, nthreads=4)
     assert np.sqrt(((F.cpu().numpy() - Fo) ** 2).sum() / (Fo ** 2).sum()) <= 1e-5
