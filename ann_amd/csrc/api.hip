@@ -340,7 +340,7 @@ static int upload_model(ann_ctx* ctx, const ann_model_desc* d) {
         int prev = nin[a], cnt = 0;
         for (int l = 0; l < D.nlayer; ++l) {
             D.nodes[l] = d->nn_nodes[d->nn_nodes_off[a] + l];
-            if (D.nodes[l] < 1 || D.nodes[l] > kMaxWidth) return fail(ANN_ERR_UNSUPPORTED, "layer width must be 1..64");
+            if (D.nodes[l] < 1 || D.nodes[l] > kMaxWidth) return fail(ANN_ERR_UNSUPPORTED, "layer width must be 1..128");
         }
         if (D.nodes[D.nlayer - 1] != 1) return fail(ANN_ERR_UNSUPPORTED, "network output size must be 1");
         D.poff = ptot;

@@ -15,7 +15,8 @@ constexpr int kWave = 64;
 constexpr int kMaxSpecies = 8;
 constexpr int kMaxNeigh = 512;     // neighbour-list capacity per atom
 constexpr int kMaxLayers = 4;      // hidden + output
-constexpr int kMaxWidth = 64;      // widest layer supported by the network kernels
+constexpr int kMaxWidth = 128;     // widest layer supported by the network kernels
+constexpr int kVecMaxWidth = 64;   // widest layer of the vector-ALU and fused network paths
 constexpr int kMaxFunc = 32;       // functions per basis block
 constexpr int kMaxUnionEta = 4;    // distinct angular widths per centre species (fast form)
 constexpr int kGridG = 4;          // fast angular tile: widths per block

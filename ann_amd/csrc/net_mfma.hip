@@ -154,10 +154,14 @@ hipError_t split_weights(ann_ctx* ctx, int* tick) {
 }
 
 // two wavefronts per 16-atom single-species tile: the column tiles of every
-// product alternate between them (the launch has fewer tiles than SIMDs)
+// product alternate between them (the launch has fewer tiles than SIMDs).
+// NT: column tiles of 16 of the widest hidden layer (4 up to 64 neurons, 8 up
+// to 128), so narrow networks keep their registers and LDS.
+template <int NT>
 __global__ __launch_bounds__(128) void k_net_mfma(MfmaArgs A) {
+    constexpr int W = 16 * NT;                  // widest hidden layer of this instance
     __shared__ float act[kTile][kMaxK + 1];     // activations / gradients, operand side
-    __shared__ float sdz[kMaxLayers - 1][kTile][kMaxWidth + 1];
+    __shared__ float sdz[kMaxLayers - 1][kTile][W + 1];
     __shared__ int atoms[kTile];
     const int t = blockIdx.x, tid = threadIdx.x, lane = tid % kWave, wv = tid / kWave;
     constexpr int kThreads = 2 * kWave;
@@ -196,13 +200,13 @@ __global__ __launch_bounds__(128) void k_net_mfma(MfmaArgs A) {
     for (int l = 0; l < L - 1; ++l) {
         const int no = D.nodes[l];
         const float* bl = A.W + D.boff[l];
-        float out[kMaxWidth / 16][4];
-        f32x4 accs[kMaxWidth / 16];
+        float out[NT][4];
+        f32x4 accs[NT];
 #pragma unroll
-        for (int ct = 0; ct < kMaxWidth / 16; ++ct) accs[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
-        mm_bf16x3<kMaxWidth / 16>(ni, no, act, A.wtab + D.wsf[l], accs, wv);
+        for (int ct = 0; ct < NT; ++ct) accs[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+        mm_bf16x3<NT>(ni, no, act, A.wtab + D.wsf[l], accs, wv);
 #pragma unroll
-        for (int ct = 0; ct < kMaxWidth / 16; ++ct) {
+        for (int ct = 0; ct < NT; ++ct) {
             if (ct * 16 >= no) break;
             if ((ct & 1) != wv) continue;
             const f32x4 acc = accs[ct];
@@ -215,12 +219,12 @@ __global__ __launch_bounds__(128) void k_net_mfma(MfmaArgs A) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 out[ct][i] = n < no ? av[i] : 0.0f;
-                sdz[l][rb + i][n < kMaxWidth ? n : kMaxWidth] = n < no ? dv[i] : 0.0f;
+                sdz[l][rb + i][n < W ? n : W] = n < no ? dv[i] : 0.0f;
             }
         }
         __syncthreads();
 #pragma unroll
-        for (int ct = 0; ct < kMaxWidth / 16; ++ct) {
+        for (int ct = 0; ct < NT; ++ct) {
             if (ct * 16 >= no) break;
             if ((ct & 1) != wv) continue;
             const int n = ct * 16 + col;
@@ -330,8 +334,12 @@ hipError_t net_fwd_mfma(ann_ctx* ctx, const float* G, float* Eatom, float* dEdG,
     A.wtab = ctx->d_wsplit;
     if (A.ntiles == 0) return hipSuccess;
     if (!ctx->wsplit_valid && (e = split_weights(ctx, nullptr)) != hipSuccess) return e;
+    int wmax = 1;
+    for (int a = 0; a < ctx->hm.S; ++a)
+        for (int l = 0; l < ctx->hm.net[a].nlayer - 1; ++l) wmax = std::max(wmax, ctx->hm.net[a].nodes[l]);
     Launch L(ctx, "net_mfma");
-    k_net_mfma<<<A.ntiles, 128, 0, ctx->stream>>>(A);
+    if (wmax <= 64) k_net_mfma<4><<<A.ntiles, 128, 0, ctx->stream>>>(A);
+    else k_net_mfma<8><<<A.ntiles, 128, 0, ctx->stream>>>(A);
     return hipGetLastError();
 }
 

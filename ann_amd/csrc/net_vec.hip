@@ -347,7 +347,7 @@ static VecExtents vec_extents(const DevModel& M) {
         x.maxin = std::max(x.maxin, D.nin);
         x.maxl1 = std::max(x.maxl1, D.nlayer - 1);
         for (int l = 0; l < D.nlayer; ++l) {
-            if (D.nodes[l] > kMaxWidth) x.ok = false;
+            if (D.nodes[l] > kVecMaxWidth) x.ok = false;
             if (l < D.nlayer - 1) x.maxw = std::max(x.maxw, D.nodes[l]);
         }
     }

@@ -898,7 +898,7 @@ __host__ __device__ inline int pw_floats(int mode, int maxn, int pws, bool haveT
     const int rows = pw_chunk(mode, maxn);
     if (mode == 3) {                                // fused: also the adjoint vectors and network vectors
         const int m = rows * pws > 3 * maxn ? rows * pws : 3 * maxn;
-        return m > 3 * kMaxWidth ? m : 3 * kMaxWidth;
+        return m > 3 * kVecMaxWidth ? m : 3 * kVecMaxWidth;
     }
     return haveT ? 2 * maxn : (rows * pws > 2 * maxn ? rows * pws : 2 * maxn);
 }
@@ -940,7 +940,7 @@ __device__ __forceinline__ int lane_groups(int n) { return n <= 16 ? 4 : (n <= 3
 // A.Eatom.  Lanes over a layer's outputs, its reduction split over up to four
 // lane groups joined by butterflies; the layer's inputs are broadcast LDS
 // reads, the weights coalesced reads of the row tables (forward: rows of W,
-// backward: rows of W^T, net_dev.h layout).  nb: [3][kMaxWidth] LDS floats.
+// backward: rows of W^T, net_dev.h layout).  nb: [3][kVecMaxWidth] LDS floats.
 __device__ __forceinline__ void fused_net(const SymmArgs& A, const NetDesc& D, int i, float* rowbuf, float* nb) {
     const int lane = threadIdx.x % kWave;
     const int L = D.nlayer, nin = D.nin, ldg = A.ldg;
@@ -969,7 +969,7 @@ __device__ __forceinline__ void fused_net(const SymmArgs& A, const NetDesc& D, i
         float z[1] = {acc + b[jj]}, av[1], dv[1], d2[1];
         act_many<false, 1>(D.neuron, D.c, z, av, dv, d2);
         sp[l] = dv[0];
-        float* h = nb + (l & 1) * kMaxWidth;
+        float* h = nb + (l & 1) * kVecMaxWidth;
         if (grp == 0 && j < no) h[j] = av[0];
         wave_sync();
         x = h;
@@ -988,7 +988,7 @@ __device__ __forceinline__ void fused_net(const SymmArgs& A, const NetDesc& D, i
         return;
     }
     // g of the last hidden layer (its lanes hold sigma' for neuron lane % gw)
-    float* dc = nb + 2 * kMaxWidth;
+    float* dc = nb + 2 * kVecMaxWidth;
     float* dn = nb;
     {
         const int G = lane_groups(ni), gw = kWave / G, j = lane % gw;
@@ -1084,7 +1084,7 @@ __device__ __forceinline__ void moment_atom(const SymmArgs& A, const int i, floa
     float* Ub = ub_in_T ? T : Am + NP * U4 * ZP;               // [S][NM][U4] (mode 1)
     float* part = Am + NP * U4 * ZP + (ADJ && !ub_in_T ? S * NM * U4 : 0);   // [NG][NM][U4] x (1 or 2)
     float* wq = part + (NG > 1 ? NG * NM * U4 * (MODE == 2 ? 2 : 1) : 0);   // [NM] monomial weights
-    float* nb = FUSED ? pw : wq + NM;                           // fused: [3][kMaxWidth] network vectors (power rows' region)
+    float* nb = FUSED ? pw : wq + NM;                           // fused: [3][kVecMaxWidth] network vectors (power rows' region)
     // cutoffs live in the power rows until the radial part is done (fused: the
     // adjoint's radial part runs after the moments, so they have their own rows)
     sm.fg = reinterpret_cast<float2*>(FUSED ? fgr : pw);
@@ -1723,7 +1723,7 @@ bool fused_supported(const ann_ctx* ctx, int form) {
         const NetDesc& D = M.net[a];
         if (D.nin > 2 * kWave) return false;
         for (int l = 0; l < D.nlayer - 1; ++l)
-            if (D.nodes[l] > kMaxWidth) return false;
+            if (D.nodes[l] > kVecMaxWidth) return false;
     }
     const SymmArgs A = make_args(const_cast<ann_ctx*>(ctx));   // reads only
     return moment_smem(A, 3, M.moment_z, M.S, M.NP) <= 65536;

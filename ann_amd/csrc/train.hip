@@ -212,6 +212,8 @@ constexpr int kB2Tile = 16;
 constexpr int kB2K = 128;      // widest input supported (same bound as the forward MFMA network)
 constexpr int kB2Chunk = 8;    // k steps (of 4) per weight chunk held in registers
 
+// NT: this wavefront's column tiles (2 for hidden layers up to 64 neurons, 4 up to 128)
+template <int NT>
 __global__ __launch_bounds__(128) void k_bwd2_mfma(Bwd2Args A, const int* __restrict__ perm,
                                                   const int2* __restrict__ tiles, const int* __restrict__ spend) {
     // dynamic LDS sized by the model: activations / gradients [16][kp] (x and its
@@ -269,7 +271,6 @@ __global__ __launch_bounds__(128) void k_bwd2_mfma(Bwd2Args A, const int* __rest
         }
     }
     const int c16 = lane % 16, k4 = lane / 16, rb = 4 * k4;
-    constexpr int NT = kMaxWidth / 32;          // this wavefront's column tiles
     int ni = D.nin;
     for (int l = 0; l < L - 1; ++l) {
         const int no = D.nodes[l];
@@ -720,9 +721,14 @@ hipError_t net_bwd2(ann_ctx* ctx, const float* G, const float* v, const double* 
         A.wp = (wmax + 1) | 1;
         A.nlh = nlh;
         const size_t smem = sizeof(float) * (2 * (size_t)kB2Tile * A.kp + 2 * (size_t)nlh * kB2Tile * A.wp);
-        if (ctx->ntiles > 0)
-            k_bwd2_mfma<<<ctx->ntiles, 128, smem, ctx->stream>>>(A, ctx->d_perm, reinterpret_cast<const int2*>(ctx->d_tiles),
-                                                               ctx->d_spend_end);
+        int hmax = 1;                                 // widest hidden layer
+        for (int s = 0; s < ctx->S; ++s)
+            for (int l = 0; l < ctx->hm.net[s].nlayer - 1; ++l) hmax = std::max(hmax, ctx->hm.net[s].nodes[l]);
+        if (ctx->ntiles > 0) {
+            void (*kb)(Bwd2Args, const int*, const int2*, const int*) = hmax <= 64 ? k_bwd2_mfma<2> : k_bwd2_mfma<4>;
+            kb<<<ctx->ntiles, 128, smem, ctx->stream>>>(A, ctx->d_perm, reinterpret_cast<const int2*>(ctx->d_tiles),
+                                                       ctx->d_spend_end);
+        }
     } else {
         Launch L(ctx, "net_bwd2");
         k_net_bwd2<<<(A.N + 3) / 4, 64, 0, ctx->stream>>>(A);

@@ -62,6 +62,9 @@ enum {
  * biases layer by layer, then each weight matrix column-major,
  * src/ml/nn.cpp:846-865) at params[nn_par_off[a]]; input scaling at
  * inpw/inpb[nn_inp_off[a]]; outw/outb/evac per species.
+ * Limits (ANN_ERR_UNSUPPORTED beyond them): 1..8 species, at most 32
+ * functions per radial or angular basis, 1..4 layers per network, layer widths
+ * 1..128.
  * Replaces NNP::read + NNH::init_input (src/nnp/nnp.cpp:243-307,
  * src/nnp/nnh.cpp:94-112) as the way a potential reaches the device. */
 typedef struct ann_model_desc {
