@@ -228,6 +228,7 @@ struct ann_ctx {
     bool pos_ref_valid = false;       // d_pos_ref holds the positions of the last build
     int* d_bin_atoms = nullptr;
     int bin_cap = 0;
+    bool nb_dirty = true;             // bin counts or build flags may be non-zero: clear before the next build
     uint64_t* d_keys = nullptr;       // [N][kMaxNeigh] sorted candidate keys
     double4* d_binrec = nullptr;      // [N] atoms in bin order: x, y, z, (atom, packed shift)
     int64_t binrec_cap = 0;
@@ -337,9 +338,9 @@ hipError_t species_perm(ann_ctx* ctx);   // species-sorted atom order and 16-ato
 hipError_t split_weights(ann_ctx* ctx, int* tick);
 // exclusive scan of n ints (out[n-1] = sum of in[0..n-2]); with rb set, also
 // copies nflags words of flags, then out[n-1], then seq (a completion marker)
-// to the pinned words rb
-hipError_t exclusive_scan(const int* in, int* out, int n, hipStream_t st, const int* flags = nullptr,
-                          int nflags = 0, int* rb = nullptr, int seq = 0);
+// to the pinned words rb, and clears the first nclear flags once copied
+hipError_t exclusive_scan(const int* in, int* out, int n, hipStream_t st, int* flags = nullptr,
+                          int nflags = 0, int* rb = nullptr, int seq = 0, int nclear = 0);
 // copy n device ints to the pinned words rb (stream ordered; read after a sync)
 hipError_t readback(const int* src, int n, int* rb, hipStream_t st);
 hipError_t symm_fwd(ann_ctx* ctx, int form, float* G);
@@ -361,6 +362,9 @@ hipError_t symm_tangent(ann_ctx* ctx, int form, const float* u, float* v);
 bool fused_supported(const ann_ctx* ctx, int form);
 hipError_t symm_fused(ann_ctx* ctx, const float* inpb, float* Eatom, float* P);   // G + network + adjoint
 hipError_t chunk_weights(ann_ctx* ctx);
+// every weight table from the FP32 weights in one launch (split: also the
+// matrix-core operand tables; tick: Adam step counter advanced by the launch)
+hipError_t weight_tables(ann_ctx* ctx, int* tick, bool split);
 // centred: G rows hold G + inpb at the FP64-evaluated inputs (precise_inputs);
 // the network then applies inpb_net instead of inpb
 hipError_t net_fwd(ann_ctx* ctx, int form, const float* G, float* Eatom, float* dEdG, bool centred = false);

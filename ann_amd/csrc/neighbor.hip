@@ -77,6 +77,8 @@ __device__ __forceinline__ double4 bin_record(int i, const double* __restrict__ 
     return make_double4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], __longlong_as_double((long long)js));
 }
 
+// Binning.  The bin counts are zero on entry: k_fill_bins takes every count
+// back to zero.
 __global__ void k_bin(int N, const double* __restrict__ pos, const int* __restrict__ sof,
                       const CellInfo* __restrict__ ci, const double* __restrict__ rinv,
                       int* __restrict__ bin_of, int* __restrict__ shift, int* __restrict__ bin_cnt,
@@ -89,15 +91,17 @@ __global__ void k_bin(int N, const double* __restrict__ pos, const int* __restri
 }
 
 // Atoms in bin order: index, and a 32-byte record (x, y, z, atom | shift)
-// that the staged search copies with one load per candidate.
+// that the staged search copies with one load per candidate.  The slot is
+// taken by counting the bin's count down, so every count is zero again when
+// the kernel ends.
 __global__ void k_fill_bins(int N, const int* __restrict__ bin_of, const int* __restrict__ bin_start,
-                            int* __restrict__ bin_fill, int* __restrict__ bin_atoms,
+                            int* __restrict__ bin_cnt, int* __restrict__ bin_atoms,
                             const double* __restrict__ pos, const int* __restrict__ shift,
                             double4* __restrict__ rec) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
     const int g = bin_of[i];
-    const int p = atomicAdd(&bin_fill[g], 1);
+    const int p = atomicSub(&bin_cnt[g], 1) - 1;
     const int slot = bin_start[g] + p;
     bin_atoms[slot] = i;
     rec[slot] = bin_record(i, pos, shift);
@@ -211,7 +215,9 @@ __global__ __launch_bounds__(64) void k_search(
 // image) into its own LDS buffer and stores them sorted by rank (no block
 // barriers after the staging).  Bins
 // whose search box holds more than kStageCap candidates or kStageImages
-// images are flagged and left to k_search.
+// images are flagged and left to k_search.  Each block writes the overflow
+// mark (0 or 1) of each of its centre bins exactly once per build, so the
+// marks need no clearing between builds.
 constexpr int kStageCap = 1280;
 constexpr int kStageImages = 216;
 constexpr int kStageThreads = 512;
@@ -256,6 +262,7 @@ __global__ __launch_bounds__(kStageThreads) void k_search_bins(
     __shared__ uint64_t cjd[kStageCap];                     // j << 32 | packed (image - shift_j)
     __shared__ uint64_t wkeys[kStageWaves][kWaveKeys];
     __shared__ int cen_bin[8], cen_pre[9], cen_img[8];
+    __shared__ int s_ovf[8];                                // centre bins left to the wide search
     __shared__ int s_total, s_max, s_xmax;
     const int g = xcd_block(blockIdx.x, gridDim.x), t = threadIdx.x;
     const int lane = t % kWave, wave = t / kWave;
@@ -282,6 +289,7 @@ __global__ __launch_bounds__(kStageThreads) void k_search_bins(
         s_max = 0;
         s_xmax = 0;
     }
+    if (t < 8) s_ovf[t] = 0;
     const int ncb = n0b * n1b * n2b;
     const int e0 = n0b + 2 * c.k[0], e1 = n1b + 2 * c.k[1], e2 = n2b + 2 * c.k[2];
     const int nimg = e0 * e1 * e2;
@@ -321,7 +329,10 @@ __global__ __launch_bounds__(kStageThreads) void k_search_bins(
     __syncthreads();
     const int total = s_total;
     const int ncen = cen_pre[ncb];
-    if (ncen == 0) return;                                  // no centres in this super-bin
+    if (ncen == 0) {                                        // no centres in this super-bin
+        if (t < ncb) bin_ovf[cen_bin[t]] = 0;
+        return;
+    }
     if (total > kStageCap) {
         if (t < ncb) bin_ovf[cen_bin[t]] = 1;
         if (t == 0) atomicOr(flag, 8);
@@ -425,7 +436,7 @@ __global__ __launch_bounds__(kStageThreads) void k_search_bins(
         if (count > kWaveKeys) {
             // more neighbours than the staged buffer holds: the bin goes to the wide search
             if (lane == 0) {
-                bin_ovf[cen_bin[cb]] = 1;
+                s_ovf[cb] = 1;
                 atomicOr(flag, 8);
             }
             continue;
@@ -448,6 +459,7 @@ __global__ __launch_bounds__(kStageThreads) void k_search_bins(
     if (lane == 0) atomicMax(&s_max, wmax);
     __syncthreads();
     if (t == 0) atomicMax(flag + 1, s_max);
+    if (t < ncb) bin_ovf[cen_bin[t]] = s_ovf[t];
 }
 
 // CSR fill: index, packed image, neighbour species and FP32 pair geometry
@@ -549,19 +561,13 @@ constexpr int kScanThreads = 1024;
 constexpr int kScanPre = 4;             // tiles of 4 elements per thread loaded up front
 constexpr int kScanMax = 1 << 16;
 
-__global__ void k_clear2(int* __restrict__ a, int na, int* __restrict__ b, int nb) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < na) a[t] = 0;
-    else if (t < na + nb) b[t - na] = 0;
-}
-
 __device__ __forceinline__ void store_host(int* rb, int q, int v) {
     __hip_atomic_store(rb + q, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-__global__ __launch_bounds__(kScanThreads) void k_scan_small(const int* __restrict__ in, int* __restrict__ out, int n,
-                                                             const int* __restrict__ flags, int nflags, int* rb,
-                                                             int seq) {
+// Exclusive scan of n values by one workgroup of kScanThreads threads; returns
+// the sum of all n values to every thread.
+__device__ int scan_block(const int* __restrict__ in, int* __restrict__ out, int n) {
     __shared__ int wsum[kScanThreads / kWave];
     __shared__ int s_carry;
     const int t = threadIdx.x, lane = t % kWave, w = t / kWave;
@@ -618,19 +624,30 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_small(const int* __restri
         if (t == 0) s_carry += wsum[kScanThreads / kWave - 1];
         __syncthreads();
     }
-    if (rb && t == kScanThreads - 1) {
-        // the last element's exclusive prefix is the carry minus the last value
+    return s_carry;
+}
+
+// One-workgroup scan; with rb, the flags, the last element's exclusive prefix
+// and then seq go to the pinned host words, and the first nclear flags are
+// cleared once read (the next build starts from zero without a clear launch).
+__global__ __launch_bounds__(kScanThreads) void k_scan_small(const int* __restrict__ in, int* __restrict__ out, int n,
+                                                             int* __restrict__ flags, int nflags, int nclear, int* rb,
+                                                             int seq) {
+    const int tot = scan_block(in, out, n);
+    if (rb && threadIdx.x == kScanThreads - 1) {
         for (int q = 0; q < nflags; ++q) store_host(rb, q, flags[q]);
-        store_host(rb, nflags, n > 0 ? s_carry - in[n - 1] : 0);
+        for (int q = 0; q < nclear; ++q) flags[q] = 0;
+        store_host(rb, nflags, n > 0 ? tot - in[n - 1] : 0);
         __threadfence_system();
         store_host(rb, nflags + 1, seq);            // the words above are complete
         __threadfence_system();
     }
 }
 
-__global__ void k_readback(const int* __restrict__ src, int n, int* rb, int seq) {
+__global__ void k_readback(int* src, int n, int* rb, int seq, int nclear) {
     const int t = threadIdx.x;
     if (t < n) store_host(rb, t, src[t]);
+    if (t < nclear) src[t] = 0;                     // cleared once read
     __threadfence_system();
     __syncthreads();
     if (seq && t == 0) {
@@ -640,14 +657,14 @@ __global__ void k_readback(const int* __restrict__ src, int n, int* rb, int seq)
 }
 
 hipError_t readback(const int* src, int n, int* rb, hipStream_t st) {
-    k_readback<<<1, 64, 0, st>>>(src, n, rb, 0);
+    k_readback<<<1, 64, 0, st>>>(const_cast<int*>(src), n, rb, 0, 0);
     return hipGetLastError();
 }
 
-hipError_t exclusive_scan(const int* in, int* out, int n, hipStream_t st, const int* flags, int nflags, int* rb,
-                          int seq) {
+hipError_t exclusive_scan(const int* in, int* out, int n, hipStream_t st, int* flags, int nflags, int* rb, int seq,
+                          int nclear) {
     if (n <= kScanMax) {
-        k_scan_small<<<1, kScanThreads, 0, st>>>(in, out, n, flags, nflags, rb, seq);
+        k_scan_small<<<1, kScanThreads, 0, st>>>(in, out, n, flags, nflags, nclear, rb, seq);
         return hipGetLastError();
     }
     size_t bytes = 0;
@@ -663,8 +680,11 @@ hipError_t exclusive_scan(const int* in, int* out, int n, hipStream_t st, const 
     }
     if ((e = hipcub::DeviceScan::ExclusiveSum(tmp, bytes, in, out, n, st)) != hipSuccess) return e;
     if (rb) {
-        if (nflags > 0 && (e = readback(flags, nflags, rb, st)) != hipSuccess) return e;
-        k_readback<<<1, 64, 0, st>>>(out + n - 1, 1, rb + nflags, seq);   // total, then the marker
+        if (nflags > 0) {
+            k_readback<<<1, 64, 0, st>>>(flags, nflags, rb, 0, nclear);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
+        k_readback<<<1, 64, 0, st>>>(out + n - 1, 1, rb + nflags, seq, 0);   // total, then the marker
         e = hipGetLastError();
     }
     return e;
@@ -718,11 +738,14 @@ static hipError_t grid_setup(ann_ctx* ctx, double rc) {
     }
     if (nbins + 1 > ctx->bin_cap) {
         if (ctx->d_bin_cnt) { hipFree(ctx->d_bin_cnt); hipFree(ctx->d_bin_start); hipFree(ctx->d_bin_struc); }
-        // counts, fill cursors and overflow marks share one allocation (one clear per build)
+        // counts and overflow marks share one allocation; the counts are
+        // cleared before the first build and return to zero by the end of
+        // every build
         HIPCHK(hipMalloc(&ctx->d_bin_cnt, sizeof(int) * 3 * (nbins + 1)));
         HIPCHK(hipMalloc(&ctx->d_bin_start, sizeof(int) * (nbins + 1)));
         HIPCHK(hipMalloc(&ctx->d_bin_struc, sizeof(int) * (nbins + 1)));
         ctx->bin_cap = nbins + 1;
+        ctx->nb_dirty = true;
     }
     ctx->d_bin_ovf = ctx->d_bin_cnt + 2 * (size_t)ctx->bin_cap;
     HIPCHK(hipMemcpyAsync(ctx->d_cellinfo, info.data(), sizeof(CellInfo) * S, hipMemcpyHostToDevice, st));
@@ -785,19 +808,27 @@ hipError_t neighbors_build(ann_ctx* ctx, double rc) {
     HIPCHK(grow(ctx->d_binrec, rcap, (int64_t)N));
     ctx->binrec_cap = rcap;
     const CellInfo* dci = reinterpret_cast<const CellInfo*>(ctx->d_cellinfo);
+    // The bin counts and flag[0..1] are zero at the start of a build without a
+    // clear launch: the fill counts every bin back down and the readback scan
+    // clears the flags it reports (flag[2], the reverse-slot error, stays set
+    // until reported).  New bin arrays, or a build that stopped part-way
+    // (nb_dirty), are cleared here first.
+    if (ctx->nb_dirty) {
+        HIPCHK(hipMemsetAsync(ctx->d_bin_cnt, 0, sizeof(int) * (size_t)ctx->bin_cap, st));
+        HIPCHK(hipMemsetAsync(ctx->d_flag, 0, sizeof(int) * 2, st));
+    }
+    ctx->nb_dirty = true;
     {
-        // bin counters, fill cursors, overflow marks and flag[0..1] (flag[2]: sticky reverse-slot error)
-        int* bin_fill = ctx->d_bin_cnt + (nbins + 1);
-        const int nc = 3 * ctx->bin_cap;
-        k_clear2<<<(nc + 2 + 255) / 256, 256, 0, st>>>(ctx->d_bin_cnt, nc, ctx->d_flag, 2);
         {
             Launch L(ctx, "nb_bin");
             k_bin<<<(N + 255) / 256, 256, 0, st>>>(N, ctx->d_pos, ctx->d_struc_of, dci, ctx->d_cellmat,
                                                   ctx->d_bin_of, ctx->d_shift, ctx->d_bin_cnt, ctx->d_flag);
         }
+        HIPCHK(hipGetLastError());
         HIPCHK(exclusive_scan(ctx->d_bin_cnt, ctx->d_bin_start, nbins + 1, st));
-        k_fill_bins<<<(N + 255) / 256, 256, 0, st>>>(N, ctx->d_bin_of, ctx->d_bin_start, bin_fill,
+        k_fill_bins<<<(N + 255) / 256, 256, 0, st>>>(N, ctx->d_bin_of, ctx->d_bin_start, ctx->d_bin_cnt,
                                                     ctx->d_bin_atoms, ctx->d_pos, ctx->d_shift, ctx->d_binrec);
+        HIPCHK(hipGetLastError());
     }
     {
         Launch L(ctx, "nb_search");
@@ -812,7 +843,8 @@ hipError_t neighbors_build(ann_ctx* ctx, double rc) {
     // speculatively into the current list arrays; it does nothing if the list
     // does not fit, and is repeated after growing them.
     int seq = ++ctx->rb_seq;
-    HIPCHK(exclusive_scan(ctx->d_cnt, ctx->nl.off, N + 1, st, ctx->d_flag, 4, ctx->d_rb, seq));
+    HIPCHK(exclusive_scan(ctx->d_cnt, ctx->nl.off, N + 1, st, ctx->d_flag, 4, ctx->d_rb, seq, 2));
+    ctx->nb_dirty = false;
     const int spec_cap = (int)std::min<int64_t>(nl.cap, INT32_MAX);
     if (spec_cap > 0) launch_compact(ctx, spec_cap);
     HIPCHK(hipGetLastError());
@@ -829,6 +861,7 @@ hipError_t neighbors_build(ann_ctx* ctx, double rc) {
     }
     if (flags[0] & 8) {
         // bins with a search box too large to stage: one wavefront per centre atom
+        ctx->nb_dirty = true;
         {
             Launch L(ctx, "nb_search_wide");
             k_search<<<N, 64, 0, st>>>(N, ctx->d_pos, ctx->d_struc_of, dci, ctx->d_cells, ctx->d_bin_of,
@@ -837,11 +870,14 @@ hipError_t neighbors_build(ann_ctx* ctx, double rc) {
         }
         HIPCHK(hipGetLastError());
         seq = ++ctx->rb_seq;
-        HIPCHK(exclusive_scan(ctx->d_cnt, ctx->nl.off, N + 1, st, ctx->d_flag, 4, ctx->d_rb, seq));
+        HIPCHK(exclusive_scan(ctx->d_cnt, ctx->nl.off, N + 1, st, ctx->d_flag, 4, ctx->d_rb, seq, 2));
+        ctx->nb_dirty = false;
         if (spec_cap > 0) launch_compact(ctx, spec_cap);
         HIPCHK(hipGetLastError());
         HIPCHK(wait_marker(ctx, 5, seq));
-        for (int q = 0; q < 4; ++q) flags[q] = rb[q];
+        // the first pass's flags were cleared when read: merge both passes
+        flags[0] |= rb[0];
+        flags[1] = std::max(flags[1], (int)rb[1]);
         total = rb[4];
     }
     if (flags[0] & 16) {

@@ -10,7 +10,7 @@
 //             dE/dG = (g_0 W_0) . inpw.
 // Activations move between the accumulator layout and the operand layout
 // through LDS; the weights come as operand tables split once per parameter
-// update (k_wsplit).
+// update (k_wtables).
 #include "net_dev.h"
 
 namespace ann {
@@ -19,7 +19,6 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 hipError_t species_perm(ann_ctx* ctx);
-hipError_t chunk_weights(ann_ctx* ctx);
 
 constexpr int kTile = 16;
 constexpr int kMaxK = 128;     // padded input width supported (n_in <= 128)
@@ -63,7 +62,7 @@ __device__ __forceinline__ bf16x8 as_bf16x8(uint4 u) {
     return v;
 }
 
-// The right operands come pre-split from a table (k_wsplit): for k block kk
+// The right operands come pre-split from a table (k_wtables): for k block kk
 // and column tile ct, the three BF16 terms of lane l are the 16-byte entries
 // ((kk * nt + ct) * 3 + p) * 64 + l, p = 0, 1, 2.
 template <int NT>
@@ -99,23 +98,36 @@ __device__ __forceinline__ void mm_bf16x3(int K, int ncol, const float (*X)[kMax
     }
 }
 
-// Operand tables of every hidden layer (blockIdx.y = (species, layer,
-// direction)): direction 0 holds W_l as the right operand of the forward
-// product (K = inputs, columns = neurons), direction 1 holds W_l^T for the
-// backward product (K = neurons, columns = inputs).  One thread per 16-byte
-// entry; tick (the Adam step counter) is advanced by one thread.
-__global__ __launch_bounds__(256) void k_wsplit(const DevModel* __restrict__ dm, const float* __restrict__ W,
-                                               uint4* __restrict__ tab, int* tick) {
-    const int seg = blockIdx.y;
+// The weight tables, all read from the FP32 weights W and rewritten after
+// every parameter update, in one launch (grid row = role and segment):
+//  * matrix-core operand tables of every hidden layer, segment (species,
+//    layer, direction): direction 0 holds W_l as the right operand of the
+//    forward product (K = inputs, columns = neurons), direction 1 holds W_l^T
+//    for the backward product (K = neurons, columns = inputs); one thread per
+//    16-byte entry; tick (the Adam step counter) is advanced by one thread;
+//  * the vector-ALU network's table of each species: the hidden layers' weight
+//    rows, their biases and the output neuron's weights and bias, one thread
+//    per float including the padding (written as zero);
+//  * its transposed weight rows W_l^T[j][k], segment (species, layer).
+struct WTabArgs {
+    const DevModel* dm;
+    const float* W;
+    uint4* split;       // null: the operand tables are not rewritten
+    float* chunk;
+    float* trans;
+    int* tick;
+    int ys, yc;         // grid rows of the operand tables and of the species tables
+};
+
+__device__ __forceinline__ void wsplit_entry(const DevModel* __restrict__ dm, const float* __restrict__ W,
+                                             uint4* __restrict__ tab, int seg, int e) {
     const int a = seg / (2 * kMaxLayers), l = (seg / 2) % kMaxLayers, dir = seg % 2;
-    if (tick && seg == 0 && blockIdx.x == 0 && threadIdx.x == 0) *tick += 1;
     if (a >= dm->S) return;
     const NetDesc& D = dm->net[a];
     if (l >= D.nlayer - 1) return;
     const int ni = l == 0 ? D.nin : D.nodes[l - 1], no = D.nodes[l];
     const int K = dir == 0 ? ni : no, ncol = dir == 0 ? no : ni;
     const int nt = (ncol + 15) / 16;
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (K + 31) / 32 * nt * 3 * kWave) return;
     const int lane = e % kWave, p = (e / kWave) % 3, ct = (e / (3 * kWave)) % nt, kk = e / (3 * kWave * nt);
     const int col = ct * 16 + lane % 16, kb = kk * 32 + 8 * (lane / 16);
@@ -134,23 +146,86 @@ __global__ __launch_bounds__(256) void k_wsplit(const DevModel* __restrict__ dm,
     tab[(dir == 0 ? D.wsf[l] : D.wsb[l]) + e] = u;
 }
 
-hipError_t split_weights(ann_ctx* ctx, int* tick) {
+__device__ __forceinline__ void wchunk_entry(const DevModel* __restrict__ dm, const float* __restrict__ W,
+                                             float* __restrict__ tab, int a, int x) {
+    if (a >= dm->S) return;
+    const NetDesc& D = dm->net[a];
+    const int e = D.wc0 + x;
+    if (e >= D.wc0 + D.wcn) return;
+    const int L = D.nlayer;
+    float v = 0.0f;
+    if (e >= D.wco) {                                       // output neuron
+        const int nlast = L > 1 ? D.nodes[L - 2] : D.nin, m = e - D.wco;
+        if (m < nlast) v = W[D.woff[L - 1] + m];
+        else if (m == vec_pad4(nlast)) v = W[D.boff[L - 1]];
+    } else if (L > 1 && e >= D.wcb[0]) {                    // hidden-layer biases
+        int l = 0;
+        while (l + 1 < L - 1 && e >= D.wcb[l + 1]) ++l;
+        const int j = e - D.wcb[l];
+        if (j < D.nodes[l]) v = W[D.boff[l] + j];
+    } else {                                                // weight rows
+        int l = 0;
+        while (l + 1 < L - 1 && e >= D.wcf[l + 1]) ++l;
+        const int ni = l == 0 ? D.nin : D.nodes[l - 1], no = D.nodes[l];
+        const int r = e - D.wcf[l], k = r / D.wcs[l], j = r % D.wcs[l];
+        if (k < ni && j < no) v = W[D.woff[l] + k * no + j];
+    }
+    tab[e] = v;
+}
+
+__device__ __forceinline__ void wtrans_entry(const DevModel* __restrict__ dm, const float* __restrict__ W,
+                                             float* __restrict__ tab, int seg, int e) {
+    const int a = seg / kMaxLayers, l = seg % kMaxLayers;
+    if (a >= dm->S) return;
+    const NetDesc& D = dm->net[a];
+    if (l >= D.nlayer - 1) return;
+    const int ni = l == 0 ? D.nin : D.nodes[l - 1], no = D.nodes[l], ps = D.wts[l];
+    if (e >= vec_pad4(no * ps)) return;
+    const int j = e / ps, k = e % ps;
+    tab[D.wtf[l] + e] = (j < no && k < ni) ? W[D.woff[l] + k * no + j] : 0.0f;
+}
+
+__global__ __launch_bounds__(256) void k_wtables(WTabArgs A) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y;
+    if (A.tick && y == 0 && e == 0) *A.tick += 1;
+    if (y < A.ys) { wsplit_entry(A.dm, A.W, A.split, y, e); return; }
+    y -= A.ys;
+    if (y < A.yc) { wchunk_entry(A.dm, A.W, A.chunk, y, e); return; }
+    wtrans_entry(A.dm, A.W, A.trans, y - A.yc, e);
+}
+
+// All weight tables after a change of the FP32 weights (split: the matrix-core
+// operand tables too); tick as in k_wtables.
+hipError_t weight_tables(ann_ctx* ctx, int* tick, bool split) {
     const DevModel& M = ctx->hm;
-    int most = 1;
+    const int S = std::max(M.S, 1);
+    int most = std::max(ctx->wchunk_max, 1);
     for (int a = 0; a < M.S; ++a)
         for (int l = 0; l < M.net[a].nlayer - 1; ++l) {
             const int ni = l == 0 ? M.net[a].nin : M.net[a].nodes[l - 1], no = M.net[a].nodes[l];
-            most = std::max(most, (ni + 31) / 32 * ((no + 15) / 16) * 3 * kWave);
-            most = std::max(most, (no + 31) / 32 * ((ni + 15) / 16) * 3 * kWave);
+            if (split) {
+                most = std::max(most, (ni + 31) / 32 * ((no + 15) / 16) * 3 * kWave);
+                most = std::max(most, (no + 31) / 32 * ((ni + 15) / 16) * 3 * kWave);
+            }
+            most = std::max(most, vec_pad4(no * M.net[a].wts[l]));
         }
-    dim3 grid((most + 255) / 256, std::max(M.S, 1) * 2 * kMaxLayers);
-    k_wsplit<<<grid, 256, 0, ctx->stream>>>(ctx->dm, ctx->da.W, ctx->d_wsplit, tick);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) ctx->wsplit_valid = true;
+    WTabArgs A{ctx->dm, ctx->da.W, split ? ctx->d_wsplit : nullptr, ctx->d_wchunk, ctx->d_wtrans, tick,
+               split ? S * 2 * kMaxLayers : 0, S};
+    dim3 grid((most + 255) / 256, A.ys + A.yc + S * kMaxLayers);
+    k_wtables<<<grid, 256, 0, ctx->stream>>>(A);
+    const hipError_t e = hipGetLastError();
+    if (e == hipSuccess) {
+        if (split) ctx->wsplit_valid = true;
+        ctx->wchunk_valid = true;
+    }
+    return e;
+}
+
+hipError_t split_weights(ann_ctx* ctx, int* tick) {
     // the vector-ALU network's tables follow every change of the weights (a
-    // graph that captures a parameter update rebuilds both on every replay)
-    ctx->wchunk_valid = false;
-    return e == hipSuccess ? chunk_weights(ctx) : e;
+    // graph that captures a parameter update rebuilds them on every replay)
+    return weight_tables(ctx, tick, true);
 }
 
 // two wavefronts per 16-atom single-species tile: the column tiles of every

@@ -42,68 +42,10 @@ struct VecArgs {
     float* dEdG;
 };
 
-// Tables of every species (blockIdx.y): the hidden layers' weight rows, their
-// biases and the output neuron's weights and bias, one thread per float
-// including the padding (which is written as zero).
-__global__ __launch_bounds__(256) void k_wchunk(const DevModel* __restrict__ dm, const float* __restrict__ W,
-                                               float* __restrict__ tab) {
-    const int a = blockIdx.y;
-    if (a >= dm->S) return;
-    const NetDesc& D = dm->net[a];
-    const int e = D.wc0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= D.wc0 + D.wcn) return;
-    const int L = D.nlayer;
-    float v = 0.0f;
-    if (e >= D.wco) {                                       // output neuron
-        const int nlast = L > 1 ? D.nodes[L - 2] : D.nin, m = e - D.wco;
-        if (m < nlast) v = W[D.woff[L - 1] + m];
-        else if (m == vec_pad4(nlast)) v = W[D.boff[L - 1]];
-    } else if (L > 1 && e >= D.wcb[0]) {                    // hidden-layer biases
-        int l = 0;
-        while (l + 1 < L - 1 && e >= D.wcb[l + 1]) ++l;
-        const int j = e - D.wcb[l];
-        if (j < D.nodes[l]) v = W[D.boff[l] + j];
-    } else {                                                // weight rows
-        int l = 0;
-        while (l + 1 < L - 1 && e >= D.wcf[l + 1]) ++l;
-        const int ni = l == 0 ? D.nin : D.nodes[l - 1], no = D.nodes[l];
-        const int r = e - D.wcf[l], k = r / D.wcs[l], j = r % D.wcs[l];
-        if (k < ni && j < no) v = W[D.woff[l] + k * no + j];
-    }
-    tab[e] = v;
-}
-
-// Transposed weight rows W_l^T[j][k] (blockIdx.y = species * kMaxLayers +
-// layer), one thread per float including the padding.
-__global__ __launch_bounds__(256) void k_wtrans(const DevModel* __restrict__ dm, const float* __restrict__ W,
-                                               float* __restrict__ tab) {
-    const int a = blockIdx.y / kMaxLayers, l = blockIdx.y % kMaxLayers;
-    if (a >= dm->S) return;
-    const NetDesc& D = dm->net[a];
-    if (l >= D.nlayer - 1) return;
-    const int ni = l == 0 ? D.nin : D.nodes[l - 1], no = D.nodes[l], ps = D.wts[l];
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= vec_pad4(no * ps)) return;
-    const int j = e / ps, k = e % ps;
-    tab[D.wtf[l] + e] = (j < no && k < ni) ? W[D.woff[l] + k * no + j] : 0.0f;
-}
-
-hipError_t chunk_weights(ann_ctx* ctx) {
-    const DevModel& M = ctx->hm;
-    dim3 grid((std::max(ctx->wchunk_max, 1) + 255) / 256, std::max(M.S, 1));
-    k_wchunk<<<grid, 256, 0, ctx->stream>>>(ctx->dm, ctx->da.W, ctx->d_wchunk);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    int most = 1;
-    for (int a = 0; a < M.S; ++a)
-        for (int l = 0; l < M.net[a].nlayer - 1; ++l)
-            most = std::max(most, vec_pad4(M.net[a].nodes[l] * M.net[a].wts[l]));
-    dim3 gt((most + 255) / 256, std::max(M.S, 1) * kMaxLayers);
-    k_wtrans<<<gt, 256, 0, ctx->stream>>>(ctx->dm, ctx->da.W, ctx->d_wtrans);
-    e = hipGetLastError();
-    if (e == hipSuccess) ctx->wchunk_valid = true;
-    return e;
-}
+// The species tables (weight rows with their biases and output neuron, and
+// the transposed weight rows) are written by k_wtables (net_mfma.hip) beside
+// the matrix-core operand tables.
+hipError_t chunk_weights(ann_ctx* ctx) { return weight_tables(ctx, nullptr, false); }
 
 // LDS written by one lane of the wavefront is visible to the others after this
 __device__ __forceinline__ void vsync() {

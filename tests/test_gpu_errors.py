@@ -51,7 +51,7 @@ def test_invalid_shapes():
     from ann_amd.model import init_network
     m = configs.model_water()
     rng = np.random.default_rng(0)
-    m.nets[0] = init_network(80, [65], 3, rng)                  # a layer wider than 64
+    m.nets[0] = init_network(80, [129], 3, rng)                 # a layer wider than 128
     assert _code(lambda: Engine(m, "cuda:0"))[0] == UNSUP
     m = configs.model_water()
     m.nets[0] = init_network(80, [8, 8, 8, 8], 3, rng)          # 5 layers
