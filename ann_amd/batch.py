@@ -44,9 +44,9 @@ class BatchSampler:
 
     def next(self) -> np.ndarray:
         """Indices of the next batch (sorted)."""
-        for i in range(self.size):
-            self.elements[i] = self.data[self.count % self.capacity]
-            self.count += 1
+        # elements[i] = data[count++ % capacity] for i < size, in one step
+        self.elements[:] = self.data[(self.count + np.arange(self.size)) % self.capacity]
+        self.count += self.size
         self.elements.sort()
         if self.count >= self.capacity:
             self.rng.shuffle(self.data)
@@ -80,7 +80,8 @@ class ResidentSet:
         idx = np.asarray(idx, dtype=np.int64)
         n = self.natoms[idx]
         atom_off = np.concatenate([[0], np.cumsum(n)]).astype(np.int32)
-        atoms = np.concatenate([np.arange(self.off[k], self.off[k] + self.natoms[k]) for k in idx])
+        # atom a of batch structure p is atom off[idx[p]] + (a - atom_off[p]) of the set
+        atoms = np.repeat(self.off[idx] - atom_off[:-1], n) + np.arange(int(atom_off[-1]), dtype=np.int64)
         ai = torch.from_numpy(atoms).to(self.device, non_blocking=True)
         si = torch.from_numpy(idx).to(self.device, non_blocking=True)
         out = {"atom_off": atom_off,

@@ -510,7 +510,7 @@ extern "C" int ann_destroy(ann_ctx* ctx) {
                     ctx->d_bin_start, ctx->d_bin_atoms, ctx->d_keys, ctx->d_cnt, ctx->d_flag, ctx->nl.off,
                     ctx->nl.idx, ctx->nl.img, ctx->nl.rev, ctx->nl.geo, ctx->nl.nsp, ctx->d_G, ctx->d_dEdG,
                     ctx->d_Eatom, ctx->d_P, ctx->d_u, ctx->d_v, ctx->d_F, ctx->d_E, ctx->d_gE, ctx->d_atomvir,
-                    ctx->d_scr, ctx->d_gpart, ctx->d_rank, ctx->d_spcount, ctx->d_perm, ctx->d_tiles, ctx->d_spend, ctx->d_grad_tiles, ctx->d_gsplit,
+                    ctx->d_scr, ctx->d_gpart, ctx->d_rank, ctx->d_spcount, ctx->d_rkflag, ctx->d_rkpre, ctx->d_perm, ctx->d_tiles, ctx->d_spend, ctx->d_grad_tiles, ctx->d_gsplit,
                     ctx->d_chunks, ctx->d_chunk_off, ctx->d_part, ctx->d_pos_ref, ctx->d_keatom, ctx->d_md_flag,
                     ctx->d_bin_struc, ctx->d_binrec, ctx->d_adam_t, ctx->d_wsplit, ctx->d_wchunk, ctx->d_wtrans, ctx->d_T,
                     ctx->da.pitems, ctx->da.preflag, ctx->da.inpb64, ctx->da.inpb_net, ctx->da.rad_rs64,
@@ -667,7 +667,7 @@ extern "C" int ann_set_batch(ann_ctx* ctx, const ann_batch* b) {
                            ctx->stream));
     ANN_HIP(hipMemsetAsync(ctx->d_cnt, 0, sizeof(int) * (N + 1), ctx->stream));
     ANN_HIP(struc_index(ctx));
-    ANN_HIP(build_items(ctx));
+    ctx->items_valid = false;      // persistent-dispatch items: built on first use
     ANN_HIP(hipStreamSynchronize(ctx->stream));
     ctx->have_batch = true;
     ctx->batch_gen++;
@@ -934,6 +934,7 @@ extern "C" int ann_energy_forces(ann_ctx* ctx, int form, int with_evac, double* 
     // sizes, the species tiles of the network (set up on the host, so they are
     // brought up to date before the key is taken) and the FP64-evaluated inputs
     if (form != ANN_FORM_PLAIN) RET_HIP(species_perm(ctx), "species permutation");
+    if (ctx->tune_queue == 1) RET_HIP(ensure_items(ctx), "work items");
     const NList& nl = ctx->nl;
     std::vector<uint64_t> key = {
         (uint64_t)form, (uint64_t)with_evac, (uint64_t)(uintptr_t)E, (uint64_t)(uintptr_t)F, (uint64_t)(uintptr_t)V,
@@ -1065,6 +1066,7 @@ extern "C" int ann_train_step(ann_ctx* ctx, int form, const double* Eref, const 
                       ctx->gkey_w[1] == wF && ctx->gkey_w[2] == gamma;
     if (ctx->graph_on && ctx->train_warm && ctx->G_valid) {
         if (!same) {
+            if (ctx->tune_queue == 1) RET_HIP(ensure_items(ctx), "work items");
             if (ctx->gexec) { hipGraphExecDestroy(ctx->gexec); ctx->gexec = nullptr; }
             hipStream_t user = ctx->stream;
             RET_HIP(hipEventRecord(ctx->gev[0], user), "graph event");

@@ -198,6 +198,10 @@ struct ann_ctx {
     bool G_valid = false;             // G of the current batch is cached (training)
     int* d_rank = nullptr;            // rank of each atom within its species
     int* d_spcount = nullptr;         // atoms per species
+    int* d_rkflag = nullptr;          // scratch of the rank scans: flags and prefixes [rank_cap]
+    int* d_rkpre = nullptr;
+    int64_t rank_cap = 0, perm_cap = 0, tiles_cap = 0;
+    bool items_valid = false;         // persistent-dispatch work items of the current batch
     bool rank_valid = false;
     std::vector<int> hm_counts;       // atoms per species (host copy)
     int* d_perm = nullptr;            // species-sorted atom order
@@ -271,7 +275,7 @@ struct ann_ctx {
     int tune_queue = 0;               // moment kernels: 0 grid dispatch, 1 persistent work queues
     int tune_net = 0;                 // networks (TRIPLET, FAST): 0 vector-ALU tiles where they fit, 1 matrix cores
     int tune_fused = 0;               // FAST E+F in the moment form: 0 separate G / network / adjoint kernels, 1 one fused kernel
-    // persistent dispatch: (first atom, count) items per XCD queue, built at ann_set_batch
+    // persistent dispatch: (first atom, count) items per XCD queue, built on first use after ann_set_batch
     int2* d_items = nullptr;
     int* d_qoff = nullptr;
     int* d_queue = nullptr;           // counters, one per queue on its own 128-byte line
@@ -384,6 +388,7 @@ hipError_t refresh_geometry(ann_ctx* ctx);
 int set_neighbors_host(ann_ctx* ctx, const int* off, const int* idx, const int* img, double rc);
 void pair_free(ann_ctx* ctx);
 hipError_t build_items(ann_ctx* ctx);
+hipError_t ensure_items(ann_ctx* ctx);   // build_items unless the current batch has them
 hipError_t wrap_positions(ann_ctx* ctx, double* pos);
 hipError_t md_kick_drift(ann_ctx* ctx, double dt, const double* mass, const double* F, double* vel, double* pos,
                          double lim2);

@@ -1598,7 +1598,18 @@ hipError_t build_items(ann_ctx* ctx) {
     if ((e = hipMemcpyAsync(ctx->d_qoff, qoff.data(), sizeof(int) * (kXcds + 1), hipMemcpyHostToDevice,
                             ctx->stream)) != hipSuccess)
         return e;
-    return hipStreamSynchronize(ctx->stream);     // the host vectors go out of scope
+    e = hipStreamSynchronize(ctx->stream);        // the host vectors go out of scope
+    if (e == hipSuccess) ctx->items_valid = true;
+    return e;
+}
+
+// Work items of the current batch, built on the first persistent dispatch
+// after ann_set_batch (they synchronise the stream, so callers that capture a
+// graph build them before the capture).
+hipError_t ensure_items(ann_ctx* ctx) {
+    if (ctx->items_valid) return hipSuccess;
+    if (ctx->capturing) return hipErrorStreamCaptureUnsupported;
+    return build_items(ctx);
 }
 
 static SymmArgs make_args(ann_ctx* ctx) {
@@ -1658,7 +1669,11 @@ static hipError_t run(ann_ctx* ctx, int form, SymmArgs A) {
         A.smem_wave = (int)sw;
         unsigned grid = (unsigned)((A.N + apb - 1) / apb), block = 64u * apb;
         const size_t sm = sw * apb;
-        const bool queued = ctx->tune_queue == 1 && ctx->d_items;
+        const bool queued = ctx->tune_queue == 1;
+        if (queued) {
+            const hipError_t ei = ensure_items(ctx);
+            if (ei != hipSuccess) return ei;
+        }
         void (*kern)(SymmArgs) =
             queued ? (M.moment_z == 1 ? k_moment_queue<MODE, 1> : M.moment_z == 2 ? k_moment_queue<MODE, 2>
                       : M.moment_z == 3 ? k_moment_queue<MODE, 3> : k_moment_queue<MODE, 4>)

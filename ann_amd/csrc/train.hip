@@ -606,17 +606,34 @@ __global__ void k_take_rank(int N, const int* __restrict__ species, int s, const
     if (i == N) count[s] = pre[N];
 }
 
+// Device int arrays that only grow (a new batch of the same or smaller size
+// reuses them: no allocation, and no implicit synchronisation of a free).
+static hipError_t grow_ints(int*& p, int64_t& cap, int64_t want) {
+    if (p && want <= cap) return hipSuccess;
+    if (p) {
+        const hipError_t e = hipFree(p);
+        p = nullptr;
+        cap = 0;
+        if (e != hipSuccess) return e;
+    }
+    const int64_t c = std::max<int64_t>(want, 1) + std::max<int64_t>(want, 1) / 4;
+    const hipError_t e = hipMalloc(&p, sizeof(int) * c);
+    if (e == hipSuccess) cap = c;
+    return e;
+}
+
 hipError_t species_ranks(ann_ctx* ctx) {
     if (ctx->rank_valid) return hipSuccess;
     const int N = ctx->natoms;
     hipError_t e;
-    if (ctx->d_rank) hipFree(ctx->d_rank);
-    if (ctx->d_spcount) hipFree(ctx->d_spcount);
-    if ((e = hipMalloc(&ctx->d_rank, sizeof(int) * (N + 1))) != hipSuccess) return e;
-    if ((e = hipMalloc(&ctx->d_spcount, sizeof(int) * kMaxSpecies)) != hipSuccess) return e;
-    int *flag = nullptr, *pre = nullptr;
-    if ((e = hipMalloc(&flag, sizeof(int) * (N + 1))) != hipSuccess) return e;
-    if ((e = hipMalloc(&pre, sizeof(int) * (N + 1))) != hipSuccess) return e;
+    int64_t c1 = ctx->rank_cap, c2 = ctx->rank_cap, c3 = ctx->rank_cap;
+    if ((e = grow_ints(ctx->d_rank, c1, N + 1)) != hipSuccess) return e;
+    if ((e = grow_ints(ctx->d_rkflag, c2, N + 1)) != hipSuccess) return e;
+    if ((e = grow_ints(ctx->d_rkpre, c3, N + 1)) != hipSuccess) return e;
+    ctx->rank_cap = std::min(c1, std::min(c2, c3));
+    if (!ctx->d_spcount && (e = hipMalloc(&ctx->d_spcount, sizeof(int) * kMaxSpecies)) != hipSuccess) return e;
+    int* flag = ctx->d_rkflag;
+    int* pre = ctx->d_rkpre;
     for (int s = 0; s < ctx->S; ++s) {
         k_flags<<<(N + 256) / 256, 256, 0, ctx->stream>>>(N, ctx->d_species, s, flag);
         if ((e = exclusive_scan(flag, pre, N + 1, ctx->stream)) != hipSuccess) return e;
@@ -625,8 +642,6 @@ hipError_t species_ranks(ann_ctx* ctx) {
     std::vector<int> counts(ctx->S);
     e = hipMemcpyAsync(counts.data(), ctx->d_spcount, sizeof(int) * ctx->S, hipMemcpyDeviceToHost, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    hipFree(flag);
-    hipFree(pre);
     if (e != hipSuccess) return e;
     // scratch layout: species-major
     int64_t tot = 0;
@@ -663,12 +678,9 @@ hipError_t species_perm(ann_ctx* ctx) {
         acc += ctx->hm_counts[s];
         end[s] = acc;
     }
-    for (int* p : {ctx->d_perm, ctx->d_tiles, ctx->d_spend})
-        if (p) hipFree(p);
-    ctx->d_perm = ctx->d_tiles = ctx->d_spend = nullptr;
-    if ((e = hipMalloc(&ctx->d_perm, sizeof(int) * std::max(N, 1))) != hipSuccess) return e;
-    if ((e = hipMalloc(&ctx->d_tiles, sizeof(int) * std::max<size_t>(tiles.size(), 2))) != hipSuccess) return e;
-    if ((e = hipMalloc(&ctx->d_spend, sizeof(int) * 2 * S)) != hipSuccess) return e;
+    if ((e = grow_ints(ctx->d_perm, ctx->perm_cap, N)) != hipSuccess) return e;
+    if ((e = grow_ints(ctx->d_tiles, ctx->tiles_cap, (int64_t)tiles.size())) != hipSuccess) return e;
+    if (!ctx->d_spend && (e = hipMalloc(&ctx->d_spend, sizeof(int) * 2 * kMaxSpecies)) != hipSuccess) return e;
     std::vector<int> be(base);
     be.insert(be.end(), end.begin(), end.end());
     if ((e = hipMemcpyAsync(ctx->d_spend, be.data(), sizeof(int) * 2 * S, hipMemcpyHostToDevice, ctx->stream)) != hipSuccess) return e;
