@@ -1173,6 +1173,7 @@ extern "C" int ann_autotune(ann_ctx* ctx, int form) {
     if ((rc = check_form(form))) return rc;
     RET_HIP(ensure_work(ctx), "allocation");
     hipError_t e;
+    ctx->tune_fused = 0;          // the E+F timings below run the separate kernels until the fused comparison
     RET_HIP(symm_fwd(ctx, form, ctx->d_G), "symm_fwd");
     RET_HIP(net_fwd(ctx, form, ctx->d_G, ctx->d_Eatom, ctx->d_dEdG), "net_fwd");
     RET_HIP(hipMemsetAsync(ctx->d_u, 0, sizeof(float) * 3 * (size_t)ctx->natoms, ctx->stream), "memset");
@@ -1213,19 +1214,10 @@ extern "C" int ann_autotune(ann_ctx* ctx, int form) {
         if (e != hipSuccess) return fail(ANN_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
         ctx->tune_angular = tt < best_total ? 1 : 0;
     }
-    // fused E+F (FAST, moment form) against the separate kernels, whole E+F steps
-    ctx->tune_fused = 0;
-    if (fused_supported(ctx, form)) {
-        double t[2];
-        for (int q = 0; q < 2; ++q) {
-            ctx->tune_fused = q;
-            t[q] = time_ms(ctx, run_ef, form, 3, e);
-            if (e != hipSuccess) return fail(ANN_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
-        }
-        ctx->tune_fused = t[1] < t[0] ? 1 : 0;
-    }
     // network kernel (TRIPLET, FAST): vector-ALU tiles against the matrix cores,
-    // timed as whole E+F steps
+    // timed as whole E+F steps with the separate kernels (chosen before the
+    // fused comparison, which then times the separate path with its network)
+    ctx->tune_fused = 0;
     if (form != ANN_FORM_PLAIN && vec_net_supported(ctx) && mfma_net_supported(M)) {
         double t[2];
         for (int q = 0; q < 2; ++q) {
@@ -1234,6 +1226,16 @@ extern "C" int ann_autotune(ann_ctx* ctx, int form) {
             if (e != hipSuccess) return fail(ANN_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
         }
         ctx->tune_net = t[1] < t[0] ? 1 : 0;
+    }
+    // fused E+F (FAST, moment form) against the separate kernels, whole E+F steps
+    if (fused_supported(ctx, form)) {
+        double t[2];
+        for (int q = 0; q < 2; ++q) {
+            ctx->tune_fused = q;
+            t[q] = time_ms(ctx, run_ef, form, 3, e);
+            if (e != hipSuccess) return fail(ANN_ERR_HIP, std::string("autotune: ") + hipGetErrorString(e));
+        }
+        ctx->tune_fused = t[1] < t[0] ? 1 : 0;
     }
     // neighbour search blocking
     {

@@ -514,6 +514,9 @@ def main():
     ap.add_argument("--md-steps", type=int, default=100)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 256 MiB device-to-device copy three times (calibrates HBM counters)")
+    ap.add_argument("--ef-loop", type=int, default=0, metavar="N",
+                    help="only run N back-to-back E+F steps (neighbour build + energy_forces, FAST, after the "
+                         "autotuner and five untimed steps) on water 64 x 192, for a kernel-trace timeline")
     args = ap.parse_args()
     if args.copy_probe:
         import torch
@@ -538,6 +541,23 @@ def main():
     eng.set_batch(batch)
     eng.neighbors()
     tuning = eng.autotune(FAST)          # atoms per workgroup, kernel family, search blocking
+    if args.ef_loop > 0:
+        dev = eng.device
+        o = (torch.empty(batch.nstruc, dtype=torch.float64, device=dev),
+             torch.empty((batch.ntotal, 3), dtype=torch.float64, device=dev),
+             torch.empty((batch.nstruc, 3, 3), dtype=torch.float64, device=dev))
+        for _ in range(5):
+            eng.neighbors()
+            eng.energy_forces(FAST, out=o)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.ef_loop):
+            eng.neighbors()
+            eng.energy_forces(FAST, out=o)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / args.ef_loop
+        print(json.dumps({"ef_loop_steps": args.ef_loop, "ms_per_step": dt * 1e3, "tuning": tuning}))
+        return
     nl = eng.nlist()
     flops = work_model(model, batch, nl)
     dev = eng.device
