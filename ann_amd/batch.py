@@ -16,9 +16,14 @@ while every relation above holds.
 in HBM and gathers the structures of a step into contiguous batch arrays with
 device index operations, so that a rotating-batch training step moves only
 the index lists across PCIe.
+
+``validation_error`` is the trainer's error report on a set of structures
+(the validation loss and RMSE of NNPTEFR::error_cost_o1,
+src/nnp/nnptefr2.cpp:1375-1411, and their averages at :986-991).
 """
 from __future__ import annotations
 
+import math
 from typing import Optional
 
 import numpy as np
@@ -91,3 +96,61 @@ class ResidentSet:
         out["wE"] = None if self.wE is None else self.wE.index_select(0, si)
         out["wF"] = None if self.wF is None else self.wF.index_select(0, si)
         return out
+
+
+def _energy_norm(norm: str, n: np.ndarray) -> np.ndarray:
+    """Per-structure energy normalisation (src/nnp/nnptefr2.cpp:600-624)."""
+    norm = norm.upper()
+    if norm == "IDENTITY":
+        return np.ones_like(n)
+    if norm == "LINEAR":
+        return 1.0 / n
+    if norm == "SQRT":
+        return 1.0 / np.sqrt(n)
+    if norm == "CBRT":
+        return 1.0 / np.cbrt(n)
+    if norm == "LOG":
+        return 1.0 / (np.log(n) + 1.0)
+    raise ValueError("Invalid normalization method.")
+
+
+def _energy_loss(loss: str, delta: float, dU: np.ndarray) -> np.ndarray:
+    """Loss of a normalised energy error (src/nnp/nnptefr2.cpp:1392-1410)."""
+    loss = loss.upper()
+    if loss == "MSE":
+        return 0.5 * dU * dU
+    if loss == "MAE":
+        return np.abs(dU)
+    a = dU / delta
+    q = np.sqrt(1.0 + a * a)
+    if loss == "HUBER":
+        return delta * delta * (q - 1.0)
+    if loss == "ASINH":
+        return delta * delta * (1.0 - q + a * np.arcsinh(a))
+    raise ValueError("Invalid loss function.")
+
+
+def validation_error(eng, rs: ResidentSet, idx, loss: str = "MSE", delta: float = 1.0, norm: str = "LINEAR",
+                     form: int = 1, chunk: int = 512) -> dict:   # form 1: FAST
+    """Validation loss and RMSE of structures ``idx`` of a resident set with the
+    engine's current parameters, as the reference trainer reports them
+    (src/nnp/nnptefr2.cpp:1375-1411, averaged at :986-991): for each structure
+    the network energy pe (no vacuum energies, like the labels), dE = pe - E_ref,
+    dU = dE * norm(n); loss = sum loss(dU) / n_val and
+    rmse = sqrt(sum (dE / n)^2 / n_val).  The energies are computed on the GPU
+    (energy only: no adjoint) in chunks of ``chunk`` structures; the engine is
+    left holding the last chunk as its batch."""
+    idx = np.asarray(idx, dtype=np.int64)
+    if idx.size == 0:
+        raise ValueError("NNPTEFR::train(int): No validation data provided.")
+    lsum, rsum = 0.0, 0.0
+    for c0 in range(0, idx.size, chunk):
+        b = rs.gather(idx[c0:c0 + chunk])
+        eng.set_batch_tensors(b["atom_off"], b["pos"], b["cells"], b["species"])
+        eng.neighbors()
+        E = eng.energies(form, with_evac=False).cpu().numpy()
+        n = np.diff(b["atom_off"]).astype(np.float64)
+        dE = E - b["energy"].cpu().numpy()
+        lsum += float(_energy_loss(loss, delta, dE * _energy_norm(norm, n)).sum())
+        rsum += float(((dE / n) ** 2).sum())
+    return {"loss": lsum / idx.size, "rmse": math.sqrt(rsum / idx.size), "structures": int(idx.size)}

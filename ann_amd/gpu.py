@@ -305,6 +305,13 @@ class Engine:
         _check(lib().ann_energy_forces(self.h, form, int(with_evac), _ptr(E), _ptr(F), _ptr(V)))
         return E, F, V
 
+    def energies(self, form: int = FAST, with_evac: bool = True, out: Optional[torch.Tensor] = None) -> torch.Tensor:
+        """Structure energies only (ann_energy_forces without force or virial
+        outputs: G, the networks and the per-structure sums, no adjoint)."""
+        E = out if out is not None else torch.empty(self.S, dtype=torch.float64, device=self.device)
+        _check(lib().ann_energy_forces(self.h, form, int(with_evac), _ptr(E), None, None))
+        return E
+
     def tangent(self, u: torch.Tensor, form: int = FAST) -> torch.Tensor:
         v = torch.empty((self.N, self.ldg), dtype=torch.float32, device=self.device)
         _check(lib().ann_symm_tangent(self.h, form, _ptr(u.to(torch.float32).contiguous()), _ptr(v)))

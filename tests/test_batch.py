@@ -37,3 +37,49 @@ def test_reference_argument_checks():
         BatchSampler(-1, 4)
     s = BatchSampler(0, 3)
     assert len(s.next()) == 0
+
+
+class _OracleEngine:
+    """Engine stand-in for the host side of validation_error: the energies of
+    the batch it is given come from the FP64 oracle (no vacuum energies)."""
+
+    def __init__(self, orc, model):
+        self.orc, self.model = orc, model
+
+    def set_batch_tensors(self, atom_off, pos, cells, species):
+        from ann_amd.configs import Batch
+        self.b = Batch(np.diff(atom_off), pos.numpy(), cells.numpy().reshape(-1, 3, 3), species.numpy())
+
+    def neighbors(self):
+        pass
+
+    def energies(self, form, with_evac=True):
+        import torch
+        E, _, _ = self.orc.compute_batch(self.b, nthreads=2)
+        ev = np.array([sum(self.model.evac[s] for s in self.b.struc(k)[2]) for k in range(self.b.nstruc)])
+        return torch.tensor(E - (0.0 if with_evac else ev))
+
+
+@pytest.mark.parametrize("loss,norm", [("MSE", "LINEAR"), ("MAE", "IDENTITY"), ("HUBER", "SQRT"),
+                                       ("ASINH", "CBRT"), ("HUBER", "LOG")])
+def test_validation_error_formulas_against_oracle(oracle_mod, loss, norm):
+    """The report's loss (src/nnp/nnptefr2.cpp:1375-1411, averaged at :986-991)
+    on the oracle's own energies equals the oracle's energy loss with zero force
+    weight, and its RMSE the mean squared per-atom error, to FP64 rounding."""
+    from ann_amd import configs
+    from ann_amd.batch import ResidentSet, validation_error
+    from ann_amd.gpu import Engine
+    model = configs.model_lips()
+    full = configs.batch_lips(4, seed=98)
+    orc = oracle_mod.Oracle(model)
+    E0, F0, _ = orc.compute_batch(full, nthreads=2)
+    evac = np.array([sum(model.evac[s] for s in full.struc(k)[2]) for k in range(full.nstruc)])
+    Eref = E0 - evac + np.random.default_rng(99).normal(size=full.nstruc) * 0.05 * full.natoms
+    rs = ResidentSet(full, Eref, np.zeros_like(F0), device="cpu")
+    idx = np.array([3, 0, 2])
+    v = validation_error(_OracleEngine(orc, model), rs, idx, loss, 0.3, norm, chunk=2)
+    sub = full.subset(idx)
+    _, _, Eo, _, parts = orc.train_loss_grad(sub, Eref[idx], np.zeros((sub.ntotal, 3)), 1.0, 0.0,
+                                             loss=Engine.LOSS[loss], delta=0.3, norm=Engine.NORM[norm])
+    assert np.isclose(v["loss"], parts[0], rtol=1e-12, atol=0), (v, parts)
+    assert np.isclose(v["rmse"], np.sqrt(np.mean(((Eo - Eref[idx]) / sub.natoms) ** 2)), rtol=1e-12, atol=0)
