@@ -92,7 +92,9 @@ class ResidentSet:
         out = {"atom_off": atom_off,
                "pos": self.pos.index_select(0, ai), "cells": self.cells.index_select(0, si),
                "species": self.species.index_select(0, ai),
-               "energy": self.energy.index_select(0, si), "forces": self.forces.index_select(0, ai)}
+               "energy": self.energy.index_select(0, si), "forces": self.forces.index_select(0, ai),
+               # host copies for Engine.set_batch_host (no device readback in the set-up)
+               "cells_host": self.batch.cells[idx], "species_host": self.batch.species[atoms]}
         out["wE"] = None if self.wE is None else self.wE.index_select(0, si)
         out["wF"] = None if self.wF is None else self.wF.index_select(0, si)
         return out
@@ -146,7 +148,7 @@ def validation_error(eng, rs: ResidentSet, idx, loss: str = "MSE", delta: float 
     lsum, rsum = 0.0, 0.0
     for c0 in range(0, idx.size, chunk):
         b = rs.gather(idx[c0:c0 + chunk])
-        eng.set_batch_tensors(b["atom_off"], b["pos"], b["cells"], b["species"])
+        eng.set_batch_host(b["atom_off"], b["pos"], b["cells_host"], b["species_host"])
         eng.neighbors()
         E = eng.energies(form, with_evac=False).cpu().numpy()
         n = np.diff(b["atom_off"]).astype(np.float64)

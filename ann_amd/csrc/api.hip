@@ -68,6 +68,33 @@ bool mfma_net_supported(const DevModel& M);
 hipError_t train_step_impl(ann_ctx* ctx, int form, const double* Eref, const double* Fref, double wE, double wF,
                            double gamma, double* loss, double* grad);
 
+hipError_t staged_upload(ann_ctx* ctx, int slot, void* dst, const void* src, size_t bytes) {
+    if (bytes == 0) return hipSuccess;
+    ann_ctx::Staged& s = ctx->staged[slot];
+    hipError_t e;
+    if (s.pending) {
+        if ((e = hipEventSynchronize(s.ev)) != hipSuccess) return e;
+        s.pending = false;
+    }
+    if (bytes > s.cap) {
+        if (s.p) {
+            e = hipHostFree(s.p);
+            s.p = nullptr;
+            s.cap = 0;
+            if (e != hipSuccess) return e;
+        }
+        const size_t c = bytes + bytes / 4;
+        if ((e = hipHostMalloc(&s.p, c)) != hipSuccess) return e;
+        s.cap = c;
+    }
+    if (!s.ev && (e = hipEventCreateWithFlags(&s.ev, hipEventDisableTiming)) != hipSuccess) return e;
+    std::memcpy(s.p, src, bytes);
+    if ((e = hipMemcpyAsync(dst, s.p, bytes, hipMemcpyHostToDevice, ctx->stream)) != hipSuccess) return e;
+    if ((e = hipEventRecord(s.ev, ctx->stream)) != hipSuccess) return e;
+    s.pending = true;
+    return hipSuccess;
+}
+
 template <typename T>
 static hipError_t dalloc(T*& p, size_t n) {
     if (p) { hipFree(p); p = nullptr; }
@@ -504,7 +531,7 @@ extern "C" int ann_destroy(ann_ctx* ctx) {
     if (!ctx) return ANN_OK;
     hipDeviceSynchronize();
     pair_free(ctx);
-    void* ptrs[] = {ctx->dm, ctx->da.rad_rs, ctx->da.rad_eta, ctx->da.ang_eta, ctx->da.ang_zeta, ctx->da.ang_lam,
+    void* ptrs[] = {ctx->d_off_own, ctx->d_cells_own, ctx->d_species_own, ctx->dm, ctx->da.rad_rs, ctx->da.rad_eta, ctx->da.ang_eta, ctx->da.ang_zeta, ctx->da.ang_lam,
                     ctx->da.W, ctx->da.inpw, ctx->da.inpb, ctx->da.params, ctx->da.adam_m, ctx->da.adam_v,
                     ctx->d_struc_of, ctx->d_cellinfo, ctx->d_cellmat, ctx->d_bin_of, ctx->d_shift, ctx->d_bin_cnt,
                     ctx->d_bin_start, ctx->d_bin_atoms, ctx->d_keys, ctx->d_cnt, ctx->d_flag, ctx->nl.off,
@@ -518,6 +545,10 @@ extern "C" int ann_destroy(ann_ctx* ctx) {
                     ctx->d_queue};
     for (void* p : ptrs)
         if (p) hipFree(p);
+    for (ann_ctx::Staged& st : ctx->staged) {
+        if (st.p) hipHostFree(st.p);
+        if (st.ev) hipEventDestroy(st.ev);
+    }
     if (ctx->h_rb) hipHostFree(ctx->h_rb);
     for (hipEvent_t e : ctx->evpool) hipEventDestroy(e);
     if (ctx->gexec) hipGraphExecDestroy(ctx->gexec);
@@ -661,14 +692,13 @@ extern "C" int ann_set_batch(ann_ctx* ctx, const ann_batch* b) {
         ctx->chunk_cap = cap;
     }
     ctx->nchunks = nch;
-    ANN_HIP(hipMemcpyAsync(ctx->d_chunks, chunks.data(), sizeof(int) * chunks.size(), hipMemcpyHostToDevice,
-                           ctx->stream));
-    ANN_HIP(hipMemcpyAsync(ctx->d_chunk_off, chunk_off.data(), sizeof(int) * (S + 1), hipMemcpyHostToDevice,
-                           ctx->stream));
+    ANN_HIP(staged_upload(ctx, kStChunks, ctx->d_chunks, chunks.data(), sizeof(int) * chunks.size()));
+    ANN_HIP(staged_upload(ctx, kStChunkOff, ctx->d_chunk_off, chunk_off.data(), sizeof(int) * (S + 1)));
     ANN_HIP(hipMemsetAsync(ctx->d_cnt, 0, sizeof(int) * (N + 1), ctx->stream));
     ANN_HIP(struc_index(ctx));
     ctx->items_valid = false;      // persistent-dispatch items: built on first use
-    ANN_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->h_cells_valid = false;    // host copies: only with ann_set_batch_host
+    ctx->h_counts_valid = false;
     ctx->have_batch = true;
     ctx->batch_gen++;
     ctx->have_nlist = false;
@@ -677,6 +707,43 @@ extern "C" int ann_set_batch(ann_ctx* ctx, const ann_batch* b) {
     ctx->G_valid = false;
     ctx->rank_valid = false;
     ctx->perm_valid = false;
+    return ANN_OK;
+}
+
+extern "C" int ann_set_batch_host(ann_ctx* ctx, int nstruc, const int* atom_off, const double* d_pos,
+                                  const double* cells, const int* species) {
+    CHECK_CTX(ctx);
+    if (nstruc < 1 || !atom_off || !d_pos || !cells || !species) return fail(ANN_ERR_ARG, "invalid batch");
+    if (atom_off[0] != 0) return fail(ANN_ERR_ARG, "atom_off[0] must be 0");
+    for (int s = 0; s < nstruc; ++s)
+        if (atom_off[s + 1] <= atom_off[s]) return fail(ANN_ERR_ARG, "every structure needs at least one atom");
+    const int N = atom_off[nstruc];
+    std::vector<int> count(ctx->S, 0);
+    for (int i = 0; i < N; ++i) {
+        if (species[i] < 0 || species[i] >= ctx->S) return fail(ANN_ERR_ARG, "species index out of range");
+        ++count[species[i]];
+    }
+    // the context's own device copies (grow-only; a free waits for their
+    // previous readers), filled through the pinned staging without a wait
+    if (nstruc + 1 > ctx->own_struc_cap) {
+        ANN_HIP(dalloc(ctx->d_off_own, 2 * ((size_t)nstruc + 1)));
+        ANN_HIP(dalloc(ctx->d_cells_own, 18 * (size_t)nstruc));
+        ctx->own_struc_cap = 2 * (nstruc + 1);
+    }
+    if (N > ctx->own_atom_cap) {
+        ANN_HIP(dalloc(ctx->d_species_own, 2 * (size_t)N));
+        ctx->own_atom_cap = 2 * N;
+    }
+    ANN_HIP(staged_upload(ctx, kStAtomOff, ctx->d_off_own, atom_off, sizeof(int) * ((size_t)nstruc + 1)));
+    ANN_HIP(staged_upload(ctx, kStCells, ctx->d_cells_own, cells, sizeof(double) * 9 * (size_t)nstruc));
+    ANN_HIP(staged_upload(ctx, kStSpecies, ctx->d_species_own, species, sizeof(int) * (size_t)N));
+    const ann_batch b{nstruc, atom_off, ctx->d_off_own, d_pos, ctx->d_cells_own, ctx->d_species_own};
+    const int rc = ann_set_batch(ctx, &b);
+    if (rc) return rc;
+    ctx->h_cells.assign(cells, cells + 9 * (size_t)nstruc);
+    ctx->h_species_count = count;
+    ctx->h_cells_valid = true;
+    ctx->h_counts_valid = true;
     return ANN_OK;
 }
 

@@ -204,6 +204,25 @@ struct ann_ctx {
     bool items_valid = false;         // persistent-dispatch work items of the current batch
     bool rank_valid = false;
     std::vector<int> hm_counts;       // atoms per species (host copy)
+    // host copies given with the batch (ann_set_batch_host): the bin grid and
+    // the species tiles are then set up without reading the device
+    std::vector<double> h_cells;      // [S][9]
+    std::vector<int> h_species_count; // atoms per species
+    bool h_cells_valid = false, h_counts_valid = false;
+    // pinned staging of the set-up uploads, one slot per upload site (see
+    // staged_upload); a slot is rewritten only after its previous copy ended
+    struct Staged {
+        void* p = nullptr;
+        size_t cap = 0;
+        hipEvent_t ev = nullptr;
+        bool pending = false;
+    };
+    Staged staged[12];                // kNStaged slots
+    // the batch arrays of ann_set_batch_host (context-owned device copies)
+    int* d_off_own = nullptr;
+    double* d_cells_own = nullptr;
+    int* d_species_own = nullptr;
+    int own_struc_cap = 0, own_atom_cap = 0;
     int* d_perm = nullptr;            // species-sorted atom order
     int* d_tiles = nullptr;           // (species, start) per 16-atom tile
     int* d_spend = nullptr;           // [2S]: start then end of each species block in d_perm
@@ -387,6 +406,15 @@ hipError_t refresh_geometry(ann_ctx* ctx);
 // the caller's host CSR list (off, idx, img [P][3]) for the current batch (ANN status)
 int set_neighbors_host(ann_ctx* ctx, const int* off, const int* idx, const int* img, double rc);
 void pair_free(ann_ctx* ctx);
+// Upload slots of staged_upload
+enum { kStChunks, kStChunkOff, kStCellInfo, kStRinv, kStBinStruc, kStTiles, kStSpend, kStItems, kStQoff,
+       kStAtomOff, kStCells, kStSpecies, kNStaged };
+static_assert(kNStaged <= 12, "one staging slot per upload site");
+// Host-to-device copy of bytes through the context's pinned staging slot, on
+// the context's stream, without waiting for the copy: the host data may be
+// released on return (it is copied into the slot first; the slot's previous
+// copy is waited for before that).
+hipError_t staged_upload(ann_ctx* ctx, int slot, void* dst, const void* src, size_t bytes);
 hipError_t build_items(ann_ctx* ctx);
 hipError_t ensure_items(ann_ctx* ctx);   // build_items unless the current batch has them
 hipError_t wrap_positions(ann_ctx* ctx, double* pos);

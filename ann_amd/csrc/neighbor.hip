@@ -699,9 +699,15 @@ static hipError_t grid_setup(ann_ctx* ctx, double rc) {
     if (ctx->grid_valid && ctx->grid_rc == rc) return hipSuccess;
     const int S = ctx->nstruc;
     hipStream_t st = ctx->stream;
-    std::vector<double> cells(9 * (size_t)S);
-    HIPCHK(hipMemcpyAsync(cells.data(), ctx->d_cells, sizeof(double) * 9 * S, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipStreamSynchronize(st));
+    // the cells: the host copy given with the batch, else read from the device
+    std::vector<double> cells;
+    if (ctx->h_cells_valid) {
+        cells = ctx->h_cells;
+    } else {
+        cells.resize(9 * (size_t)S);
+        HIPCHK(hipMemcpyAsync(cells.data(), ctx->d_cells, sizeof(double) * 9 * S, hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+    }
     std::vector<CellInfo> info(S);
     std::vector<double> rinv(9 * (size_t)S);
     std::vector<int> bin_struc;
@@ -748,10 +754,9 @@ static hipError_t grid_setup(ann_ctx* ctx, double rc) {
         ctx->nb_dirty = true;
     }
     ctx->d_bin_ovf = ctx->d_bin_cnt + 2 * (size_t)ctx->bin_cap;
-    HIPCHK(hipMemcpyAsync(ctx->d_cellinfo, info.data(), sizeof(CellInfo) * S, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(ctx->d_cellmat, rinv.data(), sizeof(double) * 9 * S, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(ctx->d_bin_struc, bin_struc.data(), sizeof(int) * bin_struc.size(), hipMemcpyHostToDevice, st));
-    HIPCHK(hipStreamSynchronize(st));      // the host vectors go out of scope
+    HIPCHK(staged_upload(ctx, kStCellInfo, ctx->d_cellinfo, info.data(), sizeof(CellInfo) * S));
+    HIPCHK(staged_upload(ctx, kStRinv, ctx->d_cellmat, rinv.data(), sizeof(double) * 9 * S));
+    HIPCHK(staged_upload(ctx, kStBinStruc, ctx->d_bin_struc, bin_struc.data(), sizeof(int) * bin_struc.size()));
     ctx->grid_valid = true;
     ctx->grid_rc = rc;
     ctx->grid_nbins = nbins;

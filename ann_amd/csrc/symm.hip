@@ -1592,19 +1592,16 @@ hipError_t build_items(ann_ctx* ctx) {
         if ((e = hipMalloc(&ctx->d_items, sizeof(int2) * items.size())) != hipSuccess) return e;
         ctx->items_cap = (int)items.size();
     }
-    if ((e = hipMemcpyAsync(ctx->d_items, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice,
-                            ctx->stream)) != hipSuccess)
+    if ((e = staged_upload(ctx, kStItems, ctx->d_items, items.data(), sizeof(int2) * items.size())) != hipSuccess)
         return e;
-    if ((e = hipMemcpyAsync(ctx->d_qoff, qoff.data(), sizeof(int) * (kXcds + 1), hipMemcpyHostToDevice,
-                            ctx->stream)) != hipSuccess)
+    if ((e = staged_upload(ctx, kStQoff, ctx->d_qoff, qoff.data(), sizeof(int) * (kXcds + 1))) != hipSuccess)
         return e;
-    e = hipStreamSynchronize(ctx->stream);        // the host vectors go out of scope
-    if (e == hipSuccess) ctx->items_valid = true;
-    return e;
+    ctx->items_valid = true;
+    return hipSuccess;
 }
 
 // Work items of the current batch, built on the first persistent dispatch
-// after ann_set_batch (they synchronise the stream, so callers that capture a
+// after ann_set_batch (host set-up and uploads, so callers that capture a
 // graph build them before the capture).
 hipError_t ensure_items(ann_ctx* ctx) {
     if (ctx->items_valid) return hipSuccess;

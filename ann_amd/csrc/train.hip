@@ -639,10 +639,15 @@ hipError_t species_ranks(ann_ctx* ctx) {
         if ((e = exclusive_scan(flag, pre, N + 1, ctx->stream)) != hipSuccess) return e;
         k_take_rank<<<(N + 256) / 256, 256, 0, ctx->stream>>>(N, ctx->d_species, s, pre, ctx->d_rank, ctx->d_spcount);
     }
+    // the counts: from the host species given with the batch, else read back
     std::vector<int> counts(ctx->S);
-    e = hipMemcpyAsync(counts.data(), ctx->d_spcount, sizeof(int) * ctx->S, hipMemcpyDeviceToHost, ctx->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) return e;
+    if (ctx->h_counts_valid) {
+        counts = ctx->h_species_count;
+    } else {
+        e = hipMemcpyAsync(counts.data(), ctx->d_spcount, sizeof(int) * ctx->S, hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) return e;
+    }
     // scratch layout: species-major
     int64_t tot = 0;
     ctx->rank_valid = true;
@@ -683,15 +688,13 @@ hipError_t species_perm(ann_ctx* ctx) {
     if (!ctx->d_spend && (e = hipMalloc(&ctx->d_spend, sizeof(int) * 2 * kMaxSpecies)) != hipSuccess) return e;
     std::vector<int> be(base);
     be.insert(be.end(), end.begin(), end.end());
-    if ((e = hipMemcpyAsync(ctx->d_spend, be.data(), sizeof(int) * 2 * S, hipMemcpyHostToDevice, ctx->stream)) != hipSuccess) return e;
-    if (!tiles.empty() &&
-        (e = hipMemcpyAsync(ctx->d_tiles, tiles.data(), sizeof(int) * tiles.size(), hipMemcpyHostToDevice, ctx->stream)) != hipSuccess)
+    if ((e = staged_upload(ctx, kStSpend, ctx->d_spend, be.data(), sizeof(int) * 2 * S)) != hipSuccess) return e;
+    if ((e = staged_upload(ctx, kStTiles, ctx->d_tiles, tiles.data(), sizeof(int) * tiles.size())) != hipSuccess)
         return e;
     k_perm<<<(N + 255) / 256, 256, 0, ctx->stream>>>(N, ctx->d_species, ctx->d_rank, ctx->d_spend, ctx->d_perm);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     // spend[] must hold the END of each species block: shift the uploaded (base, end) pair
     ctx->d_spend_end = ctx->d_spend + S;
-    if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return e;
     ctx->ntiles = (int)tiles.size() / 2;
     ctx->perm_valid = true;
     return hipSuccess;

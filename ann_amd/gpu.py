@@ -51,7 +51,7 @@ class BatchDesc(C.Structure):
 
 
 EXPORTS = ["ann_last_error", "ann_create", "ann_destroy", "ann_set_stream", "ann_info", "ann_set_params",
-           "ann_set_scaling", "ann_set_batch", "ann_neighbors", "ann_nlist_size", "ann_nlist_copy",
+           "ann_set_scaling", "ann_set_batch", "ann_set_batch_host", "ann_neighbors", "ann_nlist_size", "ann_nlist_copy",
            "ann_symm_fwd", "ann_net_fwd", "ann_symm_bwd", "ann_force_reduce", "ann_energy_forces",
            "ann_symm_tangent", "ann_net_bwd2", "ann_train_step", "ann_get_params", "ann_profile",
            "ann_profile_get", "ann_profile_only", "ann_refresh_geometry", "ann_wrap_positions", "ann_md_step", "ann_train_graph", "ann_ef_graph",
@@ -204,6 +204,21 @@ class Engine:
         self._bdesc = BatchDesc(self.S, self._off_h.ctypes.data_as(P_I), _ptr(self.d_off), _ptr(self.d_pos),
                                 _ptr(self.d_cells), _ptr(self.d_species))
         _check(lib().ann_set_batch(self.h, C.byref(self._bdesc)))
+
+    def set_batch_host(self, atom_off: np.ndarray, pos: torch.Tensor, cells: np.ndarray, species: np.ndarray) -> None:
+        """ann_set_batch_host: positions [N, 3] FP64 in device memory; atom
+        offsets, cells and species from host arrays (copied by the library
+        without a host wait, so the set-up queues behind earlier GPU work)."""
+        self.batch = None
+        self._off_h = np.ascontiguousarray(atom_off, dtype=np.int32)
+        cells = np.ascontiguousarray(cells, dtype=np.float64).reshape(-1, 9)
+        species = np.ascontiguousarray(species, dtype=np.int32)
+        self.d_pos = pos.contiguous()
+        self.d_off = self.d_cells = self.d_species = None
+        self.N = int(self._off_h[-1])
+        self.S = len(self._off_h) - 1
+        _check(lib().ann_set_batch_host(self.h, self.S, self._off_h.ctypes.data_as(P_I), _ptr(self.d_pos),
+                                        cells.ctypes.data_as(P_D), species.ctypes.data_as(P_I)))
 
     def neighbors(self, rc: float = 0.0) -> None:
         _check(lib().ann_neighbors(self.h, C.c_double(rc)))

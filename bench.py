@@ -443,9 +443,10 @@ def train_cpu_baseline():
 
 def run_rotating(steps, warmup, nset=512, size=128):
     """Training with a new batch every step: the reference's sampler draws 128
-    of 512 Li/P/S structures resident in HBM, the batch is gathered on the
-    GPU, and the step includes ann_set_batch, the neighbour build, G and the
-    training step (FAST, Adam)."""
+    of 512 Li/P/S structures resident in HBM, the positions are gathered on the
+    GPU (offsets, cells and species from the host copies), and the step
+    includes ann_set_batch_host, the neighbour build, G and the training step
+    (FAST, Adam)."""
     import torch
     from ann_amd.batch import BatchSampler, ResidentSet
     from ann_amd.gpu import Engine, FAST
@@ -458,7 +459,7 @@ def run_rotating(steps, warmup, nset=512, size=128):
 
     def step():
         b = rs.gather(sampler.next())
-        eng.set_batch_tensors(b["atom_off"], b["pos"], b["cells"], b["species"])
+        eng.set_batch_host(b["atom_off"], b["pos"], b["cells_host"], b["species_host"])
         eng.neighbors()
         eng.train_step(b["energy"], b["forces"], 1.0, 0.1, 1e-3, FAST, out=out)
         return b

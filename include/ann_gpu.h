@@ -120,6 +120,17 @@ int ann_set_scaling(ann_ctx* ctx, int species, const double* inpw, const double*
  * rev: (i->j, n) <-> (j->i, -n).  The pointers of the batch must stay valid
  * until the next ann_set_batch. */
 int ann_set_batch(ann_ctx* ctx, const ann_batch* batch);
+/* As ann_set_batch, with the atom offsets [nstruc+1], cells [nstruc][9] and
+ * species [N] in HOST memory (copied into the context's own device arrays on
+ * its stream, without waiting) and only the positions d_pos [N][3] in device
+ * memory.  The bin grid of the next build and the species tiles of the
+ * networks are then set up from the host copies, so the batch set-up waits
+ * for nothing on the device: a training loop that draws a new batch every
+ * step (Batch, src/ml/batch.hpp:18-55, at src/nnp/nnptefr2.cpp:1309-1315)
+ * queues it behind the previous step.  The host arrays may be released on
+ * return; d_pos must stay valid until the next set-up. */
+int ann_set_batch_host(ann_ctx* ctx, int nstruc, const int* atom_off, const double* d_pos, const double* cells,
+                       const int* species);
 int ann_neighbors(ann_ctx* ctx, double rc);
 /* A neighbour list supplied by the caller, in place of ann_neighbors (the
  * reference's verlet::List reuse, src/nnp/nnp.cpp:377-427, 522-587, 667-736, and

@@ -46,9 +46,9 @@ class _OracleEngine:
     def __init__(self, orc, model):
         self.orc, self.model = orc, model
 
-    def set_batch_tensors(self, atom_off, pos, cells, species):
+    def set_batch_host(self, atom_off, pos, cells, species):
         from ann_amd.configs import Batch
-        self.b = Batch(np.diff(atom_off), pos.numpy(), cells.numpy().reshape(-1, 3, 3), species.numpy())
+        self.b = Batch(np.diff(atom_off), pos.numpy(), np.asarray(cells).reshape(-1, 3, 3), np.asarray(species))
 
     def neighbors(self):
         pass

@@ -110,3 +110,67 @@ def test_rotating_batches_from_a_resident_set(oracle_mod):
         assert np.allclose(lossg.cpu().numpy(), parts, rtol=1e-6)
         err = float(np.linalg.norm(g.cpu().numpy() - go) / np.linalg.norm(go))
         assert err <= 1e-5, (step, err)
+
+
+def test_host_set_up_matches_device_set_up(oracle_mod):
+    """ann_set_batch_host (offsets, cells and species from the host, no device
+    readback in the set-up) against ann_set_batch on the same device arrays:
+    bitwise equal energies, forces, losses and gradients over four rotating
+    batches with Adam updates between them (batches of different sizes and
+    compositions, so the grid, tiles and scratch are rebuilt), and the oracle's
+    loss and gradient on the first step."""
+    import torch
+    from ann_amd import configs
+    from ann_amd.batch import BatchSampler, ResidentSet
+    from ann_amd.gpu import Engine, FAST
+    model = configs.model_lips()
+    full = configs.Batch.concat([configs.batch_lips(4, seed=93), configs.batch_lips(4, seed=95, a=5.9)])
+    orc = oracle_mod.Oracle(model)
+    Eref, Fref = _labels(orc, model, full, np.random.default_rng(7))
+    rs = ResidentSet(full, Eref, Fref)
+    e1, e2 = Engine(model, "cuda:0"), Engine(model, "cuda:0")
+    sizes = [3, 5, 2, 4]
+    rng = np.random.default_rng(8)
+    for step, k in enumerate(sizes):
+        idx = np.sort(rng.choice(full.nstruc, size=k, replace=False))
+        print(f"step {step}: structures {idx.tolist()}", flush=True)
+        b = rs.gather(idx)
+        e1.set_batch_tensors(b["atom_off"], b["pos"], b["cells"], b["species"])
+        e2.set_batch_host(b["atom_off"], b["pos"], b["cells_host"], b["species_host"])
+        outs = []
+        for e in (e1, e2):
+            e.neighbors()
+            E, F, V = e.energy_forces(FAST)
+            lossg, g = e.train_step(b["energy"], b["forces"], 1.0, 0.1, 1e-3)
+            outs.append((E.clone(), F.clone(), V.clone(), lossg.clone(), g.clone()))
+        torch.cuda.synchronize()
+        for a, c in zip(outs[0], outs[1]):
+            assert torch.equal(a, c), step
+        if step == 0:
+            sub = full.subset(idx)
+            o = full.atom_off
+            Fsub = np.concatenate([Fref[o[q]:o[q + 1]] for q in idx])
+            _, go, _, _, parts = orc.train_loss_grad(sub, Eref[idx], Fsub, 1.0, 0.1)
+            assert np.allclose(outs[1][3].cpu().numpy(), parts, rtol=1e-6)
+            err = float(np.linalg.norm(outs[1][4].cpu().numpy() - go) / np.linalg.norm(go))
+            assert err <= 1e-5, err
+    assert np.array_equal(e1.get_params(), e2.get_params())
+
+
+def test_host_set_up_rejects_bad_species(oracle_mod):
+    import torch
+    from ann_amd import configs
+    from ann_amd.gpu import AnnError, Engine
+    model = configs.model_lips()
+    b = configs.batch_lips(2, seed=96)
+    eng = Engine(model, "cuda:0")
+    pos = torch.tensor(b.pos, device="cuda:0")
+    sp = b.species.copy()
+    sp[3] = 7
+    with pytest.raises(AnnError):
+        eng.set_batch_host(b.atom_off, pos, b.cells, sp)
+    eng.set_batch_host(b.atom_off, pos, b.cells, b.species)     # still usable
+    eng.neighbors()
+    E, F, _ = eng.energy_forces()
+    Eo, Fo, _ = oracle_mod.Oracle(model).compute_batch(b, nthreads=4)
+    assert np.max(np.abs(E.cpu().numpy() - Eo) / b.natoms) <= 1e-6
