@@ -641,7 +641,6 @@ __global__ __launch_bounds__(kScanThreads) void k_scan_small(const int* __restri
         store_host(rb, nflags, n > 0 ? tot - in[n - 1] : 0);
         __threadfence_system();
         store_host(rb, nflags + 1, seq);            // the words above are complete
-        __threadfence_system();
     }
 }
 
