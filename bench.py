@@ -529,8 +529,8 @@ def main():
         print(json.dumps({"copy_probe_bytes": a.numel() * 4}))
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (structures shard "
-              "independently; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: the library has no multi-GPU path (batches shard by structure, "
+              "each GPU running the path on its share). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
     import torch
     from ann_amd import configs
@@ -627,7 +627,7 @@ def main():
             "frac": rd["frac"], "traffic": None, "algorithmic_work_per_launch": rd["work"],
             "avg_launch_us": dom_us}
     if rd["bound"] == "valu":
-        roof["note"] = ("FP32 vector ALU; peak is the MI355X FP32 figure (packed FP32 disabled in this build, "
+        roof["note"] = ("FP32 vector ALU; peak is the MI355X FP32 figure (the kernels are compiled without packed FP32, "
                         "so the attainable non-packed VALU rate is half of it)")
     traffic_file = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(traffic_file):
