@@ -94,21 +94,38 @@ def model_ar256(seed: int = 1) -> Model:
     return Model(["Ar"], np.array([39.948]), np.array([0.0]), 6.0, CUT["COS"], radial, angular, [net])
 
 
-def water_basis(nr: int = 16):
+def model_ar_zeta5(seed: int = 5) -> Model:
+    """Ar with the angular terms of the reference's Ar example
+    (examples/Ar/basis_Ar: GAUSS, zeta 5, lambda -1, three widths) beside zeta 1
+    (lambda 1), 2 (lambda -1) and 5 (lambda 1) on the same widths, and its COS
+    cutoff; 10 radial GAUSSIAN."""
+    rng = np.random.default_rng(seed)
+    rs = np.linspace(1.0, 5.5, 10)
+    rad = (RAD["GAUSSIAN"], rs, np.full(10, 1.0 / (rs[1] - rs[0])))
+    eta = np.repeat([1.5, 2.5, 4.0], 4)
+    zeta = np.tile([1.0, 2.0, 5.0, 5.0], 3)
+    lam = np.tile([1, -1, -1, 1], 3)
+    ang = (ANG["GAUSS"], eta, zeta, lam)
+    radial, angular = _uniform_basis(1, rad, ang)
+    net = init_network(22, [24, 24], NEURON["TANH"], rng)
+    return Model(["Ar"], np.array([39.948]), np.array([0.0]), 6.0, CUT["COS"], radial, angular, [net])
+
+
+def water_basis(nr: int = 16, zmax: float = 4.0):
     """16 radial GAUSSIAN per neighbour species (rs 0.8..6.0 A, eta = 1/spacing)
-    and 16 angular GAUSS per pair: eta in {1.5,2.5,4,6} A x zeta in {1,4} x lambda +-1."""
+    and 16 angular GAUSS per pair: eta in {1.5,2.5,4,6} A x zeta in {1,zmax} x lambda +-1."""
     rs = np.linspace(0.8, 6.0, nr)
     rad = (RAD["GAUSSIAN"], rs, np.full(nr, 1.0 / (rs[1] - rs[0])))
     eta = np.repeat([1.5, 2.5, 4.0, 6.0], 4)
-    zeta = np.tile([1.0, 1.0, 4.0, 4.0], 4)
+    zeta = np.tile([1.0, 1.0, zmax, zmax], 4)
     lam = np.tile([1, -1], 8)
     ang = (ANG["GAUSS"], eta, zeta, lam)
     return rad, ang
 
 
-def model_water(seed: int = 2, neuron: str = "SOFTPLUS", hidden=(32, 32)) -> Model:
+def model_water(seed: int = 2, neuron: str = "SOFTPLUS", hidden=(32, 32), zmax: float = 4.0) -> Model:
     rng = np.random.default_rng(seed)
-    rad, ang = water_basis()
+    rad, ang = water_basis(zmax=zmax)
     radial, angular = _uniform_basis(2, rad, ang)
     nets = [init_network(80, list(hidden), NEURON[neuron], rng) for _ in range(2)]
     return Model(["H", "O"], np.array([1.008, 15.999]), np.array([0.0, 0.0]), 6.0, CUT["TANH"],

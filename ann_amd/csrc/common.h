@@ -21,7 +21,7 @@ constexpr int kMaxFunc = 32;       // functions per basis block
 constexpr int kMaxUnionEta = 4;    // distinct angular widths per centre species (fast form)
 constexpr int kGridG = 4;          // fast angular tile: widths per block
 constexpr int kGridK = 8;          // fast angular tile: (lambda, zeta) combinations per block
-constexpr int kMomZ = 4;           // largest integer zeta of the moment form
+constexpr int kMomZ = 5;           // largest integer zeta of the moment form
 constexpr int kSumChunk = 4096;    // atoms per block of the per-structure sums
 constexpr int kSumQ = 11;          // summed quantities: energy, virial[9], kinetic energy
 constexpr int kReadbackWords = 16; // pinned host words of a context (flags and counts for the host)

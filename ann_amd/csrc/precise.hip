@@ -102,14 +102,15 @@ __device__ inline double angular64(int name, double ieta2, double zeta, double l
     return pow(fabs(0.5 + lamh * c), zeta) * fr * f0 * f1;
 }
 
-// monomials x^a y^b z^c of total degree <= 4 in degree order (as the FP32
+// monomials x^a y^b z^c of total degree <= kMomZ in degree order (as the FP32
 // moment kernels number them) and their multinomial weights m!/(a! b! c!)
+constexpr int kMonoAll = (kMomZ + 1) * (kMomZ + 2) * (kMomZ + 3) / 6;
 struct MonoFixed {
-    int a[35], b[35], c[35], m[35];
-    double w[35];
+    int a[kMonoAll], b[kMonoAll], c[kMonoAll], m[kMonoAll];
+    double w[kMonoAll];
     constexpr MonoFixed() : a(), b(), c(), m(), w() {
         int q = 0;
-        for (int mm = 0; mm <= 4; ++mm)
+        for (int mm = 0; mm <= kMomZ; ++mm)
             for (int aa = mm; aa >= 0; --aa)
                 for (int bb = mm - aa; bb >= 0; --bb) {
                     const int cc = mm - aa - bb;
@@ -329,6 +330,7 @@ __global__ __launch_bounds__(64) void k_precise(PreciseArgs A, int maxn) {
     }
     if (q == q1 && !wantT) return;
     if constexpr (Z > 0) {
+        static_assert(Z <= kMomZ, "the monomial table covers the moment form's degrees");
         constexpr MonoFixed mo{};
         for (int b = 0; b < S; ++b) {
             const int s0 = seg[b], s1 = seg[b + 1];
@@ -490,7 +492,8 @@ hipError_t precise_inputs(ann_ctx* ctx, float* Gc, double* G64, float* Tout) {
         case 1: k_precise<1><<<ctx->natoms, 64, smem, ctx->stream>>>(A, maxn); break;
         case 2: k_precise<2><<<ctx->natoms, 64, smem, ctx->stream>>>(A, maxn); break;
         case 3: k_precise<3><<<ctx->natoms, 64, smem, ctx->stream>>>(A, maxn); break;
-        default: k_precise<4><<<ctx->natoms, 64, smem, ctx->stream>>>(A, maxn); break;
+        case 4: k_precise<4><<<ctx->natoms, 64, smem, ctx->stream>>>(A, maxn); break;
+        default: k_precise<5><<<ctx->natoms, 64, smem, ctx->stream>>>(A, maxn); break;
     }
     return hipGetLastError();
 }

@@ -864,7 +864,7 @@ __global__ __launch_bounds__(64) void k_fast(SymmArgs A) {
 
 // =========================================================== MOMENT form ====
 //
-// For GAUSS angular blocks with integer zeta <= 4 the triplet sum factorises
+// For GAUSS angular blocks with integer zeta <= 5 the triplet sum factorises
 // exactly.  With e_u(r) = exp(-r^2/eta_u^2) fc(r) and c = r^_j . r^_k,
 //   (1/2 + lambda c / 2)^zeta = sum_m C(zeta,m) 2^(m-zeta) (lambda/2)^m c^m,
 //   c^m = sum_{|alpha|=m} (m!/alpha!) r^_j^alpha r^_k^alpha,
@@ -880,7 +880,7 @@ __global__ __launch_bounds__(64) void k_fast(SymmArgs A) {
 // Q(r^) = sum_alpha dE/dT_b[u][alpha] r^^alpha and its gradient.  The
 // tangent (force-loss second backward) is the forward-mode derivative.
 // FP32 throughout: the alternating binomial sum amplifies rounding by at most
-// C(4,2) = 6 for zeta <= 4; larger zeta keeps the triplet kernels.
+// C(5,2) = 10 for zeta <= 5; larger zeta keeps the triplet kernels.
 
 __host__ __device__ constexpr int nmono(int Z) { return (Z + 1) * (Z + 2) * (Z + 3) / 6; }
 // neighbours per chunk of power rows (G: 32 and adjoint: 64, which frees LDS
@@ -1159,7 +1159,7 @@ __device__ __forceinline__ void moment_atom(const SymmArgs& A, const int i, floa
     };
     wave_sync();
     // moments T_b[u][alpha]: lane = (alpha, group); groups stride the neighbours.
-    // G with zeta up to 4 (35 monomials, one group would leave 29 lanes idle):
+    // G with zeta 4 (35 monomials, one group would leave 29 lanes idle):
     // monomials 1, x, y, z are summed with the diagonal sums below and the other
     // 31 run as two groups of 32 lanes, joined by one cross-half exchange.
     constexpr bool SPLIT = GM == 0 && NM == 35;
@@ -1673,11 +1673,13 @@ static hipError_t run(ann_ctx* ctx, int form, SymmArgs A) {
         }
         void (*kern)(SymmArgs) =
             queued ? (M.moment_z == 1 ? k_moment_queue<MODE, 1> : M.moment_z == 2 ? k_moment_queue<MODE, 2>
-                      : M.moment_z == 3 ? k_moment_queue<MODE, 3> : k_moment_queue<MODE, 4>)
+                      : M.moment_z == 3 ? k_moment_queue<MODE, 3> : M.moment_z == 4 ? k_moment_queue<MODE, 4>
+                      : k_moment_queue<MODE, 5>)
                    : MODE == 0 ? (M.moment_z == 1 ? k_moment_g<1> : M.moment_z == 2 ? k_moment_g<2>
-                                  : M.moment_z == 3 ? k_moment_g<3> : k_moment_g<4>)
+                                  : M.moment_z == 3 ? k_moment_g<3> : M.moment_z == 4 ? k_moment_g<4> : k_moment_g<5>)
                    : (M.moment_z == 1 ? k_moment<MODE, 1> : M.moment_z == 2 ? k_moment<MODE, 2>
-                      : M.moment_z == 3 ? k_moment<MODE, 3> : k_moment<MODE, 4>);
+                      : M.moment_z == 3 ? k_moment<MODE, 3> : M.moment_z == 4 ? k_moment<MODE, 4>
+                      : k_moment<MODE, 5>);
         if (queued) {
             // persistent: enough workgroups for the most wavefronts a CU can hold
             // (32); those that find no room start later, find the queues empty
@@ -1760,7 +1762,8 @@ hipError_t symm_fused(ann_ctx* ctx, const float* inpb, float* Eatom, float* P) {
     A.smem_wave = (int)sw;
     const unsigned grid = (unsigned)((A.N + apb - 1) / apb), block = 64u * apb;
     void (*kern)(SymmArgs) = M.moment_z == 1 ? k_moment<3, 1> : M.moment_z == 2 ? k_moment<3, 2>
-                             : M.moment_z == 3 ? k_moment<3, 3> : k_moment<3, 4>;
+                             : M.moment_z == 3 ? k_moment<3, 3> : M.moment_z == 4 ? k_moment<3, 4>
+                             : k_moment<3, 5>;
     kern<<<grid, block, sw * apb, ctx->stream>>>(A);
     ctx->T_gen = ~0ull;                                     // no stored moments
     return hipGetLastError();

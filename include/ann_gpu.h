@@ -43,7 +43,7 @@ enum {
     ANN_FORM_PLAIN = 0,  /* FP32, reference-shaped loops: per-triplet function
                             evaluation, LDS-atomic accumulation, FP32 GEMV network */
     ANN_FORM_FAST = 1,   /* angular sums as per-neighbour tensor moments where every
-                            angular block is GAUSS with integer zeta <= 4 (exact
+                            angular block is GAUSS with integer zeta <= 5 (exact
                             factorisation, O(n) per atom), otherwise species-blocked
                             triplet tiles (GAUSS: pair-separable exponentials; other
                             families: a distance factor per pair of legs); atomic-free
