@@ -741,6 +741,7 @@ extern "C" int ann_set_batch_host(ann_ctx* ctx, int nstruc, const int* atom_off,
     const int rc = ann_set_batch(ctx, &b);
     if (rc) return rc;
     ctx->h_cells.assign(cells, cells + 9 * (size_t)nstruc);
+    ctx->h_species.assign(species, species + N);
     ctx->h_species_count = count;
     ctx->h_cells_valid = true;
     ctx->h_counts_valid = true;

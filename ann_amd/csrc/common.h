@@ -208,6 +208,8 @@ struct ann_ctx {
     // the species tiles are then set up without reading the device
     std::vector<double> h_cells;      // [S][9]
     std::vector<int> h_species_count; // atoms per species
+    std::vector<int> h_species;       // [N]
+    std::vector<int> h_rank;          // [N] rank of each atom within its species (from h_species)
     bool h_cells_valid = false, h_counts_valid = false;
     // pinned staging of the set-up uploads, one slot per upload site (see
     // staged_upload); a slot is rewritten only after its previous copy ended
@@ -217,7 +219,7 @@ struct ann_ctx {
         hipEvent_t ev = nullptr;
         bool pending = false;
     };
-    Staged staged[12];                // kNStaged slots
+    Staged staged[16];                // kNStaged slots
     // the batch arrays of ann_set_batch_host (context-owned device copies)
     int* d_off_own = nullptr;
     double* d_cells_own = nullptr;
@@ -408,8 +410,8 @@ int set_neighbors_host(ann_ctx* ctx, const int* off, const int* idx, const int* 
 void pair_free(ann_ctx* ctx);
 // Upload slots of staged_upload
 enum { kStChunks, kStChunkOff, kStCellInfo, kStRinv, kStBinStruc, kStTiles, kStSpend, kStItems, kStQoff,
-       kStAtomOff, kStCells, kStSpecies, kNStaged };
-static_assert(kNStaged <= 12, "one staging slot per upload site");
+       kStAtomOff, kStCells, kStSpecies, kStRank, kStSpCount, kStPerm, kNStaged };
+static_assert(kNStaged <= 16, "one staging slot per upload site");
 // Host-to-device copy of bytes through the context's pinned staging slot, on
 // the context's stream, without waiting for the copy: the host data may be
 // released on return (it is copied into the slot first; the slot's previous

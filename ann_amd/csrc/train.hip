@@ -632,9 +632,21 @@ hipError_t species_ranks(ann_ctx* ctx) {
     if ((e = grow_ints(ctx->d_rkpre, c3, N + 1)) != hipSuccess) return e;
     ctx->rank_cap = std::min(c1, std::min(c2, c3));
     if (!ctx->d_spcount && (e = hipMalloc(&ctx->d_spcount, sizeof(int) * kMaxSpecies)) != hipSuccess) return e;
+    if (ctx->h_counts_valid) {
+        // species known on the host (ann_set_batch_host): ranks in atom order,
+        // as the device scans give them, and the counts, uploaded without kernels
+        ctx->h_rank.resize(N);
+        std::vector<int> next(ctx->S, 0);
+        for (int i = 0; i < N; ++i) ctx->h_rank[i] = next[ctx->h_species[i]]++;
+        if ((e = staged_upload(ctx, kStRank, ctx->d_rank, ctx->h_rank.data(), sizeof(int) * (size_t)N)) != hipSuccess)
+            return e;
+        if ((e = staged_upload(ctx, kStSpCount, ctx->d_spcount, ctx->h_species_count.data(), sizeof(int) * ctx->S)) !=
+            hipSuccess)
+            return e;
+    }
     int* flag = ctx->d_rkflag;
     int* pre = ctx->d_rkpre;
-    for (int s = 0; s < ctx->S; ++s) {
+    for (int s = 0; s < (ctx->h_counts_valid ? 0 : ctx->S); ++s) {
         k_flags<<<(N + 256) / 256, 256, 0, ctx->stream>>>(N, ctx->d_species, s, flag);
         if ((e = exclusive_scan(flag, pre, N + 1, ctx->stream)) != hipSuccess) return e;
         k_take_rank<<<(N + 256) / 256, 256, 0, ctx->stream>>>(N, ctx->d_species, s, pre, ctx->d_rank, ctx->d_spcount);
@@ -691,8 +703,14 @@ hipError_t species_perm(ann_ctx* ctx) {
     if ((e = staged_upload(ctx, kStSpend, ctx->d_spend, be.data(), sizeof(int) * 2 * S)) != hipSuccess) return e;
     if ((e = staged_upload(ctx, kStTiles, ctx->d_tiles, tiles.data(), sizeof(int) * tiles.size())) != hipSuccess)
         return e;
-    k_perm<<<(N + 255) / 256, 256, 0, ctx->stream>>>(N, ctx->d_species, ctx->d_rank, ctx->d_spend, ctx->d_perm);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (ctx->h_counts_valid) {
+        std::vector<int> perm(N);
+        for (int i = 0; i < N; ++i) perm[base[ctx->h_species[i]] + ctx->h_rank[i]] = i;
+        if ((e = staged_upload(ctx, kStPerm, ctx->d_perm, perm.data(), sizeof(int) * (size_t)N)) != hipSuccess) return e;
+    } else {
+        k_perm<<<(N + 255) / 256, 256, 0, ctx->stream>>>(N, ctx->d_species, ctx->d_rank, ctx->d_spend, ctx->d_perm);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
     // spend[] must hold the END of each species block: shift the uploaded (base, end) pair
     ctx->d_spend_end = ctx->d_spend + S;
     ctx->ntiles = (int)tiles.size() / 2;
