@@ -978,6 +978,8 @@ static hipError_t ef_body(ann_ctx* ctx, int form, int with_evac, double* E, doub
         if ((e = symm_fused(ctx, ctx->da.inpb_net, ctx->d_Eatom, ctx->d_P)) != hipSuccess) return e;
         return force_reduce(ctx, ctx->d_P, ctx->d_Eatom, with_evac, F, E, V);
     }
+    // the network's species tiles before G: their host set-up overlaps the G kernel
+    if (form != ANN_FORM_PLAIN && (e = species_perm(ctx)) != hipSuccess) return e;
     if ((e = network_rows(ctx, form, ctx->d_G)) != hipSuccess) return e;
     if ((e = net_fwd(ctx, form, ctx->d_G, ctx->d_Eatom, F || V ? ctx->d_dEdG : nullptr, true)) != hipSuccess) return e;
     if (F || V) {

@@ -290,11 +290,13 @@ struct ann_ctx {
     int64_t work_atoms = 0, work_pairs = 0;
     int* d_adam_t = nullptr;          // Adam step counter (device, so a captured step can advance it)
     // tuning (set by ann_autotune or ann_set_tuning)
-    int tune_apb[3] = {1, 1, 1};      // atoms (wavefronts) per workgroup of the moment kernels, per mode
+    int tune_apb[3] = {4, 4, 1};      // atoms (wavefronts) per workgroup of the moment kernels, per mode
+                                      // (the autotuner's usual choice; clamped to 64 KiB of LDS per workgroup)
     int tune_angular = 0;             // 0: moment form where it applies, 1: triplet tiles
     int tune_sb = 0;                  // neighbour search blocks: 0 heuristic (2x2x2 bins where they fit), 1 single bins
     int tune_queue = 0;               // moment kernels: 0 grid dispatch, 1 persistent work queues
-    int tune_net = 0;                 // networks (TRIPLET, FAST): 0 vector-ALU tiles where they fit, 1 matrix cores
+    int tune_net = 1;                 // networks (TRIPLET, FAST): 0 vector-ALU tiles where they fit, 1 matrix cores
+                                      // (the default: faster on every measured workload)
     int tune_fused = 0;               // FAST E+F in the moment form: 0 separate G / network / adjoint kernels, 1 one fused kernel
     // persistent dispatch: (first atom, count) items per XCD queue, built on first use after ann_set_batch
     int2* d_items = nullptr;

@@ -926,6 +926,9 @@ hipError_t train_step_impl(ann_ctx* ctx, int form, const double* Eref, const dou
                            double gamma, double* loss, double* grad) {
     hipError_t e;
     const int B = ctx->nstruc, N = ctx->natoms;
+    // the network's species tiles first: their host set-up then overlaps the
+    // G kernel instead of holding the network kernel back
+    if (form != ANN_FORM_PLAIN && (e = species_perm(ctx)) != hipSuccess) return e;
     if (!ctx->G_valid) {
         if ((e = network_rows(ctx, form, ctx->d_G)) != hipSuccess) return e;
         ctx->G_valid = true;

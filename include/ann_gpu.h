@@ -305,7 +305,8 @@ int ann_input_stats(ann_ctx* ctx, const float* d_G, int* count, double* stats, i
  * 1 triplet); search blocks (0 heuristic, 1 single bins); moment-kernel
  * dispatch (0 grid, 1 persistent queues); network kernel (0 vector ALU where
  * the species' weights fit its 64 KiB of LDS, else matrix cores; 1 matrix
- * cores); E+F kernels (0 separate, 1 fused where it applies).  No counterpart
+ * cores, the default); E+F kernels (0 separate, 1 fused where it applies).
+ * Without ann_autotune the defaults are 4, 4, 1, 0, 0, 0, 1, 0.  No counterpart
  * in the reference. */
 int ann_autotune(ann_ctx* ctx, int form);
 int ann_get_tuning(ann_ctx* ctx, int* tuning, int n);

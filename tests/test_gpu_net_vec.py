@@ -16,10 +16,13 @@ pytestmark = pytest.mark.gpu
 
 
 def _engine(model, batch):
+    """An engine with the vector-ALU network selected (tuning entry 7 = 0; the
+    default is the matrix-core kernel)."""
     from ann_amd.gpu import Engine
     eng = Engine(model, "cuda:0")
     eng.set_batch(batch)
     eng.neighbors()
+    eng.set_tuning(eng.tuning()[:6] + [0])
     return eng
 
 

@@ -270,8 +270,8 @@ def test_profile_only_times_one_family(cases):
     eng.profile_only("symm_fwd_mom")
     eng.energy_forces(FAST)
     assert eng.profile_get("symm_fwd_mom")[1] == 1
-    assert eng.profile_get("net_vec")[1] == 0
+    assert eng.profile_get("net_mfma")[1] == 0
     eng.profile_only("")
     eng.energy_forces(FAST)
-    assert eng.profile_get("net_vec")[1] == 1
+    assert eng.profile_get("net_mfma")[1] == 1
     eng.profile(False)
