@@ -78,9 +78,10 @@ class ResidentSet:
         self.wE = None if wE is None else torch.as_tensor(np.asarray(wE, np.float64), device=dev)
         self.wF = None if wF is None else torch.as_tensor(np.asarray(wF, np.float64), device=dev)
 
-    def gather(self, idx: np.ndarray):
+    def gather(self, idx: np.ndarray, device_meta: bool = True):
         """Contiguous batch arrays of structures idx (host atom offsets; device
-        positions, cells, species, energies, forces and weights)."""
+        positions, energies, forces and weights; device cells and species unless
+        device_meta is false, as Engine.set_batch_host takes the host copies)."""
         import torch
         idx = np.asarray(idx, dtype=np.int64)
         n = self.natoms[idx]
@@ -90,8 +91,9 @@ class ResidentSet:
         ai = torch.from_numpy(atoms).to(self.device, non_blocking=True)
         si = torch.from_numpy(idx).to(self.device, non_blocking=True)
         out = {"atom_off": atom_off,
-               "pos": self.pos.index_select(0, ai), "cells": self.cells.index_select(0, si),
-               "species": self.species.index_select(0, ai),
+               "pos": self.pos.index_select(0, ai),
+               "cells": self.cells.index_select(0, si) if device_meta else None,
+               "species": self.species.index_select(0, ai) if device_meta else None,
                "energy": self.energy.index_select(0, si), "forces": self.forces.index_select(0, ai),
                # host copies for Engine.set_batch_host (no device readback in the set-up)
                "cells_host": self.batch.cells[idx], "species_host": self.batch.species[atoms]}
@@ -147,7 +149,7 @@ def validation_error(eng, rs: ResidentSet, idx, loss: str = "MSE", delta: float 
         raise ValueError("NNPTEFR::train(int): No validation data provided.")
     lsum, rsum = 0.0, 0.0
     for c0 in range(0, idx.size, chunk):
-        b = rs.gather(idx[c0:c0 + chunk])
+        b = rs.gather(idx[c0:c0 + chunk], device_meta=False)
         eng.set_batch_host(b["atom_off"], b["pos"], b["cells_host"], b["species_host"])
         eng.neighbors()
         E = eng.energies(form, with_evac=False).cpu().numpy()

@@ -458,7 +458,7 @@ def run_rotating(steps, warmup, nset=512, size=128):
            torch.empty(eng.nparams, dtype=torch.float64, device=eng.device))
 
     def step():
-        b = rs.gather(sampler.next())
+        b = rs.gather(sampler.next(), device_meta=False)
         eng.set_batch_host(b["atom_off"], b["pos"], b["cells_host"], b["species_host"])
         eng.neighbors()
         eng.train_step(b["energy"], b["forces"], 1.0, 0.1, 1e-3, FAST, out=out)
